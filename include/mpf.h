@@ -1,0 +1,169 @@
+/*
+ * mpf.h -- C ABI of the MI355X multi-phase-field solidification solver.
+ *
+ * Two groups of entry points:
+ *
+ * 1. The resident solver (handle API).  The library owns the device state in
+ *    its own compact layout (dense liquid + per-cell grain slots) and advances
+ *    it with one fused HIP kernel per step.  It replaces the reference's host
+ *    time loop with its three launches per step
+ *      run_randommode_3d.py:267-287   (temp -> APT -> update -> swap)
+ *      tests/verification_3d.py:830-905 (same, with the checked APT and a
+ *                                        KMaxOverflowError on overflow)
+ *    Inputs and outputs cross the boundary in the reference's dense layouts
+ *    (phi[N][nx][ny][nz], mf[depth][nx][ny][nz], nf[nx][ny][nz]).
+ *
+ * 2. Drop-in kernels on caller-owned device arrays in the reference's dense
+ *    layout, one per reference kernel (src/gpu_kernels_3d.py,
+ *    src/gpu_kernels.py).
+ *
+ * Conventions: every function returns an int; 0 (or a non-negative count) is
+ * success, negative values are errors (MPF_E*), and mpf_last_error() returns a
+ * message for the calling thread.  No exceptions cross the ABI.  A handle is
+ * not thread-safe and owns one HIP stream; handle calls are synchronous.
+ */
+#ifndef MPF_H
+#define MPF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPF_ABI_VERSION 1
+
+enum { MPF_FP32 = 0, MPF_FP64 = 1 };
+enum {
+    MPF_OK = 0,
+    MPF_EINVAL = -1,   /* bad argument or configuration */
+    MPF_EHIP = -2,     /* HIP runtime error */
+    MPF_ENOMEM = -3,   /* device allocation failed */
+    MPF_ECAPACITY = -4 /* a cell holds more grains than the slot capacity */
+};
+
+typedef struct mpf_handle mpf_handle;
+
+/* Scalars of the update kernel, in the order of its signature
+ * (src/gpu_kernels_3d.py:531-537, :684-692). */
+typedef struct mpf_scalars {
+    double dx, dt, T_melt, Sf, eps0_sl, w0_sl;
+    double a0, delta_a, mu_a, p_round, g2_floor, ksi, theta_c_rad;
+} mpf_scalars;
+
+typedef struct mpf_config {
+    int nx, ny, nz;        /* grid; 2D uses nz = 1 */
+    int dim;               /* 3: x,y periodic, z mirror; 2: x periodic, y mirror */
+    int n_phase;           /* N = n_solid + 1; phase 0 is the liquid */
+    int kmax;              /* KMAX, 1..64 (src/gpu_kernels_3d.py:20) */
+    int precision;         /* MPF_FP32 or MPF_FP64 */
+    int temp_dense;        /* 0: temperature is uniform in x,y (stored per z); 1: per cell */
+    int temp_f32;          /* 1: the temperature is a float32 array (tests/verification_3d.py:817) */
+    int enable_anisotropy; /* switchable kernel toggles (src/gpu_kernels_3d.py:692) */
+    int enable_torque;
+    int halt_on_kmax;      /* stop at the first step whose APT exceeds KMAX */
+    int slot_capacity;     /* grains stored per cell; 0 = min(KMAX, N-1).  Steps never
+                              store more than KMAX; larger values only admit uploads
+                              with more non-zero grains per cell. */
+    mpf_scalars s;
+    double cooling_rate;   /* subtracted from the temperature before every update */
+} mpf_config;
+
+typedef struct mpf_stats {
+    int max_active;                 /* largest active-phase count seen (status[1]) */
+    int kmax_overflow;              /* 1 if some cell needed more than KMAX (status[0]) */
+    unsigned long long overflow_cells;
+    unsigned long long general_cells; /* cells that took the many-phase path */
+    int halt_step;                  /* -1, or the step index that overflowed */
+    int steps_done;                 /* total steps advanced by this handle */
+    double last_step_ms;            /* GPU time of the last mpf_step call (hipEvents) */
+} mpf_stats;
+
+/* ---- resident solver ---------------------------------------------------- */
+
+/* Replaces the to_device allocations of run_randommode_3d.py:244-252. */
+int mpf_create(const mpf_config *cfg, mpf_handle **out);
+void mpf_destroy(mpf_handle *h);
+
+/* wij, aij, mij: [N][N]; n111: [N][8][3] (src/seed_modes.py:333-380,
+ * src/orientation_utils.py:182-212).  Converted to the solver precision. */
+int mpf_set_tables(mpf_handle *h, const double *wij, const double *aij, const double *mij,
+                   const double *n111);
+
+/* phi in the reference's dense layout [N][nx][ny][nz] (2D: [N][nx][ny]);
+ * dtype MPF_FP32 or MPF_FP64 (run_randommode_3d.py:244). */
+int mpf_upload_phi(mpf_handle *h, const void *phi, int dtype);
+
+/* Columnar initial state without a dense array: cell (l,m,k) holds grain
+ * grain_map[l][m] with value solid[profile[l][m]][k] and liquid
+ * liquid[profile[l][m]][k] (src/seed_modes_3d.py:38-57, :117-166). */
+int mpf_init_columns(mpf_handle *h, const int32_t *grain_map, const int32_t *profile, int nprofiles,
+                     const double *solid, const double *liquid);
+
+/* Temperature: [nz] values when temp_dense == 0, else [nx][ny][nz]. */
+int mpf_upload_temp(mpf_handle *h, const double *temp);
+int mpf_download_temp(mpf_handle *h, double *temp);
+
+/* Advance nsteps steps.  Returns the number of steps completed (< nsteps
+ * only when halt_on_kmax stopped at an overflowing step; the state is then the
+ * one that step started from) or a negative error. */
+int mpf_step(mpf_handle *h, int nsteps);
+
+int mpf_download_phi(mpf_handle *h, void *phi, int dtype);
+
+/* Checked APT of the current state in the reference layout
+ * (kernel_update_nfmf_3d_checked, src/gpu_kernels_3d.py:489-527):
+ * mf[depth][nx][ny][nz], nf[nx][ny][nz], status[2]. */
+int mpf_download_apt(mpf_handle *h, int32_t *mf, int depth, int32_t *nf, int32_t *status);
+
+int mpf_get_stats(mpf_handle *h, mpf_stats *st);
+int mpf_reset_stats(mpf_handle *h);
+
+/* FP64 observables of the current state (tests/verification_3d.py:409-416,
+ * :612-654): grain_volume[N] = sum of phi_g over cells (times dx^3 by the
+ * caller), front[nx*ny] = topmost k with 1 - phi_L > 0.5 (or -1), and
+ * out[0..5] = max|sum phi - 1|, min phi, max phi, NaN count, solid sum, cells. */
+int mpf_observables(mpf_handle *h, double *grain_volume, int32_t *front, double *out);
+
+/* Bytes of device memory held by the handle. */
+size_t mpf_device_bytes(mpf_handle *h);
+
+/* ---- drop-in kernels on the reference's dense layout ---------------------
+ * All pointers are device pointers; stream may be NULL (default stream). */
+
+/* kernel_update_nfmf_3d (checked = 0, src/gpu_kernels_3d.py:462-486) and
+ * kernel_update_nfmf_3d_checked (checked = 1, :489-527); dim 2 is
+ * kernel_update_nfmf (src/gpu_kernels.py:337-356).  phi dtype MPF_FP32/64. */
+int mpf_dense_apt(const void *phi, int dtype, int32_t *mf, int32_t *nf, int32_t *status, int n_phase,
+                  int nx, int ny, int nz, int dim, int depth, int kmax, int checked, void *stream);
+
+/* kernel_update_phasefield_active_3d(_switchable) (src/gpu_kernels_3d.py:530-830)
+ * and kernel_update_phasefield_active (src/gpu_kernels.py:361-515).  phi,
+ * phi_new and the tables share dtype; temp has temp_dtype; mf has depth rows. */
+int mpf_dense_update(const void *phi, void *phi_new, int dtype, const void *temp, int temp_dtype,
+                     const int32_t *mf, const int32_t *nf, const void *wij, const void *aij,
+                     const void *mij, const void *n111, const mpf_scalars *s, int n_phase, int nx, int ny,
+                     int nz, int dim, int depth, int kmax, int enable_anisotropy, int enable_torque,
+                     void *stream);
+
+/* kernel_update_temp_3d (src/gpu_kernels_3d.py:833-838) / kernel_update_temp
+ * (src/gpu_kernels.py:520-525): temp[i] -= cooling_rate over n values. */
+int mpf_dense_temp(void *temp, int temp_dtype, double cooling_rate, size_t n, void *stream);
+
+/* ---- device memory for callers without another allocator ---------------- */
+int mpf_malloc(void **ptr, size_t bytes);
+int mpf_free(void *ptr);
+int mpf_memcpy_htod(void *dst, const void *src, size_t bytes);
+int mpf_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int mpf_memset(void *ptr, int value, size_t bytes);
+int mpf_device_synchronize(void);
+
+const char *mpf_last_error(void);
+int mpf_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPF_H */

@@ -1,0 +1,694 @@
+// mpf_api.cpp -- host side of the C ABI declared in include/mpf.h.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+#include "mpf.h"
+#include "mpf_math.h"
+#include "mpf_state.h"
+
+using namespace mpf;
+
+// ------------------------------------------------ launchers of both builds
+#define MPF_DECLARE_LAUNCHERS(R, SFX)                                                                    \
+    extern "C" hipError_t mpf_launch_step##SFX(const StepArgs<R> *, int, int, hipStream_t);              \
+    extern "C" hipError_t mpf_launch_apt_slots##SFX(const Consts<R> *, SlotView<R>, int, int32_t *,      \
+                                                    int32_t *, int32_t *, int, hipStream_t);             \
+    extern "C" hipError_t mpf_launch_pack##SFX(const void *, int, int, int, int, int, SlotView<R>, int,  \
+                                               int *, hipStream_t);                                      \
+    extern "C" hipError_t mpf_launch_unpack##SFX(SlotView<R>, int, int, int, void *, int, hipStream_t);  \
+    extern "C" hipError_t mpf_launch_init_columns##SFX(const int32_t *, const int32_t *, const double *, \
+                                                       const double *, int, int, int, SlotView<R>,      \
+                                                       hipStream_t);                                     \
+    extern "C" hipError_t mpf_launch_temp_layout##SFX(const double *, double *, int, int, int, int, int, \
+                                                      hipStream_t);                                      \
+    extern "C" hipError_t mpf_launch_dense_apt##SFX(const R *, int32_t *, int32_t *, int32_t *, int,     \
+                                                    int, int, int, int, int, int, int, hipStream_t);     \
+    extern "C" hipError_t mpf_launch_dense_update##SFX(const Consts<R> *, int, const R *, R *,           \
+                                                       const void *, int, const int32_t *,               \
+                                                       const int32_t *, int, hipStream_t);               \
+    extern "C" hipError_t mpf_launch_dense_temp##SFX(void *, int, double, size_t, hipStream_t);          \
+    extern "C" hipError_t mpf_launch_observe##SFX(SlotView<R>, int, int, int, int, double *, double *,   \
+                                                  int32_t *, hipStream_t);
+
+MPF_DECLARE_LAUNCHERS(double, _f64)
+MPF_DECLARE_LAUNCHERS(float, _f32)
+
+// ------------------------------------------------------------------ errors
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                           \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return fail(MPF_EHIP, "%s: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+extern "C" const char *mpf_last_error(void) { return g_err; }
+extern "C" int mpf_abi_version(void) { return MPF_ABI_VERSION; }
+
+// ------------------------------------------------------------------ handle
+struct Buffers {
+    void *liq = nullptr;
+    uint8_t *cnt = nullptr;
+    uint16_t *sid = nullptr;
+    void *sval = nullptr;
+};
+
+struct mpf_handle {
+    mpf_config cfg;
+    int dim = 3;
+    int nz = 1;
+    size_t ncell = 0;
+    int scap = 1;
+    size_t rsz = 8; // sizeof(Real)
+    hipStream_t stream = nullptr;
+    Buffers b[2];
+    double *T[2] = {nullptr, nullptr};
+    size_t nT = 0;
+    void *tab = nullptr; // wij, aij, mij, n111 contiguous
+    int *worklist = nullptr;
+    int *wl_count = nullptr;
+    Stats *stats = nullptr;
+    int cur = 0, tcur = 0;
+    int step_counter = 0;
+    int steps_done = 0;
+    int tables_set = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    size_t bytes = 0;
+};
+
+template <typename R>
+static SlotView<R> view(mpf_handle *h, int i)
+{
+    SlotView<R> v;
+    v.liq = (R *)h->b[i].liq;
+    v.cnt = h->b[i].cnt;
+    v.sid = h->b[i].sid;
+    v.sval = (R *)h->b[i].sval;
+    return v;
+}
+
+template <typename R>
+static Consts<R> make_consts(const mpf_scalars &s, int nx, int ny, int nz, int N, int kmax, int aniso,
+                             int torque, const void *tab)
+{
+    Consts<R> K;
+    K.nx = nx;
+    K.ny = ny;
+    K.nz = nz;
+    K.N = N;
+    K.kmax = kmax;
+    K.aniso = aniso;
+    K.torque = torque;
+    // Derived constants are formed exactly as the reference writes them
+    // inline (2.0*dx, 1.0/(dx*dx), 0.25*inv_dx2, 0.5/dx, ...).
+    double inv_dx2 = 1.0 / (s.dx * s.dx);
+    K.dx = (R)s.dx;
+    K.dt = (R)s.dt;
+    K.two_dx = (R)(2.0 * s.dx);
+    K.inv_dx2 = (R)inv_dx2;
+    K.q_inv_dx2 = (R)(0.25 * inv_dx2);
+    K.half_over_dx = (R)(0.5 / s.dx);
+    K.eps0_sl = (R)s.eps0_sl;
+    K.eps0_sq = (R)(s.eps0_sl * s.eps0_sl);
+    K.w0_sl = (R)s.w0_sl;
+    K.tan_a0 = (R)mpf_tan(s.a0);
+    K.delta_a = (R)s.delta_a;
+    K.mu_a = (R)s.mu_a;
+    K.mu_delta = (R)(s.mu_a * s.delta_a);
+    K.p2 = (R)(s.p_round * s.p_round);
+    K.g2_floor = (R)s.g2_floor;
+    K.ksi = (R)s.ksi;
+    K.theta_c = (R)s.theta_c_rad;
+    K.Sf = s.Sf;
+    K.T_melt = s.T_melt;
+    const R *t = (const R *)tab;
+    size_t nn = (size_t)N * N;
+    K.wij = t;
+    K.aij = t + nn;
+    K.mij = t + 2 * nn;
+    K.n111 = t + 3 * nn;
+    return K;
+}
+
+static int dev_alloc(mpf_handle *h, void **p, size_t bytes)
+{
+    if (bytes == 0) bytes = 1;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(MPF_ENOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    }
+    h->bytes += bytes;
+    return 0;
+}
+
+extern "C" void mpf_destroy(mpf_handle *h)
+{
+    if (!h) return;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (int i = 0; i < 2; ++i) {
+        hipFree(h->b[i].liq);
+        hipFree(h->b[i].cnt);
+        hipFree(h->b[i].sid);
+        hipFree(h->b[i].sval);
+        hipFree(h->T[i]);
+    }
+    hipFree(h->tab);
+    hipFree(h->worklist);
+    hipFree(h->wl_count);
+    hipFree(h->stats);
+    if (h->ev0) hipEventDestroy(h->ev0);
+    if (h->ev1) hipEventDestroy(h->ev1);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+}
+
+static int reset_stats_dev(mpf_handle *h)
+{
+    Stats s;
+    memset(&s, 0, sizeof(s));
+    s.halt_step = -1;
+    HIP_TRY(hipMemcpyAsync(h->stats, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemsetAsync(h->wl_count, 0, 2 * sizeof(int), h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int mpf_create(const mpf_config *cfg, mpf_handle **out)
+{
+    if (!cfg || !out) return fail(MPF_EINVAL, "mpf_create: null argument");
+    *out = nullptr;
+    const mpf_config &c = *cfg;
+    if (c.dim != 2 && c.dim != 3) return fail(MPF_EINVAL, "dim must be 2 or 3 (got %d)", c.dim);
+    if (c.nx < 1 || c.ny < 1 || c.nz < 1) return fail(MPF_EINVAL, "grid must be positive");
+    if (c.dim == 2 && c.nz != 1) return fail(MPF_EINVAL, "2D grids need nz == 1");
+    if (c.n_phase < 1 || c.n_phase > 65535) return fail(MPF_EINVAL, "n_phase must be in [1, 65535]");
+    if (c.kmax < 1 || c.kmax > KCAP) return fail(MPF_EINVAL, "kmax must be in [1, %d]", KCAP);
+    if (c.precision != MPF_FP32 && c.precision != MPF_FP64) return fail(MPF_EINVAL, "bad precision");
+    size_t ncell = (size_t)c.nx * c.ny * c.nz;
+    if (ncell > (size_t)0x7fffffff) return fail(MPF_EINVAL, "grid has more than 2^31-1 cells");
+    if (c.dim == 2 && c.n_phase < 2) return fail(MPF_EINVAL, "the 2D update needs n_phase >= 2");
+
+    mpf_handle *h = new (std::nothrow) mpf_handle();
+    if (!h) return fail(MPF_ENOMEM, "host allocation failed");
+    h->cfg = c;
+    h->dim = c.dim;
+    h->nz = c.nz;
+    h->ncell = ncell;
+    h->rsz = c.precision == MPF_FP64 ? 8 : 4;
+    int cap = c.kmax < c.n_phase - 1 ? c.kmax : c.n_phase - 1;
+    if (c.slot_capacity > 0) {
+        // a step may store up to min(KMAX, N-1) grains per cell
+        if (c.slot_capacity < cap) {
+            delete h;
+            return fail(MPF_EINVAL, "slot_capacity must be at least min(kmax, n_phase-1) = %d", cap);
+        }
+        cap = c.slot_capacity;
+    }
+    if (cap > 255) {
+        delete h;
+        return fail(MPF_EINVAL, "slot_capacity must be at most 255");
+    }
+    h->scap = cap < 1 ? 1 : cap;
+    int rc = 0;
+#define TRY_RC(x)                       \
+    do {                                \
+        if ((rc = (x)) != 0) {          \
+            mpf_destroy(h);             \
+            return rc;                  \
+        }                               \
+    } while (0)
+    hipError_t e = hipStreamCreate(&h->stream);
+    if (e != hipSuccess) {
+        delete h;
+        return fail(MPF_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    for (int i = 0; i < 2; ++i) {
+        TRY_RC(dev_alloc(h, &h->b[i].liq, ncell * h->rsz));
+        TRY_RC(dev_alloc(h, (void **)&h->b[i].cnt, ncell));
+        TRY_RC(dev_alloc(h, (void **)&h->b[i].sid, ncell * h->scap * sizeof(uint16_t)));
+        TRY_RC(dev_alloc(h, &h->b[i].sval, ncell * h->scap * h->rsz));
+    }
+    h->nT = c.temp_dense ? ncell : (size_t)c.nz;
+    for (int i = 0; i < 2; ++i) TRY_RC(dev_alloc(h, (void **)&h->T[i], h->nT * sizeof(double)));
+    size_t nn = (size_t)c.n_phase * c.n_phase;
+    TRY_RC(dev_alloc(h, &h->tab, (3 * nn + (size_t)c.n_phase * 24) * h->rsz));
+    TRY_RC(dev_alloc(h, (void **)&h->worklist, ncell * sizeof(int)));
+    TRY_RC(dev_alloc(h, (void **)&h->wl_count, 2 * sizeof(int)));
+    TRY_RC(dev_alloc(h, (void **)&h->stats, sizeof(Stats)));
+    // empty state: all liquid, no grains
+    for (int i = 0; i < 2; ++i) {
+        e = hipMemsetAsync(h->b[i].cnt, 0, ncell, h->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(h->b[i].liq, 0, ncell * h->rsz, h->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(h->T[i], 0, h->nT * sizeof(double), h->stream);
+        if (e != hipSuccess) {
+            mpf_destroy(h);
+            return fail(MPF_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+        }
+    }
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+        mpf_destroy(h);
+        return fail(MPF_EHIP, "hipEventCreate failed");
+    }
+    TRY_RC(reset_stats_dev(h));
+#undef TRY_RC
+    *out = h;
+    return 0;
+}
+
+extern "C" size_t mpf_device_bytes(mpf_handle *h) { return h ? h->bytes : 0; }
+
+extern "C" int mpf_set_tables(mpf_handle *h, const double *wij, const double *aij, const double *mij,
+                              const double *n111)
+{
+    if (!h || !wij || !aij || !mij || !n111) return fail(MPF_EINVAL, "mpf_set_tables: null argument");
+    int N = h->cfg.n_phase;
+    size_t nn = (size_t)N * N, tot = 3 * nn + (size_t)N * 24;
+    std::vector<double> all(tot);
+    memcpy(all.data(), wij, nn * sizeof(double));
+    memcpy(all.data() + nn, aij, nn * sizeof(double));
+    memcpy(all.data() + 2 * nn, mij, nn * sizeof(double));
+    memcpy(all.data() + 3 * nn, n111, (size_t)N * 24 * sizeof(double));
+    if (h->rsz == 8) {
+        HIP_TRY(hipMemcpy(h->tab, all.data(), tot * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(tot);
+        for (size_t i = 0; i < tot; ++i) f[i] = (float)all[i];
+        HIP_TRY(hipMemcpy(h->tab, f.data(), tot * sizeof(float), hipMemcpyHostToDevice));
+    }
+    h->tables_set = 1;
+    return 0;
+}
+
+extern "C" int mpf_upload_phi(mpf_handle *h, const void *phi, int dtype)
+{
+    if (!h || !phi) return fail(MPF_EINVAL, "mpf_upload_phi: null argument");
+    if (dtype != MPF_FP32 && dtype != MPF_FP64) return fail(MPF_EINVAL, "bad dtype");
+    size_t esz = dtype == MPF_FP64 ? 8 : 4;
+    size_t bytes = h->ncell * (size_t)h->cfg.n_phase * esz;
+    void *d = nullptr;
+    HIP_TRY(hipMalloc(&d, bytes));
+    int *err = nullptr;
+    hipError_t e = hipMalloc((void **)&err, sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(d, phi, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(err, 0, sizeof(int));
+    if (e == hipSuccess) {
+        const mpf_config &c = h->cfg;
+        if (h->rsz == 8)
+            e = mpf_launch_pack_f64(d, dtype == MPF_FP64, c.n_phase, c.nx, c.ny, c.nz, view<double>(h, h->cur),
+                                    h->scap, err, h->stream);
+        else
+            e = mpf_launch_pack_f32(d, dtype == MPF_FP64, c.n_phase, c.nx, c.ny, c.nz, view<float>(h, h->cur),
+                                    h->scap, err, h->stream);
+    }
+    int herr = 0;
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost);
+    hipFree(d);
+    hipFree(err);
+    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_upload_phi: %s", hipGetErrorString(e));
+    if (herr) return fail(MPF_ECAPACITY, "a cell holds more than %d non-zero solid phases", h->scap);
+    return 0;
+}
+
+extern "C" int mpf_init_columns(mpf_handle *h, const int32_t *grain_map, const int32_t *profile, int nprof,
+                                const double *solid, const double *liquid)
+{
+    if (!h || !grain_map || !profile || !solid || !liquid || nprof < 1)
+        return fail(MPF_EINVAL, "mpf_init_columns: bad argument");
+    const mpf_config &c = h->cfg;
+    size_t ncol = (size_t)c.nx * c.ny;
+    for (size_t i = 0; i < ncol; ++i) {
+        if (profile[i] < 0 || profile[i] >= nprof) return fail(MPF_EINVAL, "profile index out of range");
+        if (grain_map[i] < 0 || grain_map[i] >= c.n_phase) return fail(MPF_EINVAL, "grain id out of range");
+    }
+    void *d = nullptr;
+    size_t bytes = ncol * 2 * sizeof(int32_t) + (size_t)nprof * c.nz * 2 * sizeof(double);
+    HIP_TRY(hipMalloc(&d, bytes));
+    int32_t *dg = (int32_t *)d, *dp = dg + ncol;
+    double *ds = (double *)(dp + ncol), *dl = ds + (size_t)nprof * c.nz;
+    hipError_t e = hipMemcpy(dg, grain_map, ncol * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dp, profile, ncol * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ds, solid, (size_t)nprof * c.nz * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dl, liquid, (size_t)nprof * c.nz * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        if (h->rsz == 8)
+            e = mpf_launch_init_columns_f64(dg, dp, ds, dl, c.nx, c.ny, c.nz, view<double>(h, h->cur), h->stream);
+        else
+            e = mpf_launch_init_columns_f32(dg, dp, ds, dl, c.nx, c.ny, c.nz, view<float>(h, h->cur), h->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    hipFree(d);
+    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_init_columns: %s", hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int mpf_upload_temp(mpf_handle *h, const double *temp)
+{
+    if (!h || !temp) return fail(MPF_EINVAL, "mpf_upload_temp: null argument");
+    const mpf_config &c = h->cfg;
+    if (!c.temp_dense) {
+        HIP_TRY(hipMemcpy(h->T[h->tcur], temp, h->nT * sizeof(double), hipMemcpyHostToDevice));
+        return 0;
+    }
+    double *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, h->ncell * sizeof(double)));
+    hipError_t e = hipMemcpy(d, temp, h->ncell * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = mpf_launch_temp_layout_f64(d, h->T[h->tcur], c.nx, c.ny, c.nz, 1, 1, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    hipFree(d);
+    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_upload_temp: %s", hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int mpf_download_temp(mpf_handle *h, double *temp)
+{
+    if (!h || !temp) return fail(MPF_EINVAL, "mpf_download_temp: null argument");
+    const mpf_config &c = h->cfg;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (!c.temp_dense) {
+        HIP_TRY(hipMemcpy(temp, h->T[h->tcur], h->nT * sizeof(double), hipMemcpyDeviceToHost));
+        return 0;
+    }
+    double *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, h->ncell * sizeof(double)));
+    hipError_t e = mpf_launch_temp_layout_f64(h->T[h->tcur], d, c.nx, c.ny, c.nz, 0, 1, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(temp, d, h->ncell * sizeof(double), hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_download_temp: %s", hipGetErrorString(e));
+    return 0;
+}
+
+template <typename R>
+static hipError_t launch_one_step(mpf_handle *h, int general_blocks)
+{
+    const mpf_config &c = h->cfg;
+    StepArgs<R> a;
+    memset(&a, 0, sizeof(a));
+    a.K = make_consts<R>(c.s, c.nx, c.ny, c.nz, c.n_phase, c.kmax, c.enable_anisotropy,
+                         c.dim == 2 ? 1 : c.enable_torque, h->tab);
+    if (c.dim == 2) a.K.aniso = 1;
+    a.in = view<R>(h, h->cur);
+    a.out = view<R>(h, h->cur ^ 1);
+    a.T_in = h->T[h->tcur];
+    a.T_out = h->T[h->tcur ^ 1];
+    a.temp_dense = c.temp_dense;
+    a.temp_f32 = c.temp_f32;
+    a.cooling = c.cooling_rate;
+    a.worklist = h->worklist;
+    a.wl_count = h->wl_count;
+    a.step = h->step_counter;
+    a.halt_mode = c.halt_on_kmax;
+    a.stats = h->stats;
+    if (sizeof(R) == 8)
+        return mpf_launch_step_f64((const StepArgs<double> *)&a, c.dim, general_blocks, h->stream);
+    return mpf_launch_step_f32((const StepArgs<float> *)&a, c.dim, general_blocks, h->stream);
+}
+
+extern "C" int mpf_step(mpf_handle *h, int nsteps)
+{
+    if (!h) return fail(MPF_EINVAL, "mpf_step: null handle");
+    if (nsteps < 0) return fail(MPF_EINVAL, "mpf_step: negative step count");
+    if (!h->tables_set) return fail(MPF_EINVAL, "mpf_step: tables not set (mpf_set_tables)");
+    if (nsteps == 0) return 0;
+    const mpf_config &c = h->cfg;
+    // a batch starts with clear worklist counters and no halt
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    int m1 = -1;
+    HIP_TRY(hipMemcpy(&h->stats->halt_step, &m1, sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(h->wl_count, 0, 2 * sizeof(int)));
+    const int s0 = h->step_counter, cur0 = h->cur, tcur0 = h->tcur;
+    const int general_blocks = 256;
+    HIP_TRY(hipEventRecord(h->ev0, h->stream));
+    for (int i = 0; i < nsteps; ++i) {
+        hipError_t e = (h->rsz == 8) ? launch_one_step<double>(h, general_blocks)
+                                     : launch_one_step<float>(h, general_blocks);
+        if (e != hipSuccess) return fail(MPF_EHIP, "step launch: %s", hipGetErrorString(e));
+        h->cur ^= 1;
+        h->tcur ^= 1;
+        h->step_counter += 1;
+    }
+    HIP_TRY(hipEventRecord(h->ev1, h->stream));
+    HIP_TRY(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms = ms;
+    int done = nsteps;
+    if (c.halt_on_kmax) {
+        int hs = -1;
+        HIP_TRY(hipMemcpy(&hs, &h->stats->halt_step, sizeof(int), hipMemcpyDeviceToHost));
+        if (hs >= 0) {
+            done = hs - s0;
+            h->cur = cur0 ^ (done & 1);
+            h->tcur = tcur0 ^ ((done + 1) & 1);
+            h->step_counter = hs + 1;
+        }
+    }
+    h->steps_done += done;
+    return done;
+}
+
+extern "C" int mpf_download_phi(mpf_handle *h, void *phi, int dtype)
+{
+    if (!h || !phi) return fail(MPF_EINVAL, "mpf_download_phi: null argument");
+    if (dtype != MPF_FP32 && dtype != MPF_FP64) return fail(MPF_EINVAL, "bad dtype");
+    const mpf_config &c = h->cfg;
+    size_t esz = dtype == MPF_FP64 ? 8 : 4;
+    size_t bytes = h->ncell * (size_t)c.n_phase * esz;
+    void *d = nullptr;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMalloc(&d, bytes));
+    hipError_t e = hipMemsetAsync(d, 0, bytes, h->stream);
+    if (e == hipSuccess) {
+        if (h->rsz == 8)
+            e = mpf_launch_unpack_f64(view<double>(h, h->cur), c.nx, c.ny, c.nz, d, dtype == MPF_FP64, h->stream);
+        else
+            e = mpf_launch_unpack_f32(view<float>(h, h->cur), c.nx, c.ny, c.nz, d, dtype == MPF_FP64, h->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(phi, d, bytes, hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_download_phi: %s", hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int mpf_download_apt(mpf_handle *h, int32_t *mf, int depth, int32_t *nf, int32_t *status)
+{
+    if (!h || !mf || !nf || !status || depth < 1) return fail(MPF_EINVAL, "mpf_download_apt: bad argument");
+    const mpf_config &c = h->cfg;
+    size_t bm = (size_t)depth * h->ncell * sizeof(int32_t), bn = h->ncell * sizeof(int32_t);
+    void *d = nullptr;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMalloc(&d, bm + bn + 2 * sizeof(int32_t)));
+    int32_t *dmf = (int32_t *)d, *dnf = dmf + (size_t)depth * h->ncell, *dst = dnf + h->ncell;
+    hipError_t e = hipMemsetAsync(d, 0, bm + bn + 2 * sizeof(int32_t), h->stream);
+    if (e == hipSuccess) {
+        if (h->rsz == 8) {
+            Consts<double> K = make_consts<double>(c.s, c.nx, c.ny, c.nz, c.n_phase, c.kmax, 1, 1, h->tab);
+            e = mpf_launch_apt_slots_f64(&K, view<double>(h, h->cur), c.dim, dmf, dnf, dst, depth, h->stream);
+        } else {
+            Consts<float> K = make_consts<float>(c.s, c.nx, c.ny, c.nz, c.n_phase, c.kmax, 1, 1, h->tab);
+            e = mpf_launch_apt_slots_f32(&K, view<float>(h, h->cur), c.dim, dmf, dnf, dst, depth, h->stream);
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(mf, dmf, bm, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(nf, dnf, bn, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(status, dst, 2 * sizeof(int32_t), hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_download_apt: %s", hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int mpf_get_stats(mpf_handle *h, mpf_stats *st)
+{
+    if (!h || !st) return fail(MPF_EINVAL, "mpf_get_stats: null argument");
+    Stats s;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(&s, h->stats, sizeof(s), hipMemcpyDeviceToHost));
+    st->max_active = s.max_active;
+    st->kmax_overflow = s.kmax_overflow;
+    st->overflow_cells = s.overflow_cells;
+    st->general_cells = s.slow_cells;
+    st->halt_step = s.halt_step;
+    st->steps_done = h->steps_done;
+    st->last_step_ms = h->last_ms;
+    return 0;
+}
+
+extern "C" int mpf_reset_stats(mpf_handle *h)
+{
+    if (!h) return fail(MPF_EINVAL, "mpf_reset_stats: null handle");
+    return reset_stats_dev(h);
+}
+
+extern "C" int mpf_observables(mpf_handle *h, double *grain_volume, int32_t *front, double *out)
+{
+    if (!h || !grain_volume || !front || !out) return fail(MPF_EINVAL, "mpf_observables: null argument");
+    const mpf_config &c = h->cfg;
+    int N = c.n_phase, nz = c.nz;
+    size_t ncol = (size_t)c.nx * c.ny;
+    if (((size_t)N + 256 * 5) * sizeof(double) > 64 * 1024)
+        return fail(MPF_EINVAL, "mpf_observables supports at most %d phases", 64 * 1024 / 8 - 1280);
+    size_t bytes = (size_t)N * 8 + (size_t)nz * 5 * 8 + ncol * 4;
+    void *d = nullptr;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMalloc(&d, bytes));
+    double *gv = (double *)d, *part = gv + N;
+    int32_t *fr = (int32_t *)(part + (size_t)nz * 5);
+    hipError_t e = hipMemsetAsync(d, 0, bytes, h->stream);
+    if (e == hipSuccess) {
+        if (h->rsz == 8)
+            e = mpf_launch_observe_f64(view<double>(h, h->cur), c.nx, c.ny, nz, N, gv, part, fr, h->stream);
+        else
+            e = mpf_launch_observe_f32(view<float>(h, h->cur), c.nx, c.ny, nz, N, gv, part, fr, h->stream);
+    }
+    std::vector<double> hp((size_t)nz * 5);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(grain_volume, gv, (size_t)N * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(front, fr, ncol * 4, hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_observables: %s", hipGetErrorString(e));
+    double emax = 0.0, pmin = 1e300, pmax = -1e300, nanc = 0.0, solid = 0.0;
+    for (int k = 0; k < nz; ++k) {
+        const double *q = hp.data() + (size_t)k * 5;
+        emax = q[0] > emax ? q[0] : emax;
+        pmin = q[1] < pmin ? q[1] : pmin;
+        pmax = q[2] > pmax ? q[2] : pmax;
+        nanc += q[3];
+        solid += q[4];
+    }
+    out[0] = emax;
+    out[1] = pmin;
+    out[2] = pmax;
+    out[3] = nanc;
+    out[4] = solid;
+    out[5] = (double)h->ncell;
+    return 0;
+}
+
+// ------------------------------------------------------ drop-in kernels
+static hipStream_t as_stream(void *s) { return (hipStream_t)s; }
+
+extern "C" int mpf_dense_apt(const void *phi, int dtype, int32_t *mf, int32_t *nf, int32_t *status, int N,
+                             int nx, int ny, int nz, int dim, int depth, int kmax, int checked, void *stream)
+{
+    if (!phi || !mf || !nf || (checked && !status)) return fail(MPF_EINVAL, "mpf_dense_apt: null argument");
+    if (dim != 2 && dim != 3) return fail(MPF_EINVAL, "dim must be 2 or 3");
+    if (depth < 1 || kmax < 1) return fail(MPF_EINVAL, "depth and kmax must be positive");
+    hipError_t e;
+    if (dtype == MPF_FP64)
+        e = mpf_launch_dense_apt_f64((const double *)phi, mf, nf, status, N, nx, ny, nz, dim, depth, kmax, checked,
+                                     as_stream(stream));
+    else if (dtype == MPF_FP32)
+        e = mpf_launch_dense_apt_f32((const float *)phi, mf, nf, status, N, nx, ny, nz, dim, depth, kmax, checked,
+                                     as_stream(stream));
+    else
+        return fail(MPF_EINVAL, "bad dtype");
+    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_dense_apt: %s", hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int mpf_dense_update(const void *phi, void *phi_new, int dtype, const void *temp, int temp_dtype,
+                                const int32_t *mf, const int32_t *nf, const void *wij, const void *aij,
+                                const void *mij, const void *n111, const mpf_scalars *s, int N, int nx, int ny,
+                                int nz, int dim, int depth, int kmax, int aniso, int torque, void *stream)
+{
+    if (!phi || !phi_new || !temp || !mf || !nf || !wij || !aij || !mij || !n111 || !s)
+        return fail(MPF_EINVAL, "mpf_dense_update: null argument");
+    if (dim != 2 && dim != 3) return fail(MPF_EINVAL, "dim must be 2 or 3");
+    if (kmax < 1 || kmax > KCAP) return fail(MPF_EINVAL, "kmax must be in [1, %d]", KCAP);
+    if (depth < 1) return fail(MPF_EINVAL, "depth must be positive");
+    // The drop-in kernel takes the four tables as separate arrays; gather
+    // them into the contiguous block the device code indexes.
+    size_t esz = dtype == MPF_FP64 ? 8 : 4, nn = (size_t)N * N;
+    size_t tb = (3 * nn + (size_t)N * 24) * esz;
+    void *tab = nullptr;
+    HIP_TRY(hipMalloc(&tab, tb));
+    hipStream_t st = as_stream(stream);
+    hipError_t e = hipMemcpyAsync(tab, wij, nn * esz, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync((char *)tab + nn * esz, aij, nn * esz, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync((char *)tab + 2 * nn * esz, mij, nn * esz, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync((char *)tab + 3 * nn * esz, n111, (size_t)N * 24 * esz, hipMemcpyDeviceToDevice, st);
+    int t64 = temp_dtype == MPF_FP64;
+    int nzz = dim == 3 ? nz : 1;
+    if (e == hipSuccess) {
+        if (dtype == MPF_FP64) {
+            Consts<double> K = make_consts<double>(*s, nx, ny, nzz, N, kmax, aniso, dim == 2 ? 1 : torque, tab);
+            if (dim == 2) K.aniso = 1;
+            e = mpf_launch_dense_update_f64(&K, dim, (const double *)phi, (double *)phi_new, temp, t64, mf, nf,
+                                            depth, st);
+        } else {
+            Consts<float> K = make_consts<float>(*s, nx, ny, nzz, N, kmax, aniso, dim == 2 ? 1 : torque, tab);
+            if (dim == 2) K.aniso = 1;
+            e = mpf_launch_dense_update_f32(&K, dim, (const float *)phi, (float *)phi_new, temp, t64, mf, nf,
+                                            depth, st);
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(tab);
+    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_dense_update: %s", hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int mpf_dense_temp(void *temp, int temp_dtype, double cooling, size_t n, void *stream)
+{
+    if (!temp) return fail(MPF_EINVAL, "mpf_dense_temp: null argument");
+    hipError_t e = mpf_launch_dense_temp_f64(temp, temp_dtype == MPF_FP64, cooling, n, as_stream(stream));
+    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_dense_temp: %s", hipGetErrorString(e));
+    return 0;
+}
+
+// ------------------------------------------------------------ memory
+extern "C" int mpf_malloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return fail(MPF_EINVAL, "mpf_malloc: null argument");
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) return fail(MPF_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return 0;
+}
+extern "C" int mpf_free(void *ptr)
+{
+    HIP_TRY(hipFree(ptr));
+    return 0;
+}
+extern "C" int mpf_memcpy_htod(void *dst, const void *src, size_t bytes)
+{
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+extern "C" int mpf_memcpy_dtoh(void *dst, const void *src, size_t bytes)
+{
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+extern "C" int mpf_memset(void *ptr, int value, size_t bytes)
+{
+    HIP_TRY(hipMemset(ptr, value, bytes));
+    return 0;
+}
+extern "C" int mpf_device_synchronize(void)
+{
+    HIP_TRY(hipDeviceSynchronize());
+    return 0;
+}

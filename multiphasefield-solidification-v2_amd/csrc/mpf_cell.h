@@ -1,0 +1,647 @@
+// mpf_cell.h -- per-cell multi-phase-field update for gfx950.
+//
+// The physics of one cell, independent of how the state is stored: the caller
+// gathers the active set (liquid flag + ascending solid ids) and the values of
+// every active phase at the cell and its face neighbours, plus a functor that
+// reads the liquid field at any cell (the anisotropy terms need radius 2), and
+// gets back the new values.  The slot-layout step kernel and the dense
+// drop-in kernels both use it.
+//
+// Reference: src/gpu_kernels_3d.py:24-441 (helpers), :530-830 (update kernels)
+// and src/gpu_kernels.py:25-515 (2D).  In FP64 every expression is evaluated in
+// the order the reference writes it (built with -ffp-contract=off), so the
+// result is bit-identical to the CPU oracle.  Values the reference recomputes
+// several times per cell (liquid gradients at the 7 points, the facet search,
+// Laplacians inside the O(n^3) loop) are computed once here; because they are
+// deterministic functions of the same inputs this changes no bits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mpf_math.h"
+
+namespace mpf {
+
+// Stencil point order used everywhere: centre, x+, x-, y+, y-, z+, z-.
+enum { P_C = 0, P_XP = 1, P_XM = 2, P_YP = 3, P_YM = 4, P_ZP = 5, P_ZM = 6 };
+
+template <typename Real>
+struct Consts {
+    int nx, ny, nz, N, kmax;
+    int aniso, torque;
+    Real dx, dt;
+    Real two_dx;           // 2.0*dx
+    Real inv_dx2;          // 1.0/(dx*dx)
+    Real q_inv_dx2;        // 0.25*inv_dx2
+    Real half_over_dx;     // 0.5/dx
+    Real eps0_sl, eps0_sq; // eps0, eps0*eps0
+    Real w0_sl;
+    Real tan_a0, delta_a, mu_a, mu_delta, p2; // mu_a*delta_a, p_round*p_round
+    Real g2_floor, ksi, theta_c;
+    double Sf, T_melt;
+    const Real *wij, *aij, *mij, *n111; // [N][N] x3, [N][8][3]
+};
+
+// ---------------------------------------------------------------- math glue
+__device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float m_sqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double m_abs(double x) { return fabs(x); }
+__device__ __forceinline__ float m_abs(float x) { return fabsf(x); }
+__device__ __forceinline__ double m_tan(double x) { return mpf_tan(x); }
+__device__ __forceinline__ float m_tan(float x) { return tanf(x); }
+__device__ __forceinline__ double m_acos(double x) { return mpf_acos(x); }
+__device__ __forceinline__ float m_acos(float x) { return acosf(x); }
+__device__ __forceinline__ double m_tanh(double x) { return mpf_tanh(x); }
+__device__ __forceinline__ float m_tanh(float x) { return tanhf(x); }
+template <typename Real>
+__device__ __forceinline__ Real pymax(Real a, Real b) { return (b > a) ? b : a; }
+
+// calc_a_from_cos, src/gpu_kernels_3d.py:24-30
+template <typename Real>
+__device__ __forceinline__ Real calc_a(const Consts<Real> &K, Real cost)
+{
+    Real c2 = cost * cost;
+    Real C = m_sqrt(c2 + K.p2);
+    Real S = m_sqrt(pymax(Real(1.0) - c2, Real(0.0)) + K.p2);
+    return K.mu_a * (Real(1.0) + K.delta_a * (C + K.tan_a0 * S));
+}
+
+// calc_b_from_cos, src/gpu_kernels_3d.py:33-55
+template <typename Real>
+__device__ __forceinline__ Real calc_b(const Consts<Real> &K, Real best_cost)
+{
+    Real c = best_cost;
+    if (c > Real(1.0))
+        c = Real(1.0);
+    else if (c < Real(-1.0))
+        c = Real(-1.0);
+    Real theta = m_acos(c);
+    if (theta >= K.theta_c) return Real(1.0);
+    Real x = theta / K.theta_c;
+    const Real eps = Real(1.0e-6);
+    const Real half_pi = Real(0.5 * 3.14159265358979311600e+00);
+    Real y = half_pi * x;
+    if (y > half_pi - eps) y = half_pi - eps;
+    if (y < eps) y = eps;
+    Real t = m_tan(y);
+    return K.ksi + (Real(1.0) - K.ksi) * t * m_tanh(Real(1.0) / t);
+}
+
+// ------------------------------------------------------ boundary conditions
+// src/gpu_kernels_3d.py:58-91 (x,y periodic; z mirror) and
+// src/gpu_kernels.py:84-105 (2D: x periodic, y mirror).
+__device__ __forceinline__ int bc_pp(int i, int n) { return i < n - 1 ? i + 1 : 0; }
+__device__ __forceinline__ int bc_pm(int i, int n) { return i > 0 ? i - 1 : n - 1; }
+__device__ __forceinline__ int bc_mp(int i, int n) { return i < n - 1 ? i + 1 : n - 1; }
+__device__ __forceinline__ int bc_mm(int i, int n) { return i > 0 ? i - 1 : 0; }
+
+// Result of the {111} facet search at one stencil point for one grain.
+template <typename Real>
+struct Facet {
+    Real cs, ca, nx, ny, nz; // signed cos, |cos|, normal of the best facet
+};
+
+// facet_cos_and_nxyz_from_grad (src/gpu_kernels_3d.py:251-301) given the unit
+// gradient; best_cos_from_grad_3d (:110-134) returns the same |cos|.
+template <typename Real, int DIM>
+__device__ __forceinline__ Facet<Real> facet_search(const Real *n, Real gx, Real gy, Real gz)
+{
+    Facet<Real> f;
+    f.cs = Real(0.0);
+    f.ca = Real(0.0);
+    f.nx = Real(1.0);
+    f.ny = Real(0.0);
+    f.nz = Real(0.0);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        Real ux = n[t * 3 + 0], uy = n[t * 3 + 1], uz = n[t * 3 + 2];
+        Real dot;
+        if constexpr (DIM == 3)
+            dot = gx * ux + gy * uy + gz * uz;
+        else
+            dot = gx * ux + gy * uy;
+        Real ad = m_abs(dot);
+        if (ad > f.ca) {
+            f.ca = ad;
+            f.cs = dot;
+            f.nx = ux;
+            f.ny = uy;
+            f.nz = uz;
+        }
+    }
+    if (f.ca > Real(1.0)) f.ca = Real(1.0);
+    if (f.cs > Real(1.0)) f.cs = Real(1.0);
+    if (f.cs < Real(-1.0)) f.cs = Real(-1.0);
+    return f;
+}
+
+// Liquid gradient at one stencil point with its derived quantities.
+template <typename Real>
+struct LGrad {
+    Real gx, gy, gz; // central differences (grad_phi_xyz)
+    Real q;          // |grad|^2
+    Real inv_g;      // 1/sqrt(q)       (valid when q >= g2_floor)
+    Real inv_g3;     // 1/(q*sqrt(q))
+    Real ux, uy, uz; // unit gradient
+    bool ok;         // q >= g2_floor
+};
+
+template <typename Real, int DIM>
+__device__ __forceinline__ void lgrad_finish(const Consts<Real> &K, LGrad<Real> &G)
+{
+    if constexpr (DIM == 3)
+        G.q = G.gx * G.gx + G.gy * G.gy + G.gz * G.gz;
+    else
+        G.q = G.gx * G.gx + G.gy * G.gy;
+    G.ok = !(G.q < K.g2_floor);
+    if (G.ok) {
+        Real sq = m_sqrt(G.q);
+        G.inv_g = Real(1.0) / sq;
+        G.inv_g3 = Real(1.0) / (G.q * sq);
+        G.ux = G.gx * G.inv_g;
+        G.uy = G.gy * G.inv_g;
+        G.uz = G.gz * G.inv_g;
+    } else {
+        G.inv_g = G.inv_g3 = G.ux = G.uy = G.uz = Real(0.0);
+    }
+}
+
+// A12 quantities at one point: a, coef and dc/dphi (da/dphi = mu_delta*coef*dc).
+template <typename Real>
+struct A12v {
+    Real a, dax, day, daz;
+};
+
+// da_dphixyz_A12, src/gpu_kernels_3d.py:304-341 (2D: src/gpu_kernels.py:237-264)
+template <typename Real, int DIM>
+__device__ __forceinline__ A12v<Real> a12_at(const Consts<Real> &K, const LGrad<Real> &G,
+                                             const Facet<Real> &f)
+{
+    A12v<Real> r;
+    if (!G.ok) {
+        r.a = r.dax = r.day = r.daz = Real(0.0);
+        return r;
+    }
+    r.a = calc_a(K, f.ca);
+    Real udotg;
+    if constexpr (DIM == 3)
+        udotg = G.gx * f.nx + G.gy * f.ny + G.gz * f.nz;
+    else
+        udotg = G.gx * f.nx + G.gy * f.ny;
+    Real dcx = f.nx * G.inv_g - udotg * G.gx * G.inv_g3;
+    Real dcy = f.ny * G.inv_g - udotg * G.gy * G.inv_g3;
+    Real c = f.cs;
+    Real C = m_sqrt(c * c + K.p2);
+    Real S = m_sqrt(pymax(Real(1.0) - c * c, Real(0.0)) + K.p2);
+    Real coef = (c / C) - K.tan_a0 * (c / S);
+    r.dax = K.mu_delta * coef * dcx;
+    r.day = K.mu_delta * coef * dcy;
+    if constexpr (DIM == 3) {
+        Real dcz = f.nz * G.inv_g - udotg * G.gz * G.inv_g3;
+        r.daz = K.mu_delta * coef * dcz;
+    } else {
+        r.daz = Real(0.0);
+    }
+    return r;
+}
+
+// Reads the liquid field at absolute cell coordinates.
+//   Real operator()(int l, int m, int k) const
+// Liquid anisotropy context of one cell: gradients at the centre and the face
+// neighbours plus the second derivatives at the centre.
+template <typename Real, int DIM>
+struct LiqCtx {
+    static constexpr int NP = (DIM == 3) ? 7 : 5;
+    LGrad<Real> G[NP];
+    Real v[NP];                         // liquid values at the stencil points
+    Real dxx, dyy, dzz, dxy, dxz, dyz;  // d2_phi_xyz at the centre
+};
+
+template <typename Real, int DIM, typename LiqAt>
+__device__ __forceinline__ void lgrad_at(const Consts<Real> &K, const LiqAt &L, int l, int m, int k,
+                                         LGrad<Real> &G)
+{
+    if constexpr (DIM == 3) {
+        int lp = bc_pp(l, K.nx), lm = bc_pm(l, K.nx);
+        int mp = bc_pp(m, K.ny), mm = bc_pm(m, K.ny);
+        int kp = bc_mp(k, K.nz), km = bc_mm(k, K.nz);
+        G.gx = (L(lp, m, k) - L(lm, m, k)) / K.two_dx;
+        G.gy = (L(l, mp, k) - L(l, mm, k)) / K.two_dx;
+        G.gz = (L(l, m, kp) - L(l, m, km)) / K.two_dx;
+    } else {
+        int lp = bc_pp(l, K.nx), lm = bc_pm(l, K.nx);
+        int mp = bc_mp(m, K.ny), mm = bc_mm(m, K.ny);
+        G.gx = (L(lp, m, 0) - L(lm, m, 0)) / K.two_dx;
+        G.gy = (L(l, mp, 0) - L(l, mm, 0)) / K.two_dx;
+        G.gz = Real(0.0);
+    }
+    lgrad_finish<Real, DIM>(K, G);
+}
+
+template <typename Real, int DIM, typename LiqAt>
+__device__ __forceinline__ void build_liq_ctx(const Consts<Real> &K, const LiqAt &L, int l, int m, int k,
+                                           LiqCtx<Real, DIM> &X)
+{
+    int lp = bc_pp(l, K.nx), lm = bc_pm(l, K.nx);
+    if constexpr (DIM == 3) {
+        int mp = bc_pp(m, K.ny), mm = bc_pm(m, K.ny);
+        int kp = bc_mp(k, K.nz), km = bc_mm(k, K.nz);
+        lgrad_at<Real, 3>(K, L, l, m, k, X.G[P_C]);
+        lgrad_at<Real, 3>(K, L, lp, m, k, X.G[P_XP]);
+        lgrad_at<Real, 3>(K, L, lm, m, k, X.G[P_XM]);
+        lgrad_at<Real, 3>(K, L, l, mp, k, X.G[P_YP]);
+        lgrad_at<Real, 3>(K, L, l, mm, k, X.G[P_YM]);
+        lgrad_at<Real, 3>(K, L, l, m, kp, X.G[P_ZP]);
+        lgrad_at<Real, 3>(K, L, l, m, km, X.G[P_ZM]);
+        Real c = L(l, m, k);
+        Real xp = L(lp, m, k), xm = L(lm, m, k), yp = L(l, mp, k), ym = L(l, mm, k);
+        Real zp = L(l, m, kp), zm = L(l, m, km);
+        X.v[P_C] = c; X.v[P_XP] = xp; X.v[P_XM] = xm; X.v[P_YP] = yp; X.v[P_YM] = ym;
+        X.v[P_ZP] = zp; X.v[P_ZM] = zm;
+        // d2_phi_xyz, src/gpu_kernels_3d.py:207-248
+        X.dxx = (xp - Real(2.0) * c + xm) * K.inv_dx2;
+        X.dyy = (yp - Real(2.0) * c + ym) * K.inv_dx2;
+        X.dzz = (zp - Real(2.0) * c + zm) * K.inv_dx2;
+        X.dxy = (L(lp, mp, k) - L(lp, mm, k) - L(lm, mp, k) + L(lm, mm, k)) * K.q_inv_dx2;
+        X.dxz = (L(lp, m, kp) - L(lp, m, km) - L(lm, m, kp) + L(lm, m, km)) * K.q_inv_dx2;
+        X.dyz = (L(l, mp, kp) - L(l, mp, km) - L(l, mm, kp) + L(l, mm, km)) * K.q_inv_dx2;
+    } else {
+        int mp = bc_mp(m, K.ny), mm = bc_mm(m, K.ny);
+        lgrad_at<Real, 2>(K, L, l, m, 0, X.G[P_C]);
+        lgrad_at<Real, 2>(K, L, lp, m, 0, X.G[P_XP]);
+        lgrad_at<Real, 2>(K, L, lm, m, 0, X.G[P_XM]);
+        lgrad_at<Real, 2>(K, L, l, mp, 0, X.G[P_YP]);
+        lgrad_at<Real, 2>(K, L, l, mm, 0, X.G[P_YM]);
+        Real c = L(l, m, 0);
+        Real xp = L(lp, m, 0), xm = L(lm, m, 0), yp = L(l, mp, 0), ym = L(l, mm, 0);
+        X.v[P_C] = c; X.v[P_XP] = xp; X.v[P_XM] = xm; X.v[P_YP] = yp; X.v[P_YM] = ym;
+        // d2_phi_xy, src/gpu_kernels.py:184-202
+        X.dxx = (xp - Real(2.0) * c + xm) * K.inv_dx2;
+        X.dyy = (yp - Real(2.0) * c + ym) * K.inv_dx2;
+        X.dzz = Real(0.0);
+        X.dxy = (L(lp, mp, 0) - L(lp, mm, 0) - L(lm, mp, 0) + L(lm, mm, 0)) * K.q_inv_dx2;
+        X.dxz = X.dyz = Real(0.0);
+    }
+}
+
+// lap_sl and b_sl of one active solid s (src/gpu_kernels_3d.py:558-577,
+// :709-733; 2D src/gpu_kernels.py:394-408).  Requires K.aniso.
+template <typename Real, int DIM>
+__device__ __forceinline__ void solid_liquid_terms(const Consts<Real> &K, const LiqCtx<Real, DIM> &X,
+                                                int s, Real &lap_sl, Real &b_sl)
+{
+    constexpr int NP = LiqCtx<Real, DIM>::NP;
+    Real nrm[24];
+#pragma unroll
+    for (int u = 0; u < 24; ++u) nrm[u] = K.n111[(size_t)s * 24 + u];
+    Facet<Real> f[NP];
+    Real e2[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        // eps2_at_cell_from_liquid_3d (:137-146): best_cos is 0 below the floor.
+        Real bc = Real(0.0);
+        if (X.G[p].ok) {
+            f[p] = facet_search<Real, DIM>(nrm, X.G[p].ux, X.G[p].uy, X.G[p].uz);
+            bc = f[p].ca;
+        } else {
+            f[p].cs = f[p].ca = Real(0.0);
+            f[p].nx = Real(1.0);
+            f[p].ny = f[p].nz = Real(0.0);
+        }
+        Real a = calc_a(K, bc);
+        Real e = K.eps0_sl * a;
+        e2[p] = e * e;
+    }
+    // aniso_term1_solid_3d (:149-204)
+    Real pc = X.v[P_C];
+    Real Fx_p = (e2[P_C] + e2[P_XP]) * (X.v[P_XP] - pc) / K.two_dx;
+    Real Fx_m = (e2[P_C] + e2[P_XM]) * (pc - X.v[P_XM]) / K.two_dx;
+    Real Fy_p = (e2[P_C] + e2[P_YP]) * (X.v[P_YP] - pc) / K.two_dx;
+    Real Fy_m = (e2[P_C] + e2[P_YM]) * (pc - X.v[P_YM]) / K.two_dx;
+    Real lap_val;
+    if constexpr (DIM == 3) {
+        Real Fz_p = (e2[P_C] + e2[P_ZP]) * (X.v[P_ZP] - pc) / K.two_dx;
+        Real Fz_m = (e2[P_C] + e2[P_ZM]) * (pc - X.v[P_ZM]) / K.two_dx;
+        lap_val = (Fx_p - Fx_m) / K.dx + (Fy_p - Fy_m) / K.dx + (Fz_p - Fz_m) / K.dx;
+    } else {
+        lap_val = (Fx_p - Fx_m) / K.dx + (Fy_p - Fy_m) / K.dx;
+    }
+
+    // torque_A11_3d (:386-441); 2D torque_A11 (src/gpu_kernels.py:290-332).
+    // Below the gradient floor the torque is 0.0 and is still added.
+    Real tq = Real(0.0);
+    if (K.torque && X.G[P_C].ok) {
+        const LGrad<Real> &G = X.G[P_C];
+        Real phix = G.gx, phiy = G.gy, phiz = G.gz;
+        Real q = G.q;
+        Real q_x, q_y, q_z;
+        if constexpr (DIM == 3) {
+            q_x = Real(2.0) * (phix * X.dxx + phiy * X.dxy + phiz * X.dxz);
+            q_y = Real(2.0) * (phix * X.dxy + phiy * X.dyy + phiz * X.dyz);
+            q_z = Real(2.0) * (phix * X.dxz + phiy * X.dyz + phiz * X.dzz);
+        } else {
+            q_x = Real(2.0) * (phix * X.dxx + phiy * X.dxy);
+            q_y = Real(2.0) * (phix * X.dxy + phiy * X.dyy);
+            q_z = Real(0.0);
+        }
+        A12v<Real> ac = a12_at<Real, DIM>(K, G, f[P_C]);
+        const Facet<Real> &fc = f[P_C];
+        Real udotg;
+        if constexpr (DIM == 3)
+            udotg = phix * fc.nx + phiy * fc.ny + phiz * fc.nz;
+        else
+            udotg = phix * fc.nx + phiy * fc.ny;
+        Real dcx = fc.nx * G.inv_g - udotg * phix * G.inv_g3;
+        Real dcy = fc.ny * G.inv_g - udotg * phiy * G.inv_g3;
+        Real c = fc.cs;
+        Real C = m_sqrt(c * c + K.p2);
+        Real S = m_sqrt(pymax(Real(1.0) - c * c, Real(0.0)) + K.p2);
+        Real coef = (c / C) - K.tan_a0 * (c / S);
+        Real a_val = ac.a;
+        // A13_divergence_3d (:344-383): da/dphi_p at the face neighbours
+        Real d_dx = (a12_at<Real, DIM>(K, X.G[P_XP], f[P_XP]).dax -
+                     a12_at<Real, DIM>(K, X.G[P_XM], f[P_XM]).dax) * K.half_over_dx;
+        Real d_dy = (a12_at<Real, DIM>(K, X.G[P_YP], f[P_YP]).day -
+                     a12_at<Real, DIM>(K, X.G[P_YM], f[P_YM]).day) * K.half_over_dx;
+        if constexpr (DIM == 3) {
+            Real dcz = fc.nz * G.inv_g - udotg * phiz * G.inv_g3;
+            Real c_x = dcx * X.dxx + dcy * X.dxy + dcz * X.dxz;
+            Real c_y = dcx * X.dxy + dcy * X.dyy + dcz * X.dyz;
+            Real c_z = dcx * X.dxz + dcy * X.dyz + dcz * X.dzz;
+            Real da_dx = K.mu_delta * coef * c_x;
+            Real da_dy = K.mu_delta * coef * c_y;
+            Real da_dz = K.mu_delta * coef * c_z;
+            Real d_dz = (a12_at<Real, DIM>(K, X.G[P_ZP], f[P_ZP]).daz -
+                         a12_at<Real, DIM>(K, X.G[P_ZM], f[P_ZM]).daz) * K.half_over_dx;
+            Real Ex = (da_dx * ac.dax * q) + (a_val * d_dx * q) + (a_val * ac.dax * q_x);
+            Real Ey = (da_dy * ac.day * q) + (a_val * d_dy * q) + (a_val * ac.day * q_y);
+            Real Ez = (da_dz * ac.daz * q) + (a_val * d_dz * q) + (a_val * ac.daz * q_z);
+            tq = K.eps0_sq * (Ex + Ey + Ez);
+        } else {
+            Real c_x = dcx * X.dxx + dcy * X.dxy;
+            Real c_y = dcx * X.dxy + dcy * X.dyy;
+            Real da_dx = K.mu_delta * coef * c_x;
+            Real da_dy = K.mu_delta * coef * c_y;
+            Real Ex = (da_dx * ac.dax * q) + (a_val * d_dx * q) + (a_val * ac.dax * q_x);
+            Real Ey = (da_dy * ac.day * q) + (a_val * d_dy * q) + (a_val * ac.day * q_y);
+            tq = K.eps0_sq * (Ex + Ey);
+        }
+    }
+    if (K.torque) lap_val += tq;
+    lap_sl = lap_val;
+    // b_sl: kinetic anisotropy from the liquid gradient at the centre
+    b_sl = calc_b(K, X.G[P_C].ok ? f[P_C].ca : Real(0.0));
+}
+
+// Read/write a small register array at a run-time index without dynamic
+// register indexing (which would move the array to scratch memory).
+template <int NA, bool SMALL>
+__device__ __forceinline__ int sel_int(const int *a, int t)
+{
+    if constexpr (!SMALL) return a[t];
+    int r = a[0];
+#pragma unroll
+    for (int u = 1; u < NA; ++u)
+        if (u == t) r = a[u];
+    return r;
+}
+template <typename Real, int NA, bool SMALL>
+__device__ __forceinline__ void put_real(Real *a, int t, Real v)
+{
+    if constexpr (!SMALL) {
+        a[t] = v;
+    } else {
+#pragma unroll
+        for (int u = 0; u < NA; ++u)
+            if (u == t) a[u] = v;
+    }
+}
+
+// Active set of one cell in gathered form.  Index 0 is the liquid (present iff
+// liq), indices 1..nsol are the active solids in ascending id order.
+template <typename Real, int DIM, int A>
+struct Cell {
+    static constexpr int NP = (DIM == 3) ? 7 : 5;
+    static constexpr int NA = A + 1;
+    bool liq;
+    int nsol;
+    int gid[NA];       // gid[0] = 0
+    Real v[NA][NP];    // values at the stencil points
+    Real dom1[NP];     // 2D only: values of phase 1 (default dominant grain)
+    Real out[NA];      // new values after clip + renormalisation
+};
+
+// The update of one cell: src/gpu_kernels_3d.py:550-680 / :699-830 (3D) and
+// src/gpu_kernels.py:386-515 (2D).  Tcell is the (already cooled) temperature.
+// Cells with many active phases (A large) keep the arrays in scratch memory.
+template <typename Real, int DIM, int A, typename LiqAt>
+__device__ __forceinline__ void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, int k,
+                                            double Tcell, Cell<Real, DIM, A> &S)
+{
+    constexpr int NA = A + 1;
+    constexpr int NP = Cell<Real, DIM, A>::NP;
+    constexpr bool SMALL = (A <= 8);
+    constexpr int UNR_NA = SMALL ? NA : 1;
+    constexpr int UNR_A = SMALL ? A : 1;
+    const int n_act = (S.liq ? 1 : 0) + S.nsol;
+    Real lap_sl[NA], b_sl[NA], lap[NA];
+#pragma unroll UNR_NA
+    for (int t = 0; t < NA; ++t) {
+        lap_sl[t] = Real(0.0);
+        b_sl[t] = Real(1.0);
+    }
+
+    // Laplacian of every active phase (laplacian_3d, :444-459 / inline :620-625)
+#pragma unroll UNR_NA
+    for (int t = 0; t < NA; ++t) {
+        const Real *w = S.v[t];
+        if constexpr (DIM == 3)
+            lap[t] = (w[P_XP] + w[P_XM] + w[P_YP] + w[P_YM] + w[P_ZP] + w[P_ZM] - Real(6.0) * w[P_C]) *
+                     K.inv_dx2;
+        else
+            lap[t] = (w[P_XP] + w[P_XM] + w[P_YP] + w[P_YM] - Real(4.0) * w[P_C]) * K.inv_dx2;
+    }
+
+    // Per-solid terms are read only in solid-liquid pairs, so they are needed
+    // only when the liquid is active (the reference computes them regardless).
+    Real w_sl = K.w0_sl;
+    if (S.liq && S.nsol > 0) {
+        if (K.aniso || DIM == 2) {
+            LiqCtx<Real, DIM> X;
+            build_liq_ctx<Real, DIM>(K, L, l, m, k, X);
+            for (int t = 1; t <= S.nsol; ++t) {
+                int g = sel_int<NA, SMALL>(S.gid, t);
+                Real ls, bs;
+                solid_liquid_terms<Real, DIM>(K, X, g, ls, bs);
+                put_real<Real, NA, SMALL>(lap_sl, t, ls);
+                put_real<Real, NA, SMALL>(b_sl, t, bs);
+            }
+        } else {
+            // anisotropy off (switchable kernel, :729-731): eps0^2 * lap(phi_L)
+#pragma unroll UNR_A
+            for (int t = 1; t < NA; ++t) {
+                if (t > S.nsol) break;
+                lap_sl[t] = K.eps0_sq * lap[0];
+                b_sl[t] = Real(1.0);
+            }
+        }
+
+        // dominant solid and anisotropic w_sl (:579-595; 2D :410-436)
+        if constexpr (DIM == 3) {
+            if (K.aniso) {
+                int ts = -1;
+                Real maxv = Real(-1.0);
+#pragma unroll UNR_A
+                for (int t = 1; t < NA; ++t) {
+                    if (t > S.nsol) break;
+                    if (S.v[t][P_C] > maxv) {
+                        maxv = S.v[t][P_C];
+                        ts = t;
+                    }
+                }
+                if (ts >= 0) {
+                    Real sx = Real(0.0), sy = Real(0.0), sz = Real(0.0);
+                    int gs = 0;
+#pragma unroll UNR_A
+                    for (int t = 1; t < NA; ++t) {
+                        if (t == ts) {
+                            sx = (S.v[t][P_XP] - S.v[t][P_XM]) / K.two_dx;
+                            sy = (S.v[t][P_YP] - S.v[t][P_YM]) / K.two_dx;
+                            sz = (S.v[t][P_ZP] - S.v[t][P_ZM]) / K.two_dx;
+                            gs = S.gid[t];
+                        }
+                    }
+                    Real g2 = sx * sx + sy * sy + sz * sz;
+                    Real cmax = Real(0.0);
+                    if (!(g2 < K.g2_floor)) {
+                        Real inv_g = Real(1.0) / m_sqrt(g2);
+                        Facet<Real> f = facet_search<Real, 3>(K.n111 + (size_t)gs * 24, sx * inv_g,
+                                                              sy * inv_g, sz * inv_g);
+                        cmax = f.ca;
+                    }
+                    Real a_loc = calc_a(K, cmax);
+                    w_sl = K.w0_sl * (a_loc * a_loc);
+                }
+            }
+        } else {
+            int ts = -1;
+            int gs = 1;
+            Real maxv = Real(0.0);
+#pragma unroll UNR_A
+            for (int t = 1; t < NA; ++t) {
+                if (t > S.nsol) break;
+                if (S.v[t][P_C] > maxv) {
+                    maxv = S.v[t][P_C];
+                    ts = t;
+                }
+            }
+            Real wxp = S.dom1[P_XP], wxm = S.dom1[P_XM], wyp = S.dom1[P_YP], wym = S.dom1[P_YM];
+#pragma unroll UNR_A
+            for (int t = 1; t < NA; ++t) {
+                if (t == ts) {
+                    wxp = S.v[t][P_XP]; wxm = S.v[t][P_XM]; wyp = S.v[t][P_YP]; wym = S.v[t][P_YM];
+                    gs = S.gid[t];
+                }
+            }
+            Real phix_ = (wxp - wxm) * K.half_over_dx;
+            Real phiy_ = (wyp - wym) * K.half_over_dx;
+            Real gn2 = phix_ * phix_ + phiy_ * phiy_;
+            Real cmax = Real(0.0);
+            if (gn2 >= K.g2_floor) {
+                Real inv_gn = Real(1.0) / m_sqrt(gn2);
+                Facet<Real> f = facet_search<Real, 2>(K.n111 + (size_t)gs * 24, phix_ * inv_gn,
+                                                      phiy_ * inv_gn, Real(0.0));
+                cmax = f.ca;
+            }
+            Real a_loc = calc_a(K, cmax);
+            w_sl = K.w0_sl * (a_loc * a_loc);
+        }
+    }
+
+    // Pairwise evolution (:600-661)
+    const double dG = K.Sf * (Tcell - K.T_melt);
+    const Real df_sl = Real(-dG), df_ls = Real(dG);
+    const Real n_real = Real(n_act);
+    const int N = K.N;
+#pragma unroll UNR_NA
+    for (int t1 = 0; t1 < NA; ++t1) {
+        S.out[t1] = Real(0.0);
+        const bool p1 = (t1 == 0) ? S.liq : (t1 <= S.nsol);
+        if (!p1) continue;
+        const int i = S.gid[t1];
+        Real dpi = Real(0.0);
+#pragma unroll UNR_NA
+        for (int t2 = 0; t2 < NA; ++t2) {
+            const bool p2 = (t2 == 0) ? S.liq : (t2 <= S.nsol);
+            if (!p2 || t2 == t1) continue;
+            const int j = S.gid[t2];
+            Real driving_force = Real(0.0);
+            if (i != 0 && j == 0)
+                driving_force = df_sl;
+            else if (i == 0 && j != 0)
+                driving_force = df_ls;
+            Real ppp = Real(0.0);
+#pragma unroll UNR_NA
+            for (int t3 = 0; t3 < NA; ++t3) {
+                const bool p3 = (t3 == 0) ? S.liq : (t3 <= S.nsol);
+                if (!p3) continue;
+                const int pk = S.gid[t3];
+                const Real phi_k = S.v[t3][P_C];
+                Real wik = K.wij[(size_t)i * N + pk];
+                Real wjk = K.wij[(size_t)j * N + pk];
+                if (pk == 0) {
+                    if (i != 0) wik = w_sl;
+                    if (j != 0) wjk = w_sl;
+                }
+                Real term1 = (wik - wjk) * phi_k;
+                Real term2 = Real(0.0);
+                if (j == 0 && i != 0) {
+                    if (pk == i) term2 = Real(0.5) * lap_sl[t1];
+                } else if (i == 0 && j != 0) {
+                    if (pk == j) term2 = Real(-0.5) * lap_sl[t2];
+                } else {
+                    Real aik = K.aij[(size_t)i * N + pk], ajk = K.aij[(size_t)j * N + pk];
+                    Real eps2ik = aik * aik;
+                    Real eps2jk = ajk * ajk;
+                    term2 = Real(0.5) * (eps2ik - eps2jk) * lap[t3];
+                }
+                ppp += term1 + term2;
+            }
+            Real phii_phij = S.v[t1][P_C] * S.v[t2][P_C];
+            Real term_force = Real(8.0 / 3.1415926535) * m_sqrt(pymax(phii_phij, Real(0.0))) * driving_force;
+            Real mij_eff = K.mij[(size_t)i * N + j];
+            if (i != 0 && j == 0)
+                mij_eff = K.mij[(size_t)i * N + j] * b_sl[t1];
+            else if (i == 0 && j != 0)
+                mij_eff = K.mij[(size_t)i * N + j] * b_sl[t2];
+            dpi -= Real(2.0) * mij_eff / n_real * (ppp - term_force);
+        }
+        S.out[t1] = S.v[t1][P_C] + K.dt * dpi;
+    }
+
+    // Clip and renormalise (:663-680)
+    Real s = Real(0.0);
+#pragma unroll UNR_NA
+    for (int t = 0; t < NA; ++t) {
+        const bool pr = (t == 0) ? S.liq : (t <= S.nsol);
+        if (!pr) continue;
+        Real v = S.out[t];
+        if (v < Real(0.0)) v = Real(0.0);
+        if (v > Real(1.0)) v = Real(1.0);
+        S.out[t] = v;
+        s += v;
+    }
+    if (s > Real(1e-20)) {
+        Real invs = Real(1.0) / s;
+#pragma unroll UNR_NA
+        for (int t = 0; t < NA; ++t) {
+            const bool pr = (t == 0) ? S.liq : (t <= S.nsol);
+            if (pr) S.out[t] *= invs;
+        }
+    } else {
+        S.out[0] = Real(1.0);
+    }
+}
+
+} // namespace mpf

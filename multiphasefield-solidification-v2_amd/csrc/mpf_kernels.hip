@@ -1,0 +1,872 @@
+// mpf_kernels.hip -- HIP kernels of the multi-phase-field step for gfx950.
+//
+// This translation unit is compiled twice: with MPF_REAL=double and
+// -ffp-contract=off (the FP64 solver, bit-identical to the CPU oracle) and
+// with MPF_REAL=float (the FP32 solver, contraction allowed).  Each build
+// exports plain launcher functions named with the MPF_SFX suffix (_f64/_f32).
+//
+// Kernels
+//   k_step        fused temperature + APT + RHS + update over the slot layout
+//                 (src/gpu_kernels_3d.py:462-486, :489-527, :530-680,
+//                 :683-830, :833-838); one thread per cell, cells with more
+//                 than A active solids are queued for k_step_general
+//   k_step_general the same update with room for KMAX phases (scratch arrays)
+//   k_apt_slots   checked APT of the slot state into the reference's dense
+//                 mf/nf layout (parity export)
+//   k_pack/k_unpack dense reference layout <-> slots
+//   k_init_columns slot state from a grain map and per-column z profiles
+//   k_dense_*     the reference's three kernels on its own dense layout
+//                 (drop-in entry points)
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mpf_state.h"
+
+#ifndef MPF_REAL
+#error "MPF_REAL must be float or double"
+#endif
+#define MPF_CAT2(a, b) a##b
+#define MPF_CAT(a, b) MPF_CAT2(a, b)
+#define MPF_NAME(n) MPF_CAT(n, MPF_SFX)
+
+using Real = MPF_REAL;
+
+// Everything below lives in a precision-specific namespace so that the two
+// builds of this file can be linked into one library.
+namespace mpf {
+namespace MPF_CAT(impl, MPF_SFX) {
+
+constexpr int FAST_A = 4;   // active solids held in registers on the fast path
+constexpr int BX = 64;      // tile width in x (one wavefront per row)
+constexpr int BY = 4;       // tile rows in y
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ size_t cidx(int l, int m, int k, int nx, int ny)
+{
+    return ((size_t)k * ny + m) * nx + l;
+}
+
+// Reference dense index phi[g][l][m][k] (3D) / phi[g][l][m] (2D).
+__device__ __forceinline__ size_t ridx(int l, int m, int k, int ny, int nz)
+{
+    return ((size_t)l * ny + m) * nz + k;
+}
+
+// Stencil point cell indices in the slot layout: c, x+, x-, y+, y-, (z+, z-).
+template <int DIM>
+__device__ __forceinline__ void stencil_cells(const Consts<Real> &K, int l, int m, int k, size_t *pc)
+{
+    int lp = bc_pp(l, K.nx), lm = bc_pm(l, K.nx);
+    if constexpr (DIM == 3) {
+        int mp = bc_pp(m, K.ny), mm = bc_pm(m, K.ny);
+        int kp = bc_mp(k, K.nz), km = bc_mm(k, K.nz);
+        pc[P_C] = cidx(l, m, k, K.nx, K.ny);
+        pc[P_XP] = cidx(lp, m, k, K.nx, K.ny);
+        pc[P_XM] = cidx(lm, m, k, K.nx, K.ny);
+        pc[P_YP] = cidx(l, mp, k, K.nx, K.ny);
+        pc[P_YM] = cidx(l, mm, k, K.nx, K.ny);
+        pc[P_ZP] = cidx(l, m, kp, K.nx, K.ny);
+        pc[P_ZM] = cidx(l, m, km, K.nx, K.ny);
+    } else {
+        int mp = bc_mp(m, K.ny), mm = bc_mm(m, K.ny);
+        pc[P_C] = cidx(l, m, 0, K.nx, K.ny);
+        pc[P_XP] = cidx(lp, m, 0, K.nx, K.ny);
+        pc[P_XM] = cidx(lm, m, 0, K.nx, K.ny);
+        pc[P_YP] = cidx(l, mp, 0, K.nx, K.ny);
+        pc[P_YM] = cidx(l, mm, 0, K.nx, K.ny);
+    }
+}
+
+struct SlotLiq {
+    const Real *liq;
+    int nx, ny;
+    __device__ __forceinline__ Real operator()(int l, int m, int k) const
+    {
+        return liq[cidx(l, m, k, nx, ny)];
+    }
+};
+
+struct DenseLiq {
+    const Real *phi; // phase 0 of the dense field
+    int ny, nz;
+    __device__ __forceinline__ Real operator()(int l, int m, int k) const
+    {
+        return phi[ridx(l, m, k, ny, nz)];
+    }
+};
+
+// k-way merge of the sorted slot lists of the stencil points.  Yields the
+// union of grain ids present at any of the points in ascending order with the
+// value at every point (0 where absent).
+template <int DIM>
+struct Merge {
+    static constexpr int NP = (DIM == 3) ? 7 : 5;
+    size_t pc[NP];
+    int cnt[NP], pos[NP];
+    uint32_t hid[NP];
+    Real hv[NP];
+
+    __device__ __forceinline__ void init(const SlotView<Real> &S, size_t ncell)
+    {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            cnt[p] = S.cnt[pc[p]];
+            pos[p] = 0;
+            hid[p] = NO_ID;
+            hv[p] = Real(0.0);
+            if (cnt[p] > 0) {
+                hid[p] = S.sid[pc[p]];
+                hv[p] = S.sval[pc[p]];
+            }
+        }
+        (void)ncell;
+    }
+
+    // Next candidate; returns NO_ID when all lists are exhausted.
+    __device__ __forceinline__ uint32_t next(const SlotView<Real> &S, size_t ncell, Real *vals)
+    {
+        uint32_t g = hid[0];
+#pragma unroll
+        for (int p = 1; p < NP; ++p) g = hid[p] < g ? hid[p] : g;
+        if (g == NO_ID) return NO_ID;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            if (hid[p] == g) {
+                vals[p] = hv[p];
+                pos[p] += 1;
+                if (pos[p] < cnt[p]) {
+                    size_t o = (size_t)pos[p] * ncell + pc[p];
+                    hid[p] = S.sid[o];
+                    hv[p] = S.sval[o];
+                } else {
+                    hid[p] = NO_ID;
+                }
+            } else {
+                vals[p] = Real(0.0);
+            }
+        }
+        return g;
+    }
+};
+
+// APT rule, src/gpu_kernels_3d.py:478-482: phi > 0 at the cell, or phi == 0
+// there and > 0 at a face neighbour.
+template <int NP>
+__device__ __forceinline__ bool apt_active(const Real *vals)
+{
+    if (vals[0] > Real(0.0)) return true;
+    if (!(vals[0] == Real(0.0))) return false;
+    bool any = false;
+#pragma unroll
+    for (int p = 1; p < NP; ++p) any = any || (vals[p] > Real(0.0));
+    return any;
+}
+
+// Gather the active set of one cell (APT with the checked KMAX clamp).
+// Returns the total number of active phases (before the clamp) in *ntot and
+// whether the kept solids fit in A (*fits).
+template <int DIM, int A>
+__device__ __forceinline__ void gather(const Consts<Real> &K, const SlotView<Real> &S, size_t ncell,
+                                       int l, int m, int k, Cell<Real, DIM, A> &C, int *ntot, bool *fits)
+{
+    constexpr int NP = Cell<Real, DIM, A>::NP;
+    constexpr bool SMALL = (A <= 8);
+    Merge<DIM> M;
+    stencil_cells<DIM>(K, l, m, k, M.pc);
+    // liquid (dense)
+    Real lv[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) lv[p] = S.liq[M.pc[p]];
+    C.liq = apt_active<NP>(lv);
+    C.gid[0] = 0;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        C.v[0][p] = lv[p];
+        C.dom1[p] = Real(0.0);
+    }
+    M.init(S, ncell);
+    int keep = K.kmax - (C.liq ? 1 : 0); // solids kept under the KMAX clamp
+    int nact = 0;                        // active solids found
+    C.nsol = 0;
+    Real vals[NP];
+    if constexpr (SMALL) {
+        // unrolled over output positions so that the arrays stay in registers
+#pragma unroll
+        for (int u = 1; u <= A; ++u) {
+            if (nact < keep) {
+                uint32_t g;
+                bool act = false;
+                while ((g = M.next(S, ncell, vals)) != NO_ID) {
+                    if constexpr (DIM == 2) {
+                        if (g == 1) {
+#pragma unroll
+                            for (int p = 0; p < NP; ++p) C.dom1[p] = vals[p];
+                        }
+                    }
+                    if (apt_active<NP>(vals)) {
+                        act = true;
+                        break;
+                    }
+                }
+                if (act) {
+                    nact += 1;
+                    C.nsol = u;
+                    C.gid[u] = (int)g;
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) C.v[u][p] = vals[p];
+                }
+            }
+        }
+    }
+    // count (and, for the general path, store) the remaining active solids
+    uint32_t g;
+    while ((g = M.next(S, ncell, vals)) != NO_ID) {
+        if constexpr (DIM == 2) {
+            if (g == 1) {
+#pragma unroll
+                for (int p = 0; p < NP; ++p) C.dom1[p] = vals[p];
+            }
+        }
+        if (!apt_active<NP>(vals)) continue;
+        nact += 1;
+        if constexpr (!SMALL) {
+            if (nact <= keep && nact <= A) {
+                C.nsol = nact;
+                C.gid[nact] = (int)g;
+                for (int p = 0; p < NP; ++p) C.v[nact][p] = vals[p];
+            }
+        }
+    }
+    *ntot = nact + (C.liq ? 1 : 0);
+    int kept = nact < keep ? nact : keep;
+    *fits = (kept <= A);
+}
+
+// Wave-level statistics.  All 64 lanes must be active.  The maximum is
+// monotone, so a wave only issues the atomic when it would raise it.
+__device__ __forceinline__ void note_stats(Stats *st, int ntot, int kmax, int step, int halt_mode)
+{
+    int mx = ntot;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        int o = __shfl_xor(mx, off, 64);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0 && mx > *((volatile int *)&st->max_active)) atomicMax(&st->max_active, mx);
+    if (ntot > kmax) {
+        atomicMax(&st->kmax_overflow, 1);
+        atomicAdd(&st->overflow_cells, 1ull);
+        if (halt_mode) atomicMin(&st->halt_step, step);
+    }
+}
+
+// Append the flagged lanes of a wave to the worklist with one atomic.
+__device__ __forceinline__ void push_worklist(bool flag, int cell, int *worklist, int *count)
+{
+    unsigned long long ball = __ballot(flag);
+    if (ball == 0ull) return;
+    int lane = threadIdx.x & 63;
+    int base = 0;
+    if (lane == __ffsll((long long)ball) - 1) base = atomicAdd(count, __popcll(ball));
+    base = __shfl(base, __ffsll((long long)ball) - 1, 64);
+    if (flag) {
+        unsigned long long below = ball & ((1ull << lane) - 1ull);
+        worklist[base + __popcll(below)] = cell;
+    }
+}
+
+template <int DIM, int A>
+__device__ __forceinline__ void write_cell(const SlotView<Real> &O, size_t ncell, size_t c,
+                                           const Cell<Real, DIM, A> &C)
+{
+    constexpr bool SMALL = (A <= 8);
+    constexpr int UNR_A = SMALL ? A : 1;
+    O.liq[c] = C.out[0];
+    int w = 0;
+#pragma unroll UNR_A
+    for (int u = 1; u <= A; ++u) {
+        if (u <= C.nsol && C.out[u] != Real(0.0)) {
+            size_t o = (size_t)w * ncell + c;
+            O.sid[o] = (uint16_t)C.gid[u];
+            O.sval[o] = C.out[u];
+            w += 1;
+        }
+    }
+    O.cnt[c] = (uint8_t)w;
+}
+
+// Tile -> cells.  Consecutive blocks on one XCD (blocks are dealt round-robin
+// over the 8 XCDs) get consecutive tiles so that halos are reused in that
+// XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb)
+{
+    if ((nb & 7) != 0) return b;
+    return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(BX * BY) void k_step(StepArgs<Real> a)
+{
+    if (a.halt_mode && a.stats->halt_step >= 0 && a.stats->halt_step < a.step) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.wl_count[(a.step + 1) & 1] = 0;
+    const Consts<Real> &K = a.K;
+    int lb = xcd_remap(blockIdx.x, gridDim.x);
+    int tx = lb % a.ntx;
+    int rest = lb / a.ntx;
+    int ty = rest % a.nty;
+    int k = rest / a.nty;
+    int l = tx * BX + (threadIdx.x % BX);
+    int m = ty * BY + (threadIdx.x / BX);
+    const bool valid = (l < K.nx) && (m < K.ny);
+    const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
+    const size_t c = valid ? cidx(l, m, k, K.nx, K.ny) : 0;
+
+    // temperature, src/gpu_kernels_3d.py:833-838 (applied before the update)
+    double Tn = 0.0;
+    Cell<Real, DIM, FAST_A> C;
+    int ntot = 0;
+    bool fits = true;
+    if (valid) {
+        size_t ti = a.temp_dense ? c : (size_t)k;
+        double Tin = a.T_in[ti];
+        Tn = a.temp_f32 ? (double)((float)Tin - (float)a.cooling) : Tin - a.cooling;
+        if (a.temp_dense || (l == 0 && m == 0)) a.T_out[ti] = Tn;
+        gather<DIM, FAST_A>(K, a.in, ncell, l, m, k, C, &ntot, &fits);
+    }
+    note_stats(a.stats, ntot, K.kmax, a.step, a.halt_mode);
+    push_worklist(valid && !fits, (int)c, a.worklist, &a.wl_count[a.step & 1]);
+    if (!valid || !fits) return;
+    SlotLiq L{a.in.liq, K.nx, K.ny};
+    cell_update<Real, DIM, FAST_A>(K, L, l, m, k, Tn, C);
+    write_cell<DIM, FAST_A>(a.out, ncell, c, C);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
+{
+    if (a.halt_mode && a.stats->halt_step >= 0 && a.stats->halt_step < a.step) return;
+    const Consts<Real> &K = a.K;
+    const int n = a.wl_count[a.step & 1];
+    const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        size_t c = (size_t)a.worklist[i];
+        int l = (int)(c % K.nx);
+        int m = (int)((c / K.nx) % K.ny);
+        int k = (int)(c / ((size_t)K.nx * K.ny));
+        size_t ti = a.temp_dense ? c : (size_t)k;
+        double Tn = a.T_out[ti];
+        Cell<Real, DIM, KCAP> C;
+        int ntot;
+        bool fits;
+        gather<DIM, KCAP>(K, a.in, ncell, l, m, k, C, &ntot, &fits);
+        SlotLiq L{a.in.liq, K.nx, K.ny};
+        cell_update<Real, DIM, KCAP>(K, L, l, m, k, Tn, C);
+        write_cell<DIM, KCAP>(a.out, ncell, c, C);
+        if (i == 0) atomicAdd(&a.stats->slow_cells, (unsigned long long)n);
+    }
+}
+
+// Checked APT of the slot state in the reference's dense mf/nf layout
+// (kernel_update_nfmf_3d_checked, src/gpu_kernels_3d.py:489-527).
+template <int DIM>
+__global__ void k_apt_slots(Consts<Real> K, SlotView<Real> S, int32_t *mf, int32_t *nf, int32_t *status,
+                            int depth)
+{
+    constexpr int NP = (DIM == 3) ? 7 : 5;
+    const int nz = (DIM == 3) ? K.nz : 1;
+    const size_t ncell = (size_t)K.nx * K.ny * nz;
+    size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int l = (int)(c % K.nx);
+    int m = (int)((c / K.nx) % K.ny);
+    int k = (int)(c / ((size_t)K.nx * K.ny));
+    size_t r = (DIM == 3) ? ridx(l, m, k, K.ny, K.nz) : ((size_t)l * K.ny + m);
+    Merge<DIM> M;
+    stencil_cells<DIM>(K, l, m, k, M.pc);
+    Real lv[NP];
+    for (int p = 0; p < NP; ++p) lv[p] = S.liq[M.pc[p]];
+    int n = 0;
+    if (apt_active<NP>(lv)) {
+        n = 1;
+        if (n <= K.kmax && n <= depth) mf[(size_t)(n - 1) * ncell + r] = 0;
+    }
+    M.init(S, ncell);
+    Real vals[NP];
+    uint32_t g;
+    while ((g = M.next(S, ncell, vals)) != NO_ID) {
+        if (!apt_active<NP>(vals)) continue;
+        n += 1;
+        if (n <= K.kmax && n <= depth) mf[(size_t)(n - 1) * ncell + r] = (int32_t)g;
+    }
+    if (n > K.kmax) {
+        atomicMax(&status[0], 1);
+        nf[r] = K.kmax;
+    } else {
+        nf[r] = n;
+    }
+    atomicMax(&status[1], n);
+}
+
+// Dense reference layout -> slots.  Cells holding more non-zero solids than
+// the slot capacity raise err[0].
+template <typename In>
+__global__ void k_pack(const In *phi, int N, int nx, int ny, int nz, SlotView<Real> S, int scap, int *err)
+{
+    const size_t ncell = (size_t)nx * ny * nz;
+    size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int l = (int)(c % nx);
+    int m = (int)((c / nx) % ny);
+    int k = (int)(c / ((size_t)nx * ny));
+    size_t r = ridx(l, m, k, ny, nz);
+    S.liq[c] = (Real)phi[r];
+    int w = 0;
+    for (int g = 1; g < N; ++g) {
+        Real v = (Real)phi[(size_t)g * ncell + r];
+        if (v != Real(0.0)) {
+            if (w < scap) {
+                S.sid[(size_t)w * ncell + c] = (uint16_t)g;
+                S.sval[(size_t)w * ncell + c] = v;
+            } else {
+                atomicMax(err, 1);
+            }
+            w += 1;
+        }
+    }
+    S.cnt[c] = (uint8_t)(w < scap ? w : scap);
+}
+
+// Slots -> dense reference layout (out must be zero-filled).
+template <typename Out>
+__global__ void k_unpack(SlotView<Real> S, int nx, int ny, int nz, Out *phi)
+{
+    const size_t ncell = (size_t)nx * ny * nz;
+    size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int l = (int)(c % nx);
+    int m = (int)((c / nx) % ny);
+    int k = (int)(c / ((size_t)nx * ny));
+    size_t r = ridx(l, m, k, ny, nz);
+    phi[r] = (Out)S.liq[c];
+    int n = S.cnt[c];
+    for (int s = 0; s < n; ++s) {
+        size_t o = (size_t)s * ncell + c;
+        phi[(size_t)S.sid[o] * ncell + r] = (Out)S.sval[o];
+    }
+}
+
+// Columnar initial condition: cell (l,m,k) holds grain gmap[l,m] with value
+// solid[prof[l,m]][k] and liquid liquid[prof[l,m]][k] (the seed layers of
+// src/seed_modes_3d.py:38-57 and :117-166).
+__global__ void k_init_columns(const int32_t *gmap, const int32_t *prof, const double *solid,
+                               const double *liquid, int nx, int ny, int nz, SlotView<Real> S)
+{
+    const size_t ncell = (size_t)nx * ny * nz;
+    size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int l = (int)(c % nx);
+    int m = (int)((c / nx) % ny);
+    int k = (int)(c / ((size_t)nx * ny));
+    size_t col = (size_t)l * ny + m;
+    int pi = prof[col];
+    Real sv = (Real)solid[(size_t)pi * nz + k];
+    S.liq[c] = (Real)liquid[(size_t)pi * nz + k];
+    int g = gmap[col];
+    if (sv != Real(0.0) && g > 0) {
+        S.sid[c] = (uint16_t)g;
+        S.sval[c] = sv;
+        S.cnt[c] = 1;
+    } else {
+        S.cnt[c] = 0;
+    }
+}
+
+// Temperature between layouts: dense reference [l][m][k] <-> slot order.
+__global__ void k_temp_to_cells(const double *Tref, int nx, int ny, int nz, double *Tc)
+{
+    const size_t ncell = (size_t)nx * ny * nz;
+    size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int l = (int)(c % nx);
+    int m = (int)((c / nx) % ny);
+    int k = (int)(c / ((size_t)nx * ny));
+    Tc[c] = Tref[ridx(l, m, k, ny, nz)];
+}
+
+__global__ void k_temp_from_cells(const double *Tc, int nx, int ny, int nz, int dense, double *Tref)
+{
+    const size_t ncell = (size_t)nx * ny * nz;
+    size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int l = (int)(c % nx);
+    int m = (int)((c / nx) % ny);
+    int k = (int)(c / ((size_t)nx * ny));
+    Tref[ridx(l, m, k, ny, nz)] = dense ? Tc[c] : Tc[k];
+}
+
+// ------------------------------------------------- dense drop-in kernels
+// kernel_update_nfmf_3d / _checked (src/gpu_kernels_3d.py:462-527) and
+// kernel_update_nfmf (src/gpu_kernels.py:337-356) on phi[N][nx][ny](nz).
+// Threads run along k (the contiguous index) for coalescing.
+template <int DIM>
+__global__ void k_dense_apt(const Real *phi, int32_t *mf, int32_t *nf, int32_t *status, int N, int nx,
+                            int ny, int nz, int depth, int kmax, int checked)
+{
+    const int NZ = (DIM == 3) ? nz : 1;
+    const size_t ncell = (size_t)nx * ny * NZ;
+    size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ncell) return;
+    int k = (int)(r % NZ);
+    int m = (int)((r / NZ) % ny);
+    int l = (int)(r / ((size_t)NZ * ny));
+    int lp = bc_pp(l, nx), lm = bc_pm(l, nx);
+    int mp, mm, kp = k, km = k;
+    if (DIM == 3) {
+        mp = bc_pp(m, ny); mm = bc_pm(m, ny);
+        kp = bc_mp(k, nz); km = bc_mm(k, nz);
+    } else {
+        mp = bc_mp(m, ny); mm = bc_mm(m, ny);
+    }
+    size_t o_xp = ((size_t)lp * ny + m) * NZ + k, o_xm = ((size_t)lm * ny + m) * NZ + k;
+    size_t o_yp = ((size_t)l * ny + mp) * NZ + k, o_ym = ((size_t)l * ny + mm) * NZ + k;
+    size_t o_zp = ((size_t)l * ny + m) * NZ + kp, o_zm = ((size_t)l * ny + m) * NZ + km;
+    int n = 0;
+    for (int i = 0; i < N; ++i) {
+        const Real *q = phi + (size_t)i * ncell;
+        Real v = q[r];
+        bool act = v > Real(0.0);
+        if (!act && v == Real(0.0)) {
+            act = (q[o_xp] > Real(0.0)) || (q[o_xm] > Real(0.0)) || (q[o_yp] > Real(0.0)) ||
+                  (q[o_ym] > Real(0.0));
+            if (DIM == 3) act = act || (q[o_zp] > Real(0.0)) || (q[o_zm] > Real(0.0));
+        }
+        if (act) {
+            n += 1;
+            if ((!checked || n <= kmax) && n <= depth) mf[(size_t)(n - 1) * ncell + r] = i;
+        }
+    }
+    if (checked && n > kmax) {
+        atomicMax(&status[0], 1);
+        nf[r] = kmax;
+    } else {
+        nf[r] = n;
+    }
+    if (checked) atomicMax(&status[1], n);
+}
+
+// kernel_update_phasefield_active_3d(_switchable) (src/gpu_kernels_3d.py:530-830)
+// and kernel_update_phasefield_active (src/gpu_kernels.py:361-515).
+template <int DIM, int A>
+__device__ __forceinline__ void dense_cell(const Consts<Real> &K, const Real *phi, Real *phi_new,
+                                           const int32_t *mf, int n_act, size_t r, int l, int m, int k,
+                                           double Tcell, size_t ncell)
+{
+    constexpr int NP = (DIM == 3) ? 7 : 5;
+    constexpr bool SMALL = (A <= 8);
+    constexpr int UNR_A = SMALL ? A : 1;
+    const int NZ = (DIM == 3) ? K.nz : 1;
+    int lp = bc_pp(l, K.nx), lm = bc_pm(l, K.nx);
+    int mp, mm, kp = k, km = k;
+    if (DIM == 3) {
+        mp = bc_pp(m, K.ny); mm = bc_pm(m, K.ny);
+        kp = bc_mp(k, K.nz); km = bc_mm(k, K.nz);
+    } else {
+        mp = bc_mp(m, K.ny); mm = bc_mm(m, K.ny);
+    }
+    size_t off[7];
+    off[P_C] = r;
+    off[P_XP] = ((size_t)lp * K.ny + m) * NZ + k;
+    off[P_XM] = ((size_t)lm * K.ny + m) * NZ + k;
+    off[P_YP] = ((size_t)l * K.ny + mp) * NZ + k;
+    off[P_YM] = ((size_t)l * K.ny + mm) * NZ + k;
+    off[P_ZP] = ((size_t)l * K.ny + m) * NZ + kp;
+    off[P_ZM] = ((size_t)l * K.ny + m) * NZ + km;
+    Cell<Real, DIM, A> C;
+    C.gid[0] = 0;
+    C.liq = false;
+    C.nsol = 0;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        C.v[0][p] = phi[off[p]];
+        C.dom1[p] = (K.N > 1) ? phi[ncell + off[p]] : Real(0.0);
+    }
+    // mf is ascending with the liquid first (APT order)
+    for (int t = 0; t < n_act; ++t) {
+        int g = mf[(size_t)t * ncell + r];
+        if (g == 0) {
+            C.liq = true;
+            continue;
+        }
+        int u = C.nsol + 1;
+        if constexpr (SMALL) {
+#pragma unroll
+            for (int w = 1; w <= A; ++w) {
+                if (w == u) {
+                    C.gid[w] = g;
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) C.v[w][p] = phi[(size_t)g * ncell + off[p]];
+                }
+            }
+        } else {
+            C.gid[u] = g;
+            for (int p = 0; p < NP; ++p) C.v[u][p] = phi[(size_t)g * ncell + off[p]];
+        }
+        C.nsol = u;
+    }
+    DenseLiq L{phi, K.ny, NZ};
+    cell_update<Real, DIM, A>(K, L, l, m, k, Tcell, C);
+    for (int g = 0; g < K.N; ++g) phi_new[(size_t)g * ncell + r] = Real(0.0);
+    phi_new[r] = C.out[0];
+#pragma unroll UNR_A
+    for (int u = 1; u <= A; ++u)
+        if (u <= C.nsol) phi_new[(size_t)C.gid[u] * ncell + r] = C.out[u];
+}
+
+template <int DIM>
+__global__ __launch_bounds__(128) void k_dense_update(Consts<Real> K, const Real *phi, Real *phi_new,
+                                                      const void *temp, int temp_f64, const int32_t *mf,
+                                                      const int32_t *nf, int depth)
+{
+    const int NZ = (DIM == 3) ? K.nz : 1;
+    const size_t ncell = (size_t)K.nx * K.ny * NZ;
+    size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ncell) return;
+    int k = (int)(r % NZ);
+    int m = (int)((r / NZ) % K.ny);
+    int l = (int)(r / ((size_t)NZ * K.ny));
+    double T = temp_f64 ? ((const double *)temp)[r] : (double)((const float *)temp)[r];
+    // nf may exceed the ids written to mf for the unchecked APT (undefined in
+    // the reference); only the stored ids are used.
+    int n_act = nf[r];
+    n_act = n_act < depth ? n_act : depth;
+    n_act = n_act < KCAP + 1 ? n_act : KCAP + 1;
+    if (n_act <= FAST_A)
+        dense_cell<DIM, FAST_A>(K, phi, phi_new, mf, n_act, r, l, m, k, T, ncell);
+    else
+        dense_cell<DIM, KCAP>(K, phi, phi_new, mf, n_act, r, l, m, k, T, ncell);
+}
+
+// kernel_update_temp_3d (src/gpu_kernels_3d.py:833-838)
+__global__ void k_dense_temp(void *temp, int temp_f64, double cooling, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (temp_f64)
+        ((double *)temp)[i] -= cooling;
+    else
+        ((float *)temp)[i] = ((float *)temp)[i] - (float)cooling;
+}
+
+// FP64 observables (tests/verification_3d.py:409-416, :612-654).  One block
+// per z-plane: per-grain sums with LDS atomics, scalar partials by a
+// deterministic block reduction into part[k][5]: max|sum-1|, min, max, NaN
+// count, sum of (1 - phi_L).  front[l][m] gets the topmost k with
+// 1 - phi_L > 0.5 from a separate pass.
+__global__ __launch_bounds__(256) void k_observe(SlotView<Real> S, int nx, int ny, int nz, int N,
+                                                 double *gvol, double *part)
+{
+    extern __shared__ double sh[];
+    double *vol = sh;                  // [N]
+    double *red = sh + N;              // [256*5]
+    const int k = blockIdx.x;
+    const size_t ncell = (size_t)nx * ny * nz;
+    for (int g = threadIdx.x; g < N; g += blockDim.x) vol[g] = 0.0;
+    __syncthreads();
+    double e_max = 0.0, p_min = 1e300, p_max = -1e300, nan_c = 0.0, solid = 0.0;
+    bool any_absent = false;
+    for (int i = threadIdx.x; i < nx * ny; i += blockDim.x) {
+        size_t c = (size_t)k * nx * ny + i;
+        double lq = (double)S.liq[c];
+        double sum = lq;
+        int n = S.cnt[c];
+        if (lq != lq) nan_c += 1.0;
+        p_min = lq < p_min ? lq : p_min;
+        p_max = lq > p_max ? lq : p_max;
+        atomicAdd(&vol[0], lq);
+        for (int s = 0; s < n; ++s) {
+            size_t o = (size_t)s * ncell + c;
+            double v = (double)S.sval[o];
+            sum += v;
+            if (v != v) nan_c += 1.0;
+            p_min = v < p_min ? v : p_min;
+            p_max = v > p_max ? v : p_max;
+            atomicAdd(&vol[S.sid[o]], v);
+        }
+        if (n + 1 < N) any_absent = true;
+        double e = fabs(sum - 1.0);
+        e_max = e > e_max ? e : e_max;
+        solid += 1.0 - lq;
+    }
+    if (any_absent) {
+        p_min = 0.0 < p_min ? 0.0 : p_min;
+        p_max = 0.0 > p_max ? 0.0 : p_max;
+    }
+    double *r = red + threadIdx.x * 5;
+    r[0] = e_max; r[1] = p_min; r[2] = p_max; r[3] = nan_c; r[4] = solid;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            double *o = red + (threadIdx.x + h) * 5;
+            r[0] = o[0] > r[0] ? o[0] : r[0];
+            r[1] = o[1] < r[1] ? o[1] : r[1];
+            r[2] = o[2] > r[2] ? o[2] : r[2];
+            r[3] += o[3];
+            r[4] += o[4];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 5; ++q) part[(size_t)k * 5 + q] = red[q];
+    for (int g = threadIdx.x; g < N; g += blockDim.x) atomicAdd(&gvol[g], vol[g]);
+}
+
+__global__ void k_front(const Real *liq, int nx, int ny, int nz, int32_t *front)
+{
+    size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= (size_t)nx * ny) return;
+    int l = (int)(col % nx), m = (int)(col / nx);
+    int f = -1;
+    for (int k = nz - 1; k >= 0; --k) {
+        double s = 1.0 - (double)liq[cidx(l, m, k, nx, ny)];
+        if (s > 0.5) {
+            f = k;
+            break;
+        }
+    }
+    front[(size_t)l * ny + m] = f;
+}
+
+} // namespace impl_*
+} // namespace mpf
+
+// ------------------------------------------------------------ launchers
+using namespace mpf;
+using namespace mpf::MPF_CAT(impl, MPF_SFX);
+
+static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+extern "C" {
+
+hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, int general_blocks,
+                                     hipStream_t st)
+{
+    StepArgs<Real> a = *args;
+    const int nz = dim == 3 ? a.K.nz : 1;
+    a.ntx = (a.K.nx + BX - 1) / BX;
+    a.nty = (a.K.ny + BY - 1) / BY;
+    unsigned grid = (unsigned)a.ntx * a.nty * nz;
+    if (dim == 3) {
+        hipLaunchKernelGGL(k_step<3>, dim3(grid), dim3(BX * BY), 0, st, a);
+        hipLaunchKernelGGL(k_step_general<3>, dim3(general_blocks), dim3(64), 0, st, a);
+    } else {
+        hipLaunchKernelGGL(k_step<2>, dim3(grid), dim3(BX * BY), 0, st, a);
+        hipLaunchKernelGGL(k_step_general<2>, dim3(general_blocks), dim3(64), 0, st, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t MPF_NAME(mpf_launch_apt_slots)(const Consts<Real> *K, SlotView<Real> S, int dim, int32_t *mf,
+                                          int32_t *nf, int32_t *status, int depth, hipStream_t st)
+{
+    size_t ncell = (size_t)K->nx * K->ny * (dim == 3 ? K->nz : 1);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_apt_slots<3>, dim3(nblk(ncell, 256)), dim3(256), 0, st, *K, S, mf, nf, status,
+                           depth);
+    else
+        hipLaunchKernelGGL(k_apt_slots<2>, dim3(nblk(ncell, 256)), dim3(256), 0, st, *K, S, mf, nf, status,
+                           depth);
+    return hipGetLastError();
+}
+
+hipError_t MPF_NAME(mpf_launch_pack)(const void *phi, int phi_f64, int N, int nx, int ny, int nz,
+                                     SlotView<Real> S, int scap, int *err, hipStream_t st)
+{
+    size_t ncell = (size_t)nx * ny * nz;
+    if (phi_f64)
+        hipLaunchKernelGGL(k_pack<double>, dim3(nblk(ncell, 256)), dim3(256), 0, st, (const double *)phi, N,
+                           nx, ny, nz, S, scap, err);
+    else
+        hipLaunchKernelGGL(k_pack<float>, dim3(nblk(ncell, 256)), dim3(256), 0, st, (const float *)phi, N,
+                           nx, ny, nz, S, scap, err);
+    return hipGetLastError();
+}
+
+hipError_t MPF_NAME(mpf_launch_unpack)(SlotView<Real> S, int nx, int ny, int nz, void *phi, int phi_f64,
+                                       hipStream_t st)
+{
+    size_t ncell = (size_t)nx * ny * nz;
+    if (phi_f64)
+        hipLaunchKernelGGL(k_unpack<double>, dim3(nblk(ncell, 256)), dim3(256), 0, st, S, nx, ny, nz,
+                           (double *)phi);
+    else
+        hipLaunchKernelGGL(k_unpack<float>, dim3(nblk(ncell, 256)), dim3(256), 0, st, S, nx, ny, nz,
+                           (float *)phi);
+    return hipGetLastError();
+}
+
+hipError_t MPF_NAME(mpf_launch_init_columns)(const int32_t *gmap, const int32_t *prof, const double *solid,
+                                             const double *liquid, int nx, int ny, int nz,
+                                             SlotView<Real> S, hipStream_t st)
+{
+    size_t ncell = (size_t)nx * ny * nz;
+    hipLaunchKernelGGL(k_init_columns, dim3(nblk(ncell, 256)), dim3(256), 0, st, gmap, prof, solid, liquid,
+                       nx, ny, nz, S);
+    return hipGetLastError();
+}
+
+hipError_t MPF_NAME(mpf_launch_temp_layout)(const double *src, double *dst, int nx, int ny, int nz,
+                                            int to_cells, int dense, hipStream_t st)
+{
+    size_t ncell = (size_t)nx * ny * nz;
+    if (to_cells)
+        hipLaunchKernelGGL(k_temp_to_cells, dim3(nblk(ncell, 256)), dim3(256), 0, st, src, nx, ny, nz, dst);
+    else
+        hipLaunchKernelGGL(k_temp_from_cells, dim3(nblk(ncell, 256)), dim3(256), 0, st, src, nx, ny, nz,
+                           dense, dst);
+    return hipGetLastError();
+}
+
+hipError_t MPF_NAME(mpf_launch_dense_apt)(const Real *phi, int32_t *mf, int32_t *nf, int32_t *status, int N,
+                                          int nx, int ny, int nz, int dim, int depth, int kmax, int checked,
+                                          hipStream_t st)
+{
+    size_t ncell = (size_t)nx * ny * (dim == 3 ? nz : 1);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_dense_apt<3>, dim3(nblk(ncell, 128)), dim3(128), 0, st, phi, mf, nf, status, N,
+                           nx, ny, nz, depth, kmax, checked);
+    else
+        hipLaunchKernelGGL(k_dense_apt<2>, dim3(nblk(ncell, 128)), dim3(128), 0, st, phi, mf, nf, status, N,
+                           nx, ny, nz, depth, kmax, checked);
+    return hipGetLastError();
+}
+
+hipError_t MPF_NAME(mpf_launch_dense_update)(const Consts<Real> *K, int dim, const Real *phi, Real *phi_new,
+                                             const void *temp, int temp_f64, const int32_t *mf,
+                                             const int32_t *nf, int depth, hipStream_t st)
+{
+    size_t ncell = (size_t)K->nx * K->ny * (dim == 3 ? K->nz : 1);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_dense_update<3>, dim3(nblk(ncell, 128)), dim3(128), 0, st, *K, phi, phi_new,
+                           temp, temp_f64, mf, nf, depth);
+    else
+        hipLaunchKernelGGL(k_dense_update<2>, dim3(nblk(ncell, 128)), dim3(128), 0, st, *K, phi, phi_new,
+                           temp, temp_f64, mf, nf, depth);
+    return hipGetLastError();
+}
+
+hipError_t MPF_NAME(mpf_launch_dense_temp)(void *temp, int temp_f64, double cooling, size_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dense_temp, dim3(nblk(n, 256)), dim3(256), 0, st, temp, temp_f64, cooling, n);
+    return hipGetLastError();
+}
+
+hipError_t MPF_NAME(mpf_launch_observe)(SlotView<Real> S, int nx, int ny, int nz, int N, double *gvol,
+                                        double *part, int32_t *front, hipStream_t st)
+{
+    size_t shm = ((size_t)N + 256 * 5) * sizeof(double);
+    hipLaunchKernelGGL(k_observe, dim3(nz), dim3(256), shm, st, S, nx, ny, nz, N, gvol, part);
+    hipLaunchKernelGGL(k_front, dim3(nblk((size_t)nx * ny, 256)), dim3(256), 0, st, S.liq, nx, ny, nz, front);
+    return hipGetLastError();
+}
+
+} // extern "C"

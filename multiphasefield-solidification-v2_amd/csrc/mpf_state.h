@@ -1,0 +1,62 @@
+// mpf_state.h -- device state layout and launch arguments shared by the HIP
+// kernels and the C-ABI host code.
+//
+// Slot layout (one buffer of the double-buffered state), cells ordered
+//   c = (k*ny + m)*nx + l          (x fastest, z slowest; 2D: k = 0)
+//   liq [c]                  liquid phi (phase 0), dense
+//   cnt [c]                  number of stored solid slots (uint8)
+//   sid [s*ncell + c]        grain id of slot s (uint16), ascending in s
+//   sval[s*ncell + c]        phi of that grain (non-zero)
+// A solid grain is stored at a cell only while its phi there is non-zero;
+// the reference's dense phi[N][nx][ny][nz] (src/gpu_kernels_3d.py) holds the
+// same information with zeros.  The slot capacity is min(KMAX, N-1), so it
+// cannot overflow: after a step only active phases (at most KMAX) are
+// non-zero.  Reads touch only slots [0, cnt), so the traffic is proportional
+// to the number of grains present, not to the capacity.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "mpf_cell.h"
+
+namespace mpf {
+
+constexpr int KCAP = 64;        // largest KMAX supported
+constexpr uint32_t NO_ID = 0xFFFFu;
+
+template <typename Real>
+struct SlotView {
+    Real *liq;
+    uint8_t *cnt;
+    uint16_t *sid;
+    Real *sval;
+};
+
+struct Stats {
+    int max_active;            // largest active count seen (status[1])
+    int kmax_overflow;         // 1 if any cell exceeded KMAX (status[0])
+    unsigned long long overflow_cells; // cells that exceeded KMAX (summed over steps)
+    unsigned long long slow_cells;     // cells sent to the general path
+    int halt_step;             // first step with a KMAX overflow (halt mode), else -1
+    int pad;
+};
+
+template <typename Real>
+struct StepArgs {
+    Consts<Real> K;
+    SlotView<Real> in, out;
+    const double *T_in;
+    double *T_out;
+    int temp_dense;  // 1: temperature per cell; 0: one value per z-plane
+    int temp_f32;    // 1: the temperature is a float32 quantity
+    double cooling;
+    int *worklist;   // cells left for the general path
+    int *wl_count;   // [2] ping-pong counters
+    int step;        // step index (for the ping-pong counters and halt)
+    int halt_mode;   // stop at the first KMAX overflow
+    Stats *stats;
+    int ntx, nty;    // tiles in x and y
+};
+
+} // namespace mpf

@@ -1,0 +1,42 @@
+"""CPU checks of the boundary: the library loads and exports every entry
+point include/mpf.h declares (no device calls)."""
+import os
+import re
+
+import pytest
+
+from mpf import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "mpf.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(mpf_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_matches_binding_table():
+    assert sorted(_lib.EXPORTS) == _declared()
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    for name in _declared():
+        assert hasattr(L, name), name
+
+
+def test_abi_version_and_errors_without_gpu():
+    L = _lib.lib()
+    assert L.mpf_abi_version() == 1
+    # argument validation happens before any device call
+    import ctypes as C
+    cfg = _lib.Config()
+    cfg.nx = cfg.ny = cfg.nz = 4
+    cfg.dim = 5
+    h = C.c_void_p()
+    rc = L.mpf_create(C.byref(cfg), C.byref(h))
+    assert rc == _lib.MPF_EINVAL
+    assert b"dim" in L.mpf_last_error()
+    with pytest.raises(_lib.MPFError):
+        _lib.check(rc)

@@ -1,0 +1,205 @@
+"""GPU parity of the HIP solver against the CPU oracle.
+
+FP64: the solver and the oracle (include/mpf_math.h transcendentals, no
+contraction) must agree bit for bit on phi, and the APT sets mf/nf must be
+identical (tolerance written below: exact).  FP32: max |phi - phi_oracle_fp64|
+<= 1e-4 over short runs.
+"""
+import numpy as np
+import pytest
+
+import cases
+from mpf import Solver
+from mpf import dense as D
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10   # the north-star bound; the checks below also demand bit-equality
+FP32_TOL = 1e-4
+
+
+def _solver_for(c, precision, **kw):
+    phi = c["phi"]
+    s = Solver(phi.shape[1:], phi.shape[0], c["scalars"], precision=precision, **kw)
+    s.set_tables(*c["tables"])
+    s.upload_phi(phi)
+    s.upload_temp(c["temp"])
+    return s
+
+
+def _assert_apt_equal(solver, phi_ref, kmax=50):
+    mf, nf, st = solver.apt(depth=kmax)
+    mf_o, nf_o, st_o = O.apt(phi_ref, kmax)
+    assert np.array_equal(nf, nf_o)
+    depth_mask = np.arange(kmax)[:, None, None, None] < nf[None] if nf.ndim == 3 else \
+        np.arange(kmax)[:, None, None] < nf[None]
+    assert np.array_equal(np.where(depth_mask, mf, -1), np.where(depth_mask, mf_o, -1))
+    assert np.array_equal(st, st_o)
+
+
+@pytest.mark.parametrize("aniso,torque", [(True, True), (True, False), (False, False)])
+def test_fp64_voronoi_bit_exact(aniso, torque):
+    c = cases.voronoi_case()
+    steps = 12
+    ref, T_ref, done, st = O.run(c["phi"], c["temp"], steps, c["tables"], c["scalars"], kmax=50,
+                                 aniso=aniso, torque=torque, temp_f32=True)
+    with _solver_for(c, "fp64", temp_f32=True, anisotropy=aniso, torque=torque) as s:
+        assert s.step(steps) == steps
+        got = s.phi()
+        print("max |diff| =", np.abs(got - ref).max())
+        assert np.abs(got - ref).max() <= FP64_TOL
+        assert np.array_equal(got, ref), "FP64 solver must be bit-identical to the oracle"
+        _assert_apt_equal(s, ref)
+        assert s.stats()["max_active"] == st[1]
+
+
+def test_fp64_directional_bit_exact():
+    """Seed layer, linear temperature, uniform cooling (run_randommode_3d.py)."""
+    dc = cases.directional_case(24, 20, 32, 6, seed_height=8, G=2.0e4)
+    gm, pi_, sol, liq = cases.inputs.columns_from_grain_map_3d(dc["gmap"], 32, dc["params"]["dx"],
+                                                               dc["derived"]["delta"], 8)
+    phi0 = cases.inputs.init_phi_from_grain_map_3d(dc["gmap"], 6, 24, 20, 32, dc["params"]["dx"],
+                                                   dc["derived"]["delta"], 8).astype(np.float64)
+    steps = 40
+    cool = dc["cooling"] * 50
+    ref, T_ref, done, st = O.run(phi0, dc["temp"], steps, dc["tables"], dc["scalars"], kmax=50,
+                                 cooling_rate=cool, temp_f32=False)
+    s = Solver((24, 20, 32), 7, dc["scalars"], precision="fp64", cooling_rate=cool)
+    s.set_tables(*dc["tables"])
+    s.init_columns(gm, pi_, sol, liq)
+    assert np.array_equal(s.phi(), phi0)     # columnar init == dense init
+    s.upload_temp(dc["temp"])
+    assert s.step(steps) == steps
+    got = s.phi()
+    assert np.array_equal(got, ref)
+    np.testing.assert_array_equal(s.temp(), T_ref[0, 0, :])
+    _assert_apt_equal(s, ref)
+    s.close()
+
+
+def test_fp64_2d_disk_bit_exact():
+    """Config-1 style 2D half disc (x periodic, y mirror), 200 steps."""
+    c = cases.disk2d_case()
+    steps = 200
+    ref, _, done, st = O.run(c["phi"], c["temp"], steps, c["tables"], c["scalars"], kmax=50)
+    with _solver_for(c, "fp64") as s:
+        assert s.step(steps) == steps
+        got = s.phi()
+        assert np.array_equal(got, ref)
+        _assert_apt_equal(s, ref)
+
+
+def test_fp32_within_tolerance():
+    c = cases.voronoi_case()
+    steps = 12
+    ref, _, _, _ = O.run(c["phi"], c["temp"], steps, c["tables"], c["scalars"], kmax=50, temp_f32=True)
+    with _solver_for(c, "fp32", temp_f32=True) as s:
+        assert s.step(steps) == steps
+        err = np.abs(s.phi() - ref).max()
+        print("fp32 max |diff| =", err)
+        assert err <= FP32_TOL
+
+
+def test_dense_dropin_kernels_bit_exact():
+    """The three drop-in kernels on the reference's dense layout, FP64."""
+    c = cases.voronoi_case()
+    phi = c["phi"]
+    N = phi.shape[0]
+    grid = phi.shape[1:]
+    temp = c["temp"].copy()
+    cool = 0.125
+    T_o = temp.copy()
+    O.lib().orc_temp(T_o.reshape(-1), T_o.size, cool, 0)
+    mf_o, nf_o, st_o = O.apt(phi, 50)
+    new_o = O.update(phi, T_o, mf_o, nf_o, c["tables"], c["scalars"])
+
+    d_phi = D.to_device(phi)
+    d_new = D.to_device(np.zeros_like(phi))
+    d_T = D.to_device(temp)
+    d_mf = D.to_device(np.zeros((50,) + grid, np.int32))
+    d_nf = D.to_device(np.zeros(grid, np.int32))
+    d_st = D.to_device(np.zeros(2, np.int32))
+    tabs = [D.to_device(np.ascontiguousarray(t, np.float64)) for t in c["tables"]]
+    D.temp(d_T, cool)
+    D.apt(d_phi, d_mf, d_nf, d_st, N, checked=True, kmax=50)
+    D.update(d_phi, d_new, d_T, d_mf, d_nf, *tabs, c["scalars"], N, kmax=50)
+    np.testing.assert_array_equal(d_T.copy_to_host(), T_o)
+    assert np.array_equal(d_nf.copy_to_host(), nf_o)
+    assert np.array_equal(d_st.copy_to_host(), st_o)
+    assert np.array_equal(d_new.copy_to_host(), new_o)
+
+
+def test_kmax_overflow_probe():
+    """tests/verification_3d.py:1459-1480: 52 phases at 1/52 in one cell of a
+    4^3 grid; the checked APT reports status [1, 52] (report :156)."""
+    N = 52
+    phi = np.zeros((N, 4, 4, 4), np.float64)
+    phi[0] = 1.0
+    phi[:, 1, 1, 1] = 1.0 / N
+    d_phi = D.to_device(phi)
+    d_mf = D.to_device(np.zeros((N, 4, 4, 4), np.int32))
+    d_nf = D.to_device(np.zeros((4, 4, 4), np.int32))
+    d_st = D.to_device(np.zeros(2, np.int32))
+    D.apt(d_phi, d_mf, d_nf, d_st, N, checked=True, kmax=50)
+    st = d_st.copy_to_host()
+    assert list(st) == [1, 52]
+    mf_o, nf_o, st_o = O.apt(phi, 50)
+    assert np.array_equal(d_nf.copy_to_host(), nf_o)
+    # the resident solver halts before updating (KMaxOverflowError semantics)
+    c = cases.voronoi_case()
+    p = dict(c["scalars"])
+    tabs = cases.tables_for(c["params"], np.tile([0, 0, 0, 1.0], (N, 1)))[1]
+    s = Solver((4, 4, 4), N, p, precision="fp64", halt_on_kmax=True, slot_capacity=N - 1)
+    s.set_tables(*tabs)
+    s.upload_phi(phi)
+    s.upload_temp(np.full(4, 1680.0))
+    assert s.step(3) == 0
+    st = s.stats()
+    assert st["kmax_overflow"] == 1 and st["max_active"] == 52 and st["halt_step"] == 0
+    assert np.array_equal(s.phi(), phi)
+    s.close()
+
+
+def test_general_path_many_phases():
+    """Cells with more active solids than the register fast path holds take
+    the general path and still match the oracle bit for bit."""
+    rng = np.random.default_rng(5)
+    N = 9
+    nx, ny, nz = 8, 8, 8
+    phi = np.zeros((N, nx, ny, nz))
+    phi[0] = 0.3
+    blob = rng.uniform(0.0, 1.0, (N - 1, 4, 4, 4))
+    phi[1:, 2:6, 2:6, 2:6] = blob
+    phi[1:] = np.where(phi[1:] < 0.35, 0.0, phi[1:])
+    s_ = phi.sum(0)
+    phi /= s_
+    c = cases.voronoi_case()
+    quats = np.vstack([[0, 0, 0, 1.0], rng.standard_normal((N - 1, 4))])
+    quats[1:] /= np.linalg.norm(quats[1:], axis=1, keepdims=True)
+    tabs = cases.tables_for(c["params"], quats)[1]
+    temp = np.full((nx, ny, nz), 1682.0)
+    ref, _, _, st = O.run(phi, temp, 5, tabs, c["scalars"], kmax=50)
+    s = Solver((nx, ny, nz), N, c["scalars"], precision="fp64")
+    s.set_tables(*tabs)
+    s.upload_phi(phi)
+    s.upload_temp(temp)
+    assert s.step(5) == 5
+    assert s.stats()["general_cells"] > 0
+    assert np.array_equal(s.phi(), ref)
+    s.close()
+
+
+def test_observables_match_numpy():
+    c = cases.voronoi_case()
+    with _solver_for(c, "fp64", temp_f32=True) as s:
+        s.step(4)
+        phi = s.phi()
+        ob = s.observables()
+        np.testing.assert_allclose(ob["phase_sums"], phi.reshape(phi.shape[0], -1).sum(1), rtol=1e-12)
+        assert ob["phi_sum_error_max"] == pytest.approx(np.abs(phi.sum(0) - 1).max(), abs=1e-15)
+        solid = 1.0 - phi[0]
+        mask = solid > 0.5
+        nz = phi.shape[3]
+        pos = np.where(mask.any(2), nz - 1 - np.argmax(mask[:, :, ::-1], axis=2), np.nan)
+        np.testing.assert_array_equal(ob["front"], pos)
