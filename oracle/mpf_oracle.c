@@ -928,3 +928,9 @@ int orc_run(int dim, double *phi, double *temp, int nsteps, int kmax,
     free(buf); free(mf); free(nf);
     return done;
 }
+
+/* The shared transcendentals, exported so the tests can compare them with
+ * the C library directly. */
+double orc_mpf_tan(double x) { return mpf_tan(x); }
+double orc_mpf_acos(double x) { return mpf_acos(x); }
+double orc_mpf_tanh(double x) { return mpf_tanh(x); }

@@ -61,6 +61,9 @@ def lib():
         L.orc_run.argtypes = [C.c_int, dp, dp, C.c_int, C.c_int, dp, dp, dp, dp, P,
                               C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, ip]
         L.orc_run.restype = C.c_int
+        for fn in ("orc_mpf_tan", "orc_mpf_acos", "orc_mpf_tanh"):
+            getattr(L, fn).argtypes = [C.c_double]
+            getattr(L, fn).restype = C.c_double
         _lib = L
     return _lib
 
