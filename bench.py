@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Headline benchmark: 3D random-mode plain method at 512 x 512 x 256 with 200
+Voronoi grains in a 24-cell seed layer, random orientations, uniform cooling,
+KMAX = 50 (config_3d.yaml physics).  One "step" is one full time step
+(temperature, APT, RHS and update) over the whole grid.
+
+Prints ONE JSON line: MLUPS of the FP32 solver (value), the FP64 solver's
+MLUPS and its agreement with FP32 on the FP64 observables, the HBM roofline
+of the fused step kernel, and a CPU baseline (the FP64 CPU oracle in the
+reference's dense layout, timed on a bounded slab of the same workload).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "multiphasefield-solidification-v2_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+from mpf import Solver, inputs  # noqa: E402
+from mpf.config import flat_params, load_config  # noqa: E402
+
+METRIC = "MLUPS at 512²×256 (FP32 vs FP64 plain) + surrogate speedup at ≤1-cell front err"
+HBM_PEAK_GBS = 8000.0
+NX, NY, NZ, N_SOLID, SEED_H = 512, 512, 256, 200, 24
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(nx=NX, ny=NY, nz=NZ, n_solid=N_SOLID, seed_height=SEED_H):
+    p = flat_params(load_config())
+    d = inputs.derive_constants(p)
+    gmap = inputs.generate_random_grain_map_3d(nx, ny, n_solid, random_seed=42)
+    quats = inputs.assign_quaternions_to_grains(n_solid, "random", 42)
+    n111 = inputs.compute_rotated_n111(quats)
+    w, a, m = inputs.build_interaction_matrices(n_solid + 1, d["eps0_sl"], d["w0_sl"], d["m_sl_phi"],
+                                                d["eps_gb"], d["w_gb"], d["m_gb_phi"])
+    cols = inputs.columns_from_grain_map_3d(gmap, nz, p["dz"], d["delta"], seed_height)
+    temp = inputs.init_temperature_field_3d(1, 1, nz, p["T_melt"], p["G"], p["dz"], seed_height)[0, 0]
+    return {"p": p, "d": d, "gmap": gmap, "tables": (w, a, m, n111), "cols": cols,
+            "temp": temp.astype(np.float64), "scalars": inputs.kernel_scalars(p, d, as_float32=True),
+            "cooling": float(np.float32(d["cooling_rate"])), "N": n_solid + 1}
+
+
+def run_gpu(W, precision, steps, warmup):
+    s = Solver((NX, NY, NZ), W["N"], W["scalars"], precision=precision, kmax=50, cooling_rate=W["cooling"])
+    try:
+        s.set_tables(*W["tables"])
+        s.init_columns(*W["cols"])
+        s.upload_temp(W["temp"])
+        if warmup:
+            s.step(warmup)
+        ob0 = s.observables(dx=W["p"]["dx"])
+        s.reset_stats()
+        t0 = time.perf_counter()
+        done = s.step(steps)
+        wall = time.perf_counter() - t0
+        st = s.stats()
+        ob1 = s.observables(dx=W["p"]["dx"])
+        return {"ms": st["last_step_ms"], "wall": wall, "done": done, "ob0": ob0, "ob1": ob1, "stats": st,
+                "bytes": s.device_bytes}
+    finally:
+        s.close()
+
+
+def alg_bytes_per_step(ob0, ob1, bv, ncell, nz):
+    """Compulsory HBM bytes of one step in the slot layout: every cell's liquid
+    value and slot count read once and written once, every stored grain slot
+    (id + value) read once and written once, the per-plane temperature."""
+    slots = 0.5 * (ob0["stored_slots"] + ob1["stored_slots"])
+    return 2.0 * ncell * (bv + 1) + 2.0 * slots * (bv + 2) + 2.0 * nz * 8, slots / ncell
+
+
+def cpu_baseline(W, steps=2):
+    """FP64 CPU oracle (reference dense layout, OpenMP over cells) on a
+    128 x 128 x 48 slab holding the seed layer and the front."""
+    from oracle import oracle as O
+
+    sx, sy, sz = 128, 128, 48
+    p, d = W["p"], W["d"]
+    N = W["N"]
+    phi = np.zeros((N, sx, sy, sz), np.float64)
+    prof = inputs.tanh_profile(sz, p["dz"], d["delta"], SEED_H)
+    sub = W["gmap"][:sx, :sy]
+    for l in range(sx):
+        for m in range(sy):
+            phi[sub[l, m], l, m] = prof
+    phi[0] = inputs.liquid_from_solid(prof)[None, None, :]
+    temp = np.broadcast_to(W["temp"][:sz], (sx, sy, sz)).copy()
+    tabs = tuple(np.asarray(t, np.float64) for t in W["tables"])
+    phi, temp, _, _ = O.run(phi, temp, 1, tabs, W["scalars"], kmax=50, cooling_rate=W["cooling"])
+    t0 = time.perf_counter()
+    O.run(phi, temp, steps, tabs, W["scalars"], kmax=50, cooling_rate=W["cooling"])
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(sx * sy * sz * steps / dt / 1e6, 4), "unit": "MLUPS", "cores": threads,
+            "kind": "port",
+            "sample": f"{sx}x{sy}x{sz} slab of the workload (seed layer + front), N={N}, FP64 oracle in the "
+                      f"reference's dense layout, {steps} steps after 1 untimed step, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the solver has no "
+              "multi-GPU path in this build.", file=sys.stderr)
+        sys.exit(2)
+
+    W = workload()
+    ncell = NX * NY * NZ
+    log(f"workload ready: {NX}x{NY}x{NZ}, N={W['N']}")
+    r32 = run_gpu(W, "fp32", args.steps, args.warmup)
+    ms32 = r32["ms"] / args.steps
+    log(f"fp32: {ms32:.3f} ms/step, stats {r32['stats']}")
+    B32, slot32 = alg_bytes_per_step(r32["ob0"], r32["ob1"], 4, ncell, NZ)
+    achieved = B32 / (ms32 * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC, "value": round(ncell * args.steps / (r32["ms"] * 1e-3) / 1e6, 2), "unit": "MLUPS",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms32, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "3D random mode, plain method, 512x512x256, 200 Voronoi grains, 24-cell seed "
+                               "layer, random quaternions, uniform cooling, KMAX=50",
+                   "nx": NX, "ny": NY, "nz": NZ, "n_solid": N_SOLID, "seed_height": SEED_H, "kmax": 50,
+                   "precision": "fp32", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "alg_bytes_per_step": round(B32), "mean_stored_slots_per_cell": round(slot32, 4),
+                     "kernel": "k_step (fused temperature+APT+RHS+update) + k_step_general"},
+        "wall_s_timed": round(r32["wall"], 3),
+        "front_mean_cells": r32["ob1"]["front_mean"],
+        "general_cells": r32["stats"]["general_cells"],
+        "max_active": r32["stats"]["max_active"],
+    }
+    if not args.no_fp64:
+        k64 = args.fp64_steps or args.steps
+        r64 = run_gpu(W, "fp64", k64, args.warmup)
+        ms64 = r64["ms"] / k64
+        log(f"fp64: {ms64:.3f} ms/step")
+        out["fp64_mlups"] = round(ncell * k64 / (r64["ms"] * 1e-3) / 1e6, 2)
+        out["fp64_ms_per_step"] = round(ms64, 4)
+        if k64 == args.steps:
+            v32 = r32["ob1"]["grain_volumes"]
+            v64 = r64["ob1"]["grain_volumes"]
+            f32 = v32 / v32.sum()
+            f64 = v64 / v64.sum()
+            out["fp32_vs_fp64"] = {
+                "front_mean_diff_cells": abs(r32["ob1"]["front_mean"] - r64["ob1"]["front_mean"]),
+                "front_max_abs_diff_cells": float(np.nanmax(np.abs(r32["ob1"]["front"] - r64["ob1"]["front"]))),
+                "grain_fraction_max_rel_diff": float(np.max(np.abs(f32 - f64) / np.maximum(f64, 1e-30))),
+                "solid_fraction_diff": abs(r32["ob1"]["solid_fraction"] - r64["ob1"]["solid_fraction"]),
+            }
+    if not args.no_cpu:
+        cb = cpu_baseline(W)
+        log(f"cpu baseline: {cb}")
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -123,7 +123,8 @@ int mpf_reset_stats(mpf_handle *h);
 /* FP64 observables of the current state (tests/verification_3d.py:409-416,
  * :612-654): grain_volume[N] = sum of phi_g over cells (times dx^3 by the
  * caller), front[nx*ny] = topmost k with 1 - phi_L > 0.5 (or -1), and
- * out[0..5] = max|sum phi - 1|, min phi, max phi, NaN count, solid sum, cells. */
+ * out[0..6] = max|sum phi - 1|, min phi, max phi, NaN count, solid sum, cells,
+ * stored grain slots. */
 int mpf_observables(mpf_handle *h, double *grain_volume, int32_t *front, double *out);
 
 /* Bytes of device memory held by the handle. */

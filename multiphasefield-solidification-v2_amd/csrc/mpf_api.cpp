@@ -184,7 +184,7 @@ static int reset_stats_dev(mpf_handle *h)
 {
     Stats s;
     memset(&s, 0, sizeof(s));
-    s.halt_step = -1;
+    s.halt_step = NO_HALT;
     HIP_TRY(hipMemcpyAsync(h->stats, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemsetAsync(h->wl_count, 0, 2 * sizeof(int), h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -432,7 +432,7 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
     const mpf_config &c = h->cfg;
     // a batch starts with clear worklist counters and no halt
     HIP_TRY(hipStreamSynchronize(h->stream));
-    int m1 = -1;
+    int m1 = NO_HALT;
     HIP_TRY(hipMemcpy(&h->stats->halt_step, &m1, sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(h->wl_count, 0, 2 * sizeof(int)));
     const int s0 = h->step_counter, cur0 = h->cur, tcur0 = h->tcur;
@@ -455,7 +455,7 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
     if (c.halt_on_kmax) {
         int hs = -1;
         HIP_TRY(hipMemcpy(&hs, &h->stats->halt_step, sizeof(int), hipMemcpyDeviceToHost));
-        if (hs >= 0) {
+        if (hs != NO_HALT) {
             done = hs - s0;
             h->cur = cur0 ^ (done & 1);
             h->tcur = tcur0 ^ ((done + 1) & 1);
@@ -528,7 +528,7 @@ extern "C" int mpf_get_stats(mpf_handle *h, mpf_stats *st)
     st->kmax_overflow = s.kmax_overflow;
     st->overflow_cells = s.overflow_cells;
     st->general_cells = s.slow_cells;
-    st->halt_step = s.halt_step;
+    st->halt_step = s.halt_step == NO_HALT ? -1 : s.halt_step;
     st->steps_done = h->steps_done;
     st->last_step_ms = h->last_ms;
     return 0;
@@ -546,14 +546,14 @@ extern "C" int mpf_observables(mpf_handle *h, double *grain_volume, int32_t *fro
     const mpf_config &c = h->cfg;
     int N = c.n_phase, nz = c.nz;
     size_t ncol = (size_t)c.nx * c.ny;
-    if (((size_t)N + 256 * 5) * sizeof(double) > 64 * 1024)
-        return fail(MPF_EINVAL, "mpf_observables supports at most %d phases", 64 * 1024 / 8 - 1280);
-    size_t bytes = (size_t)N * 8 + (size_t)nz * 5 * 8 + ncol * 4;
+    if (((size_t)N + 256 * 6) * sizeof(double) > 64 * 1024)
+        return fail(MPF_EINVAL, "mpf_observables supports at most %d phases", 64 * 1024 / 8 - 1536);
+    size_t bytes = (size_t)N * 8 + (size_t)nz * 6 * 8 + ncol * 4;
     void *d = nullptr;
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipMalloc(&d, bytes));
     double *gv = (double *)d, *part = gv + N;
-    int32_t *fr = (int32_t *)(part + (size_t)nz * 5);
+    int32_t *fr = (int32_t *)(part + (size_t)nz * 6);
     hipError_t e = hipMemsetAsync(d, 0, bytes, h->stream);
     if (e == hipSuccess) {
         if (h->rsz == 8)
@@ -561,21 +561,22 @@ extern "C" int mpf_observables(mpf_handle *h, double *grain_volume, int32_t *fro
         else
             e = mpf_launch_observe_f32(view<float>(h, h->cur), c.nx, c.ny, nz, N, gv, part, fr, h->stream);
     }
-    std::vector<double> hp((size_t)nz * 5);
+    std::vector<double> hp((size_t)nz * 6);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess) e = hipMemcpy(grain_volume, gv, (size_t)N * 8, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(front, fr, ncol * 4, hipMemcpyDeviceToHost);
     hipFree(d);
     if (e != hipSuccess) return fail(MPF_EHIP, "mpf_observables: %s", hipGetErrorString(e));
-    double emax = 0.0, pmin = 1e300, pmax = -1e300, nanc = 0.0, solid = 0.0;
+    double emax = 0.0, pmin = 1e300, pmax = -1e300, nanc = 0.0, solid = 0.0, slots = 0.0;
     for (int k = 0; k < nz; ++k) {
-        const double *q = hp.data() + (size_t)k * 5;
+        const double *q = hp.data() + (size_t)k * 6;
         emax = q[0] > emax ? q[0] : emax;
         pmin = q[1] < pmin ? q[1] : pmin;
         pmax = q[2] > pmax ? q[2] : pmax;
         nanc += q[3];
         solid += q[4];
+        slots += q[5];
     }
     out[0] = emax;
     out[1] = pmin;
@@ -583,6 +584,7 @@ extern "C" int mpf_observables(mpf_handle *h, double *grain_volume, int32_t *fro
     out[3] = nanc;
     out[4] = solid;
     out[5] = (double)h->ncell;
+    out[6] = slots;
     return 0;
 }
 

@@ -307,7 +307,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
 template <int DIM>
 __global__ __launch_bounds__(BX * BY) void k_step(StepArgs<Real> a)
 {
-    if (a.halt_mode && a.stats->halt_step >= 0 && a.stats->halt_step < a.step) return;
+    if (a.halt_mode && a.stats->halt_step < a.step) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) a.wl_count[(a.step + 1) & 1] = 0;
     const Consts<Real> &K = a.K;
     int lb = xcd_remap(blockIdx.x, gridDim.x);
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(BX * BY) void k_step(StepArgs<Real> a)
 template <int DIM>
 __global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
 {
-    if (a.halt_mode && a.stats->halt_step >= 0 && a.stats->halt_step < a.step) return;
+    if (a.halt_mode && a.stats->halt_step < a.step) return;
     const Consts<Real> &K = a.K;
     const int n = a.wl_count[a.step & 1];
     const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
@@ -659,26 +659,27 @@ __global__ void k_dense_temp(void *temp, int temp_f64, double cooling, size_t n)
 
 // FP64 observables (tests/verification_3d.py:409-416, :612-654).  One block
 // per z-plane: per-grain sums with LDS atomics, scalar partials by a
-// deterministic block reduction into part[k][5]: max|sum-1|, min, max, NaN
-// count, sum of (1 - phi_L).  front[l][m] gets the topmost k with
+// deterministic block reduction into part[k][6]: max|sum-1|, min, max, NaN
+// count, sum of (1 - phi_L), stored grain slots.  front[l][m] gets the topmost k with
 // 1 - phi_L > 0.5 from a separate pass.
 __global__ __launch_bounds__(256) void k_observe(SlotView<Real> S, int nx, int ny, int nz, int N,
                                                  double *gvol, double *part)
 {
     extern __shared__ double sh[];
     double *vol = sh;                  // [N]
-    double *red = sh + N;              // [256*5]
+    double *red = sh + N;              // [256*6]
     const int k = blockIdx.x;
     const size_t ncell = (size_t)nx * ny * nz;
     for (int g = threadIdx.x; g < N; g += blockDim.x) vol[g] = 0.0;
     __syncthreads();
-    double e_max = 0.0, p_min = 1e300, p_max = -1e300, nan_c = 0.0, solid = 0.0;
+    double e_max = 0.0, p_min = 1e300, p_max = -1e300, nan_c = 0.0, solid = 0.0, slots = 0.0;
     bool any_absent = false;
     for (int i = threadIdx.x; i < nx * ny; i += blockDim.x) {
         size_t c = (size_t)k * nx * ny + i;
         double lq = (double)S.liq[c];
         double sum = lq;
         int n = S.cnt[c];
+        slots += (double)n;
         if (lq != lq) nan_c += 1.0;
         p_min = lq < p_min ? lq : p_min;
         p_max = lq > p_max ? lq : p_max;
@@ -701,22 +702,23 @@ __global__ __launch_bounds__(256) void k_observe(SlotView<Real> S, int nx, int n
         p_min = 0.0 < p_min ? 0.0 : p_min;
         p_max = 0.0 > p_max ? 0.0 : p_max;
     }
-    double *r = red + threadIdx.x * 5;
-    r[0] = e_max; r[1] = p_min; r[2] = p_max; r[3] = nan_c; r[4] = solid;
+    double *r = red + threadIdx.x * 6;
+    r[0] = e_max; r[1] = p_min; r[2] = p_max; r[3] = nan_c; r[4] = solid; r[5] = slots;
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
         if ((int)threadIdx.x < h) {
-            double *o = red + (threadIdx.x + h) * 5;
+            double *o = red + (threadIdx.x + h) * 6;
             r[0] = o[0] > r[0] ? o[0] : r[0];
             r[1] = o[1] < r[1] ? o[1] : r[1];
             r[2] = o[2] > r[2] ? o[2] : r[2];
             r[3] += o[3];
             r[4] += o[4];
+            r[5] += o[5];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int q = 0; q < 5; ++q) part[(size_t)k * 5 + q] = red[q];
+        for (int q = 0; q < 6; ++q) part[(size_t)k * 6 + q] = red[q];
     for (int g = threadIdx.x; g < N; g += blockDim.x) atomicAdd(&gvol[g], vol[g]);
 }
 
@@ -863,7 +865,7 @@ hipError_t MPF_NAME(mpf_launch_dense_temp)(void *temp, int temp_f64, double cool
 hipError_t MPF_NAME(mpf_launch_observe)(SlotView<Real> S, int nx, int ny, int nz, int N, double *gvol,
                                         double *part, int32_t *front, hipStream_t st)
 {
-    size_t shm = ((size_t)N + 256 * 5) * sizeof(double);
+    size_t shm = ((size_t)N + 256 * 6) * sizeof(double);
     hipLaunchKernelGGL(k_observe, dim3(nz), dim3(256), shm, st, S, nx, ny, nz, N, gvol, part);
     hipLaunchKernelGGL(k_front, dim3(nblk((size_t)nx * ny, 256)), dim3(256), 0, st, S.liq, nx, ny, nz, front);
     return hipGetLastError();

@@ -24,6 +24,7 @@ namespace mpf {
 
 constexpr int KCAP = 64;        // largest KMAX supported
 constexpr uint32_t NO_ID = 0xFFFFu;
+constexpr int NO_HALT = 0x7fffffff; // halt_step value while no step has overflowed
 
 template <typename Real>
 struct SlotView {
@@ -38,7 +39,7 @@ struct Stats {
     int kmax_overflow;         // 1 if any cell exceeded KMAX (status[0])
     unsigned long long overflow_cells; // cells that exceeded KMAX (summed over steps)
     unsigned long long slow_cells;     // cells sent to the general path
-    int halt_step;             // first step with a KMAX overflow (halt mode), else -1
+    int halt_step;             // first step with a KMAX overflow (halt mode), else NO_HALT
     int pad;
 };
 

@@ -165,7 +165,7 @@ class Solver:
         N = self.n_phase
         gv = np.zeros(N, np.float64)
         front = np.zeros((self.nx, self.ny), np.int32)
-        out = np.zeros(6, np.float64)
+        out = np.zeros(7, np.float64)
         check(_lib.lib().mpf_observables(self._h, ptr(gv), ptr(front, C.c_int32), ptr(out)))
         scale = float(dx) ** 3 if dx is not None else 1.0
         f = front.astype(np.float64)
@@ -176,6 +176,7 @@ class Solver:
             "grain_volumes": gv[1:] * scale,
             "phi_sum_error_max": out[0], "phi_min": out[1], "phi_max": out[2],
             "nan_count": int(out[3]), "solid_sum": out[4], "cells": int(out[5]),
+            "stored_slots": int(out[6]),
             "solid_fraction": out[4] / out[5],
             "front": f,
             "front_mean": float(np.mean(finite)) if finite.size else float("nan"),
