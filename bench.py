@@ -82,12 +82,12 @@ def alg_bytes_per_step(ob0, ob1, bv, ncell, nz):
     return 2.0 * ncell * (bv + 1) + 2.0 * slots * (bv + 2) + 2.0 * nz * 8, slots / ncell
 
 
-def cpu_baseline(W, steps=2):
+def cpu_baseline(W, steps=6):
     """FP64 CPU oracle (reference dense layout, OpenMP over cells) on a
-    128 x 128 x 48 slab holding the seed layer and the front."""
+    256 x 256 x 48 slab holding the seed layer and the front."""
     from oracle import oracle as O
 
-    sx, sy, sz = 128, 128, 48
+    sx, sy, sz = 256, 256, 48
     p, d = W["p"], W["d"]
     N = W["N"]
     phi = np.zeros((N, sx, sy, sz), np.float64)

@@ -11,13 +11,14 @@
 
 #include "mpf.h"
 #include "mpf_math.h"
+#include "mpf_consts.h"
 #include "mpf_state.h"
 
 using namespace mpf;
 
 // ------------------------------------------------ launchers of both builds
 #define MPF_DECLARE_LAUNCHERS(R, SFX)                                                                    \
-    extern "C" hipError_t mpf_launch_step##SFX(const StepArgs<R> *, int, int, hipStream_t);              \
+    extern "C" hipError_t mpf_launch_step##SFX(const StepArgs<R> *, int, const int *, hipStream_t);      \
     extern "C" hipError_t mpf_launch_apt_slots##SFX(const Consts<R> *, SlotView<R>, int, int32_t *,      \
                                                     int32_t *, int32_t *, int, hipStream_t);             \
     extern "C" hipError_t mpf_launch_pack##SFX(const void *, int, int, int, int, int, SlotView<R>, int,  \
@@ -81,8 +82,8 @@ struct mpf_handle {
     double *T[2] = {nullptr, nullptr};
     size_t nT = 0;
     void *tab = nullptr; // wij, aij, mij, n111 contiguous
-    int *worklist = nullptr;
-    int *wl_count = nullptr;
+    int *wl[3] = {nullptr, nullptr, nullptr}; // merge, front, general work lists
+    int *wl_count = nullptr;                   // [3][2]
     Stats *stats = nullptr;
     int cur = 0, tcur = 0;
     int step_counter = 0;
@@ -102,49 +103,6 @@ static SlotView<R> view(mpf_handle *h, int i)
     v.sid = h->b[i].sid;
     v.sval = (R *)h->b[i].sval;
     return v;
-}
-
-template <typename R>
-static Consts<R> make_consts(const mpf_scalars &s, int nx, int ny, int nz, int N, int kmax, int aniso,
-                             int torque, const void *tab)
-{
-    Consts<R> K;
-    K.nx = nx;
-    K.ny = ny;
-    K.nz = nz;
-    K.N = N;
-    K.kmax = kmax;
-    K.aniso = aniso;
-    K.torque = torque;
-    // Derived constants are formed exactly as the reference writes them
-    // inline (2.0*dx, 1.0/(dx*dx), 0.25*inv_dx2, 0.5/dx, ...).
-    double inv_dx2 = 1.0 / (s.dx * s.dx);
-    K.dx = (R)s.dx;
-    K.dt = (R)s.dt;
-    K.two_dx = (R)(2.0 * s.dx);
-    K.inv_dx2 = (R)inv_dx2;
-    K.q_inv_dx2 = (R)(0.25 * inv_dx2);
-    K.half_over_dx = (R)(0.5 / s.dx);
-    K.eps0_sl = (R)s.eps0_sl;
-    K.eps0_sq = (R)(s.eps0_sl * s.eps0_sl);
-    K.w0_sl = (R)s.w0_sl;
-    K.tan_a0 = (R)mpf_tan(s.a0);
-    K.delta_a = (R)s.delta_a;
-    K.mu_a = (R)s.mu_a;
-    K.mu_delta = (R)(s.mu_a * s.delta_a);
-    K.p2 = (R)(s.p_round * s.p_round);
-    K.g2_floor = (R)s.g2_floor;
-    K.ksi = (R)s.ksi;
-    K.theta_c = (R)s.theta_c_rad;
-    K.Sf = s.Sf;
-    K.T_melt = s.T_melt;
-    const R *t = (const R *)tab;
-    size_t nn = (size_t)N * N;
-    K.wij = t;
-    K.aij = t + nn;
-    K.mij = t + 2 * nn;
-    K.n111 = t + 3 * nn;
-    return K;
 }
 
 static int dev_alloc(mpf_handle *h, void **p, size_t bytes)
@@ -171,7 +129,7 @@ extern "C" void mpf_destroy(mpf_handle *h)
         hipFree(h->T[i]);
     }
     hipFree(h->tab);
-    hipFree(h->worklist);
+    for (int i = 0; i < 3; ++i) hipFree(h->wl[i]);
     hipFree(h->wl_count);
     hipFree(h->stats);
     if (h->ev0) hipEventDestroy(h->ev0);
@@ -186,7 +144,7 @@ static int reset_stats_dev(mpf_handle *h)
     memset(&s, 0, sizeof(s));
     s.halt_step = NO_HALT;
     HIP_TRY(hipMemcpyAsync(h->stats, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemsetAsync(h->wl_count, 0, 2 * sizeof(int), h->stream));
+    HIP_TRY(hipMemsetAsync(h->wl_count, 0, 6 * sizeof(int), h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     return 0;
 }
@@ -250,8 +208,8 @@ extern "C" int mpf_create(const mpf_config *cfg, mpf_handle **out)
     for (int i = 0; i < 2; ++i) TRY_RC(dev_alloc(h, (void **)&h->T[i], h->nT * sizeof(double)));
     size_t nn = (size_t)c.n_phase * c.n_phase;
     TRY_RC(dev_alloc(h, &h->tab, (3 * nn + (size_t)c.n_phase * 24) * h->rsz));
-    TRY_RC(dev_alloc(h, (void **)&h->worklist, ncell * sizeof(int)));
-    TRY_RC(dev_alloc(h, (void **)&h->wl_count, 2 * sizeof(int)));
+    for (int i = 0; i < 3; ++i) TRY_RC(dev_alloc(h, (void **)&h->wl[i], ncell * sizeof(int)));
+    TRY_RC(dev_alloc(h, (void **)&h->wl_count, 6 * sizeof(int)));
     TRY_RC(dev_alloc(h, (void **)&h->stats, sizeof(Stats)));
     // empty state: all liquid, no grains
     for (int i = 0; i < 2; ++i) {
@@ -398,7 +356,7 @@ extern "C" int mpf_download_temp(mpf_handle *h, double *temp)
 }
 
 template <typename R>
-static hipError_t launch_one_step(mpf_handle *h, int general_blocks)
+static hipError_t launch_one_step(mpf_handle *h, const int *grids)
 {
     const mpf_config &c = h->cfg;
     StepArgs<R> a;
@@ -413,14 +371,14 @@ static hipError_t launch_one_step(mpf_handle *h, int general_blocks)
     a.temp_dense = c.temp_dense;
     a.temp_f32 = c.temp_f32;
     a.cooling = c.cooling_rate;
-    a.worklist = h->worklist;
+    for (int i = 0; i < 3; ++i) a.wl[i] = h->wl[i];
     a.wl_count = h->wl_count;
     a.step = h->step_counter;
     a.halt_mode = c.halt_on_kmax;
     a.stats = h->stats;
     if (sizeof(R) == 8)
-        return mpf_launch_step_f64((const StepArgs<double> *)&a, c.dim, general_blocks, h->stream);
-    return mpf_launch_step_f32((const StepArgs<float> *)&a, c.dim, general_blocks, h->stream);
+        return mpf_launch_step_f64((const StepArgs<double> *)&a, c.dim, grids, h->stream);
+    return mpf_launch_step_f32((const StepArgs<float> *)&a, c.dim, grids, h->stream);
 }
 
 extern "C" int mpf_step(mpf_handle *h, int nsteps)
@@ -434,13 +392,19 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
     HIP_TRY(hipStreamSynchronize(h->stream));
     int m1 = NO_HALT;
     HIP_TRY(hipMemcpy(&h->stats->halt_step, &m1, sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(h->wl_count, 0, 2 * sizeof(int)));
+    HIP_TRY(hipMemset(h->wl_count, 0, 6 * sizeof(int)));
     const int s0 = h->step_counter, cur0 = h->cur, tcur0 = h->tcur;
-    const int general_blocks = 256;
+    // grid-stride work-list kernels: enough waves to fill the GPU, fewer for
+    // small grids
+    auto blocks = [&](size_t threads_per_block, int cap) {
+        size_t b = (h->ncell + threads_per_block - 1) / threads_per_block;
+        return (int)(b < (size_t)cap ? (b ? b : 1) : cap);
+    };
+    const int grids[3] = {blocks(256, 2048), blocks(128, 2048), blocks(64, 256)};
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
     for (int i = 0; i < nsteps; ++i) {
-        hipError_t e = (h->rsz == 8) ? launch_one_step<double>(h, general_blocks)
-                                     : launch_one_step<float>(h, general_blocks);
+        hipError_t e = (h->rsz == 8) ? launch_one_step<double>(h, grids)
+                                     : launch_one_step<float>(h, grids);
         if (e != hipSuccess) return fail(MPF_EHIP, "step launch: %s", hipGetErrorString(e));
         h->cur ^= 1;
         h->tcur ^= 1;

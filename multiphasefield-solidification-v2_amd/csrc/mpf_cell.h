@@ -16,8 +16,17 @@
 // deterministic functions of the same inputs this changes no bits.
 #pragma once
 
-#include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MPF_DEV __device__ __forceinline__
+#else
+// Host build of the per-cell code (CPU emulation tests).
+#include <cmath>
+#include <cstddef>
+#define MPF_DEV inline
+#endif
 
 #include "mpf_math.h"
 
@@ -44,22 +53,22 @@ struct Consts {
 };
 
 // ---------------------------------------------------------------- math glue
-__device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float m_sqrt(float x) { return sqrtf(x); }
-__device__ __forceinline__ double m_abs(double x) { return fabs(x); }
-__device__ __forceinline__ float m_abs(float x) { return fabsf(x); }
-__device__ __forceinline__ double m_tan(double x) { return mpf_tan(x); }
-__device__ __forceinline__ float m_tan(float x) { return tanf(x); }
-__device__ __forceinline__ double m_acos(double x) { return mpf_acos(x); }
-__device__ __forceinline__ float m_acos(float x) { return acosf(x); }
-__device__ __forceinline__ double m_tanh(double x) { return mpf_tanh(x); }
-__device__ __forceinline__ float m_tanh(float x) { return tanhf(x); }
+MPF_DEV double m_sqrt(double x) { return sqrt(x); }
+MPF_DEV float m_sqrt(float x) { return sqrtf(x); }
+MPF_DEV double m_abs(double x) { return fabs(x); }
+MPF_DEV float m_abs(float x) { return fabsf(x); }
+MPF_DEV double m_tan(double x) { return mpf_tan(x); }
+MPF_DEV float m_tan(float x) { return tanf(x); }
+MPF_DEV double m_acos(double x) { return mpf_acos(x); }
+MPF_DEV float m_acos(float x) { return acosf(x); }
+MPF_DEV double m_tanh(double x) { return mpf_tanh(x); }
+MPF_DEV float m_tanh(float x) { return tanhf(x); }
 template <typename Real>
-__device__ __forceinline__ Real pymax(Real a, Real b) { return (b > a) ? b : a; }
+MPF_DEV Real pymax(Real a, Real b) { return (b > a) ? b : a; }
 
 // calc_a_from_cos, src/gpu_kernels_3d.py:24-30
 template <typename Real>
-__device__ __forceinline__ Real calc_a(const Consts<Real> &K, Real cost)
+MPF_DEV Real calc_a(const Consts<Real> &K, Real cost)
 {
     Real c2 = cost * cost;
     Real C = m_sqrt(c2 + K.p2);
@@ -69,7 +78,7 @@ __device__ __forceinline__ Real calc_a(const Consts<Real> &K, Real cost)
 
 // calc_b_from_cos, src/gpu_kernels_3d.py:33-55
 template <typename Real>
-__device__ __forceinline__ Real calc_b(const Consts<Real> &K, Real best_cost)
+MPF_DEV Real calc_b(const Consts<Real> &K, Real best_cost)
 {
     Real c = best_cost;
     if (c > Real(1.0))
@@ -91,10 +100,10 @@ __device__ __forceinline__ Real calc_b(const Consts<Real> &K, Real best_cost)
 // ------------------------------------------------------ boundary conditions
 // src/gpu_kernels_3d.py:58-91 (x,y periodic; z mirror) and
 // src/gpu_kernels.py:84-105 (2D: x periodic, y mirror).
-__device__ __forceinline__ int bc_pp(int i, int n) { return i < n - 1 ? i + 1 : 0; }
-__device__ __forceinline__ int bc_pm(int i, int n) { return i > 0 ? i - 1 : n - 1; }
-__device__ __forceinline__ int bc_mp(int i, int n) { return i < n - 1 ? i + 1 : n - 1; }
-__device__ __forceinline__ int bc_mm(int i, int n) { return i > 0 ? i - 1 : 0; }
+MPF_DEV int bc_pp(int i, int n) { return i < n - 1 ? i + 1 : 0; }
+MPF_DEV int bc_pm(int i, int n) { return i > 0 ? i - 1 : n - 1; }
+MPF_DEV int bc_mp(int i, int n) { return i < n - 1 ? i + 1 : n - 1; }
+MPF_DEV int bc_mm(int i, int n) { return i > 0 ? i - 1 : 0; }
 
 // Result of the {111} facet search at one stencil point for one grain.
 template <typename Real>
@@ -105,7 +114,7 @@ struct Facet {
 // facet_cos_and_nxyz_from_grad (src/gpu_kernels_3d.py:251-301) given the unit
 // gradient; best_cos_from_grad_3d (:110-134) returns the same |cos|.
 template <typename Real, int DIM>
-__device__ __forceinline__ Facet<Real> facet_search(const Real *n, Real gx, Real gy, Real gz)
+MPF_DEV Facet<Real> facet_search(const Real *n, Real gx, Real gy, Real gz)
 {
     Facet<Real> f;
     f.cs = Real(0.0);
@@ -148,7 +157,7 @@ struct LGrad {
 };
 
 template <typename Real, int DIM>
-__device__ __forceinline__ void lgrad_finish(const Consts<Real> &K, LGrad<Real> &G)
+MPF_DEV void lgrad_finish(const Consts<Real> &K, LGrad<Real> &G)
 {
     if constexpr (DIM == 3)
         G.q = G.gx * G.gx + G.gy * G.gy + G.gz * G.gz;
@@ -175,7 +184,7 @@ struct A12v {
 
 // da_dphixyz_A12, src/gpu_kernels_3d.py:304-341 (2D: src/gpu_kernels.py:237-264)
 template <typename Real, int DIM>
-__device__ __forceinline__ A12v<Real> a12_at(const Consts<Real> &K, const LGrad<Real> &G,
+MPF_DEV A12v<Real> a12_at(const Consts<Real> &K, const LGrad<Real> &G,
                                              const Facet<Real> &f)
 {
     A12v<Real> r;
@@ -219,7 +228,7 @@ struct LiqCtx {
 };
 
 template <typename Real, int DIM, typename LiqAt>
-__device__ __forceinline__ void lgrad_at(const Consts<Real> &K, const LiqAt &L, int l, int m, int k,
+MPF_DEV void lgrad_at(const Consts<Real> &K, const LiqAt &L, int l, int m, int k,
                                          LGrad<Real> &G)
 {
     if constexpr (DIM == 3) {
@@ -240,7 +249,7 @@ __device__ __forceinline__ void lgrad_at(const Consts<Real> &K, const LiqAt &L, 
 }
 
 template <typename Real, int DIM, typename LiqAt>
-__device__ __forceinline__ void build_liq_ctx(const Consts<Real> &K, const LiqAt &L, int l, int m, int k,
+MPF_DEV void build_liq_ctx(const Consts<Real> &K, const LiqAt &L, int l, int m, int k,
                                            LiqCtx<Real, DIM> &X)
 {
     int lp = bc_pp(l, K.nx), lm = bc_pm(l, K.nx);
@@ -288,7 +297,7 @@ __device__ __forceinline__ void build_liq_ctx(const Consts<Real> &K, const LiqAt
 // lap_sl and b_sl of one active solid s (src/gpu_kernels_3d.py:558-577,
 // :709-733; 2D src/gpu_kernels.py:394-408).  Requires K.aniso.
 template <typename Real, int DIM>
-__device__ __forceinline__ void solid_liquid_terms(const Consts<Real> &K, const LiqCtx<Real, DIM> &X,
+MPF_DEV void solid_liquid_terms(const Consts<Real> &K, const LiqCtx<Real, DIM> &X,
                                                 int s, Real &lap_sl, Real &b_sl)
 {
     constexpr int NP = LiqCtx<Real, DIM>::NP;
@@ -397,7 +406,7 @@ __device__ __forceinline__ void solid_liquid_terms(const Consts<Real> &K, const 
 // Read/write a small register array at a run-time index without dynamic
 // register indexing (which would move the array to scratch memory).
 template <int NA, bool SMALL>
-__device__ __forceinline__ int sel_int(const int *a, int t)
+MPF_DEV int sel_int(const int *a, int t)
 {
     if constexpr (!SMALL) return a[t];
     int r = a[0];
@@ -407,7 +416,7 @@ __device__ __forceinline__ int sel_int(const int *a, int t)
     return r;
 }
 template <typename Real, int NA, bool SMALL>
-__device__ __forceinline__ void put_real(Real *a, int t, Real v)
+MPF_DEV void put_real(Real *a, int t, Real v)
 {
     if constexpr (!SMALL) {
         a[t] = v;
@@ -435,8 +444,10 @@ struct Cell {
 // The update of one cell: src/gpu_kernels_3d.py:550-680 / :699-830 (3D) and
 // src/gpu_kernels.py:386-515 (2D).  Tcell is the (already cooled) temperature.
 // Cells with many active phases (A large) keep the arrays in scratch memory.
-template <typename Real, int DIM, int A, typename LiqAt>
-__device__ __forceinline__ void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, int k,
+// ANISO_TERMS = false compiles the anisotropy terms out; the caller then
+// guarantees the cell has no active solid-liquid pair or anisotropy is off.
+template <typename Real, int DIM, int A, bool ANISO_TERMS = true, typename LiqAt>
+MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, int k,
                                             double Tcell, Cell<Real, DIM, A> &S)
 {
     constexpr int NA = A + 1;
@@ -444,17 +455,25 @@ __device__ __forceinline__ void cell_update(const Consts<Real> &K, const LiqAt &
     constexpr bool SMALL = (A <= 8);
     constexpr int UNR_NA = SMALL ? NA : 1;
     constexpr int UNR_A = SMALL ? A : 1;
+    // Loops run over the fixed capacity on the register path (unrolled, with
+    // a presence test) and only over the occupied entries otherwise.
+    const int TMAX = [&] {
+        if constexpr (SMALL)
+            return NA;
+        else
+            return S.nsol + 1;
+    }();
     const int n_act = (S.liq ? 1 : 0) + S.nsol;
     Real lap_sl[NA], b_sl[NA], lap[NA];
 #pragma unroll UNR_NA
-    for (int t = 0; t < NA; ++t) {
+    for (int t = 0; t < TMAX; ++t) {
         lap_sl[t] = Real(0.0);
         b_sl[t] = Real(1.0);
     }
 
     // Laplacian of every active phase (laplacian_3d, :444-459 / inline :620-625)
 #pragma unroll UNR_NA
-    for (int t = 0; t < NA; ++t) {
+    for (int t = 0; t < TMAX; ++t) {
         const Real *w = S.v[t];
         if constexpr (DIM == 3)
             lap[t] = (w[P_XP] + w[P_XM] + w[P_YP] + w[P_YM] + w[P_ZP] + w[P_ZM] - Real(6.0) * w[P_C]) *
@@ -467,15 +486,17 @@ __device__ __forceinline__ void cell_update(const Consts<Real> &K, const LiqAt &
     // only when the liquid is active (the reference computes them regardless).
     Real w_sl = K.w0_sl;
     if (S.liq && S.nsol > 0) {
-        if (K.aniso || DIM == 2) {
-            LiqCtx<Real, DIM> X;
-            build_liq_ctx<Real, DIM>(K, L, l, m, k, X);
-            for (int t = 1; t <= S.nsol; ++t) {
-                int g = sel_int<NA, SMALL>(S.gid, t);
-                Real ls, bs;
-                solid_liquid_terms<Real, DIM>(K, X, g, ls, bs);
-                put_real<Real, NA, SMALL>(lap_sl, t, ls);
-                put_real<Real, NA, SMALL>(b_sl, t, bs);
+        if (ANISO_TERMS && (K.aniso || DIM == 2)) {
+            if constexpr (ANISO_TERMS) {
+                LiqCtx<Real, DIM> X;
+                build_liq_ctx<Real, DIM>(K, L, l, m, k, X);
+                for (int t = 1; t <= S.nsol; ++t) {
+                    int g = sel_int<NA, SMALL>(S.gid, t);
+                    Real ls, bs;
+                    solid_liquid_terms<Real, DIM>(K, X, g, ls, bs);
+                    put_real<Real, NA, SMALL>(lap_sl, t, ls);
+                    put_real<Real, NA, SMALL>(b_sl, t, bs);
+                }
             }
         } else {
             // anisotropy off (switchable kernel, :729-731): eps0^2 * lap(phi_L)
@@ -488,7 +509,8 @@ __device__ __forceinline__ void cell_update(const Consts<Real> &K, const LiqAt &
         }
 
         // dominant solid and anisotropic w_sl (:579-595; 2D :410-436)
-        if constexpr (DIM == 3) {
+        if constexpr (!ANISO_TERMS) {
+        } else if constexpr (DIM == 3) {
             if (K.aniso) {
                 int ts = -1;
                 Real maxv = Real(-1.0);
@@ -565,14 +587,14 @@ __device__ __forceinline__ void cell_update(const Consts<Real> &K, const LiqAt &
     const Real n_real = Real(n_act);
     const int N = K.N;
 #pragma unroll UNR_NA
-    for (int t1 = 0; t1 < NA; ++t1) {
+    for (int t1 = 0; t1 < TMAX; ++t1) {
         S.out[t1] = Real(0.0);
         const bool p1 = (t1 == 0) ? S.liq : (t1 <= S.nsol);
         if (!p1) continue;
         const int i = S.gid[t1];
         Real dpi = Real(0.0);
 #pragma unroll UNR_NA
-        for (int t2 = 0; t2 < NA; ++t2) {
+        for (int t2 = 0; t2 < TMAX; ++t2) {
             const bool p2 = (t2 == 0) ? S.liq : (t2 <= S.nsol);
             if (!p2 || t2 == t1) continue;
             const int j = S.gid[t2];
@@ -583,7 +605,7 @@ __device__ __forceinline__ void cell_update(const Consts<Real> &K, const LiqAt &
                 driving_force = df_ls;
             Real ppp = Real(0.0);
 #pragma unroll UNR_NA
-            for (int t3 = 0; t3 < NA; ++t3) {
+            for (int t3 = 0; t3 < TMAX; ++t3) {
                 const bool p3 = (t3 == 0) ? S.liq : (t3 <= S.nsol);
                 if (!p3) continue;
                 const int pk = S.gid[t3];
@@ -623,7 +645,7 @@ __device__ __forceinline__ void cell_update(const Consts<Real> &K, const LiqAt &
     // Clip and renormalise (:663-680)
     Real s = Real(0.0);
 #pragma unroll UNR_NA
-    for (int t = 0; t < NA; ++t) {
+    for (int t = 0; t < TMAX; ++t) {
         const bool pr = (t == 0) ? S.liq : (t <= S.nsol);
         if (!pr) continue;
         Real v = S.out[t];
@@ -635,7 +657,7 @@ __device__ __forceinline__ void cell_update(const Consts<Real> &K, const LiqAt &
     if (s > Real(1e-20)) {
         Real invs = Real(1.0) / s;
 #pragma unroll UNR_NA
-        for (int t = 0; t < NA; ++t) {
+        for (int t = 0; t < TMAX; ++t) {
             const bool pr = (t == 0) ? S.liq : (t <= S.nsol);
             if (pr) S.out[t] *= invs;
         }

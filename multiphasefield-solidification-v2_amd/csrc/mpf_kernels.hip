@@ -6,11 +6,16 @@
 // exports plain launcher functions named with the MPF_SFX suffix (_f64/_f32).
 //
 // Kernels
-//   k_step        fused temperature + APT + RHS + update over the slot layout
-//                 (src/gpu_kernels_3d.py:462-486, :489-527, :530-680,
-//                 :683-830, :833-838); one thread per cell, cells with more
-//                 than A active solids are queued for k_step_general
-//   k_step_general the same update with room for KMAX phases (scratch arrays)
+//   one time step (src/gpu_kernels_3d.py:462-486, :489-527, :530-680,
+//   :683-830, :833-838) is four kernels over the slot layout:
+//   k_light       every cell: temperature, classification, and the cells whose
+//                 stencil holds only liquid or only one grain
+//   k_merge       queued cells: APT merge; finishes cells without a
+//                 solid-liquid pair (or with anisotropy off)
+//   k_front       solid-liquid interface cells with the anisotropy and torque
+//                 terms, compacted so every lane does that work
+//   k_step_general cells with more active grains than the register path
+//                 holds (scratch arrays with room for KMAX phases)
 //   k_apt_slots   checked APT of the slot state into the reference's dense
 //                 mf/nf layout (parity export)
 //   k_pack/k_unpack dense reference layout <-> slots
@@ -20,6 +25,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mpf_gather.h"
 #include "mpf_state.h"
 
 #ifndef MPF_REAL
@@ -40,219 +46,11 @@ constexpr int FAST_A = 4;   // active solids held in registers on the fast path
 constexpr int BX = 64;      // tile width in x (one wavefront per row)
 constexpr int BY = 4;       // tile rows in y
 
-// ------------------------------------------------------------------ helpers
-__device__ __forceinline__ size_t cidx(int l, int m, int k, int nx, int ny)
+// Raise the running maximum of the active count.  It is monotone, so a lane
+// issues the atomic only when it would raise it.
+__device__ __forceinline__ void note_active(Stats *st, int ntot, int kmax, int step, int halt_mode)
 {
-    return ((size_t)k * ny + m) * nx + l;
-}
-
-// Reference dense index phi[g][l][m][k] (3D) / phi[g][l][m] (2D).
-__device__ __forceinline__ size_t ridx(int l, int m, int k, int ny, int nz)
-{
-    return ((size_t)l * ny + m) * nz + k;
-}
-
-// Stencil point cell indices in the slot layout: c, x+, x-, y+, y-, (z+, z-).
-template <int DIM>
-__device__ __forceinline__ void stencil_cells(const Consts<Real> &K, int l, int m, int k, size_t *pc)
-{
-    int lp = bc_pp(l, K.nx), lm = bc_pm(l, K.nx);
-    if constexpr (DIM == 3) {
-        int mp = bc_pp(m, K.ny), mm = bc_pm(m, K.ny);
-        int kp = bc_mp(k, K.nz), km = bc_mm(k, K.nz);
-        pc[P_C] = cidx(l, m, k, K.nx, K.ny);
-        pc[P_XP] = cidx(lp, m, k, K.nx, K.ny);
-        pc[P_XM] = cidx(lm, m, k, K.nx, K.ny);
-        pc[P_YP] = cidx(l, mp, k, K.nx, K.ny);
-        pc[P_YM] = cidx(l, mm, k, K.nx, K.ny);
-        pc[P_ZP] = cidx(l, m, kp, K.nx, K.ny);
-        pc[P_ZM] = cidx(l, m, km, K.nx, K.ny);
-    } else {
-        int mp = bc_mp(m, K.ny), mm = bc_mm(m, K.ny);
-        pc[P_C] = cidx(l, m, 0, K.nx, K.ny);
-        pc[P_XP] = cidx(lp, m, 0, K.nx, K.ny);
-        pc[P_XM] = cidx(lm, m, 0, K.nx, K.ny);
-        pc[P_YP] = cidx(l, mp, 0, K.nx, K.ny);
-        pc[P_YM] = cidx(l, mm, 0, K.nx, K.ny);
-    }
-}
-
-struct SlotLiq {
-    const Real *liq;
-    int nx, ny;
-    __device__ __forceinline__ Real operator()(int l, int m, int k) const
-    {
-        return liq[cidx(l, m, k, nx, ny)];
-    }
-};
-
-struct DenseLiq {
-    const Real *phi; // phase 0 of the dense field
-    int ny, nz;
-    __device__ __forceinline__ Real operator()(int l, int m, int k) const
-    {
-        return phi[ridx(l, m, k, ny, nz)];
-    }
-};
-
-// k-way merge of the sorted slot lists of the stencil points.  Yields the
-// union of grain ids present at any of the points in ascending order with the
-// value at every point (0 where absent).
-template <int DIM>
-struct Merge {
-    static constexpr int NP = (DIM == 3) ? 7 : 5;
-    size_t pc[NP];
-    int cnt[NP], pos[NP];
-    uint32_t hid[NP];
-    Real hv[NP];
-
-    __device__ __forceinline__ void init(const SlotView<Real> &S, size_t ncell)
-    {
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            cnt[p] = S.cnt[pc[p]];
-            pos[p] = 0;
-            hid[p] = NO_ID;
-            hv[p] = Real(0.0);
-            if (cnt[p] > 0) {
-                hid[p] = S.sid[pc[p]];
-                hv[p] = S.sval[pc[p]];
-            }
-        }
-        (void)ncell;
-    }
-
-    // Next candidate; returns NO_ID when all lists are exhausted.
-    __device__ __forceinline__ uint32_t next(const SlotView<Real> &S, size_t ncell, Real *vals)
-    {
-        uint32_t g = hid[0];
-#pragma unroll
-        for (int p = 1; p < NP; ++p) g = hid[p] < g ? hid[p] : g;
-        if (g == NO_ID) return NO_ID;
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            if (hid[p] == g) {
-                vals[p] = hv[p];
-                pos[p] += 1;
-                if (pos[p] < cnt[p]) {
-                    size_t o = (size_t)pos[p] * ncell + pc[p];
-                    hid[p] = S.sid[o];
-                    hv[p] = S.sval[o];
-                } else {
-                    hid[p] = NO_ID;
-                }
-            } else {
-                vals[p] = Real(0.0);
-            }
-        }
-        return g;
-    }
-};
-
-// APT rule, src/gpu_kernels_3d.py:478-482: phi > 0 at the cell, or phi == 0
-// there and > 0 at a face neighbour.
-template <int NP>
-__device__ __forceinline__ bool apt_active(const Real *vals)
-{
-    if (vals[0] > Real(0.0)) return true;
-    if (!(vals[0] == Real(0.0))) return false;
-    bool any = false;
-#pragma unroll
-    for (int p = 1; p < NP; ++p) any = any || (vals[p] > Real(0.0));
-    return any;
-}
-
-// Gather the active set of one cell (APT with the checked KMAX clamp).
-// Returns the total number of active phases (before the clamp) in *ntot and
-// whether the kept solids fit in A (*fits).
-template <int DIM, int A>
-__device__ __forceinline__ void gather(const Consts<Real> &K, const SlotView<Real> &S, size_t ncell,
-                                       int l, int m, int k, Cell<Real, DIM, A> &C, int *ntot, bool *fits)
-{
-    constexpr int NP = Cell<Real, DIM, A>::NP;
-    constexpr bool SMALL = (A <= 8);
-    Merge<DIM> M;
-    stencil_cells<DIM>(K, l, m, k, M.pc);
-    // liquid (dense)
-    Real lv[NP];
-#pragma unroll
-    for (int p = 0; p < NP; ++p) lv[p] = S.liq[M.pc[p]];
-    C.liq = apt_active<NP>(lv);
-    C.gid[0] = 0;
-#pragma unroll
-    for (int p = 0; p < NP; ++p) {
-        C.v[0][p] = lv[p];
-        C.dom1[p] = Real(0.0);
-    }
-    M.init(S, ncell);
-    int keep = K.kmax - (C.liq ? 1 : 0); // solids kept under the KMAX clamp
-    int nact = 0;                        // active solids found
-    C.nsol = 0;
-    Real vals[NP];
-    if constexpr (SMALL) {
-        // unrolled over output positions so that the arrays stay in registers
-#pragma unroll
-        for (int u = 1; u <= A; ++u) {
-            if (nact < keep) {
-                uint32_t g;
-                bool act = false;
-                while ((g = M.next(S, ncell, vals)) != NO_ID) {
-                    if constexpr (DIM == 2) {
-                        if (g == 1) {
-#pragma unroll
-                            for (int p = 0; p < NP; ++p) C.dom1[p] = vals[p];
-                        }
-                    }
-                    if (apt_active<NP>(vals)) {
-                        act = true;
-                        break;
-                    }
-                }
-                if (act) {
-                    nact += 1;
-                    C.nsol = u;
-                    C.gid[u] = (int)g;
-#pragma unroll
-                    for (int p = 0; p < NP; ++p) C.v[u][p] = vals[p];
-                }
-            }
-        }
-    }
-    // count (and, for the general path, store) the remaining active solids
-    uint32_t g;
-    while ((g = M.next(S, ncell, vals)) != NO_ID) {
-        if constexpr (DIM == 2) {
-            if (g == 1) {
-#pragma unroll
-                for (int p = 0; p < NP; ++p) C.dom1[p] = vals[p];
-            }
-        }
-        if (!apt_active<NP>(vals)) continue;
-        nact += 1;
-        if constexpr (!SMALL) {
-            if (nact <= keep && nact <= A) {
-                C.nsol = nact;
-                C.gid[nact] = (int)g;
-                for (int p = 0; p < NP; ++p) C.v[nact][p] = vals[p];
-            }
-        }
-    }
-    *ntot = nact + (C.liq ? 1 : 0);
-    int kept = nact < keep ? nact : keep;
-    *fits = (kept <= A);
-}
-
-// Wave-level statistics.  All 64 lanes must be active.  The maximum is
-// monotone, so a wave only issues the atomic when it would raise it.
-__device__ __forceinline__ void note_stats(Stats *st, int ntot, int kmax, int step, int halt_mode)
-{
-    int mx = ntot;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        int o = __shfl_xor(mx, off, 64);
-        mx = o > mx ? o : mx;
-    }
-    if ((threadIdx.x & 63) == 0 && mx > *((volatile int *)&st->max_active)) atomicMax(&st->max_active, mx);
+    if (ntot > *((volatile int *)&st->max_active)) atomicMax(&st->max_active, ntot);
     if (ntot > kmax) {
         atomicMax(&st->kmax_overflow, 1);
         atomicAdd(&st->overflow_cells, 1ull);
@@ -260,39 +58,21 @@ __device__ __forceinline__ void note_stats(Stats *st, int ntot, int kmax, int st
     }
 }
 
-// Append the flagged lanes of a wave to the worklist with one atomic.
-__device__ __forceinline__ void push_worklist(bool flag, int cell, int *worklist, int *count)
+// Append the flagged lanes of a wave to a work list with one atomic.  All 64
+// lanes of the wave must be active.
+__device__ __forceinline__ void push_list(bool flag, int cell, int *list, int *count)
 {
     unsigned long long ball = __ballot(flag);
     if (ball == 0ull) return;
     int lane = threadIdx.x & 63;
+    int leader = __ffsll((long long)ball) - 1;
     int base = 0;
-    if (lane == __ffsll((long long)ball) - 1) base = atomicAdd(count, __popcll(ball));
-    base = __shfl(base, __ffsll((long long)ball) - 1, 64);
+    if (lane == leader) base = atomicAdd(count, __popcll(ball));
+    base = __shfl(base, leader, 64);
     if (flag) {
         unsigned long long below = ball & ((1ull << lane) - 1ull);
-        worklist[base + __popcll(below)] = cell;
+        list[base + __popcll(below)] = cell;
     }
-}
-
-template <int DIM, int A>
-__device__ __forceinline__ void write_cell(const SlotView<Real> &O, size_t ncell, size_t c,
-                                           const Cell<Real, DIM, A> &C)
-{
-    constexpr bool SMALL = (A <= 8);
-    constexpr int UNR_A = SMALL ? A : 1;
-    O.liq[c] = C.out[0];
-    int w = 0;
-#pragma unroll UNR_A
-    for (int u = 1; u <= A; ++u) {
-        if (u <= C.nsol && C.out[u] != Real(0.0)) {
-            size_t o = (size_t)w * ncell + c;
-            O.sid[o] = (uint16_t)C.gid[u];
-            O.sval[o] = C.out[u];
-            w += 1;
-        }
-    }
-    O.cnt[c] = (uint8_t)w;
 }
 
 // Tile -> cells.  Consecutive blocks on one XCD (blocks are dealt round-robin
@@ -304,52 +84,280 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
-template <int DIM>
-__global__ __launch_bounds__(BX * BY) void k_step(StepArgs<Real> a)
+__device__ __forceinline__ int *wl_counter(const StepArgs<Real> &a, int list, int step)
 {
-    if (a.halt_mode && a.stats->halt_step < a.step) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.wl_count[(a.step + 1) & 1] = 0;
-    const Consts<Real> &K = a.K;
-    int lb = xcd_remap(blockIdx.x, gridDim.x);
-    int tx = lb % a.ntx;
-    int rest = lb / a.ntx;
-    int ty = rest % a.nty;
-    int k = rest / a.nty;
-    int l = tx * BX + (threadIdx.x % BX);
-    int m = ty * BY + (threadIdx.x / BX);
-    const bool valid = (l < K.nx) && (m < K.ny);
-    const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
-    const size_t c = valid ? cidx(l, m, k, K.nx, K.ny) : 0;
-
-    // temperature, src/gpu_kernels_3d.py:833-838 (applied before the update)
-    double Tn = 0.0;
-    Cell<Real, DIM, FAST_A> C;
-    int ntot = 0;
-    bool fits = true;
-    if (valid) {
-        size_t ti = a.temp_dense ? c : (size_t)k;
-        double Tin = a.T_in[ti];
-        Tn = a.temp_f32 ? (double)((float)Tin - (float)a.cooling) : Tin - a.cooling;
-        if (a.temp_dense || (l == 0 && m == 0)) a.T_out[ti] = Tn;
-        gather<DIM, FAST_A>(K, a.in, ncell, l, m, k, C, &ntot, &fits);
-    }
-    note_stats(a.stats, ntot, K.kmax, a.step, a.halt_mode);
-    push_worklist(valid && !fits, (int)c, a.worklist, &a.wl_count[a.step & 1]);
-    if (!valid || !fits) return;
-    SlotLiq L{a.in.liq, K.nx, K.ny};
-    cell_update<Real, DIM, FAST_A>(K, L, l, m, k, Tn, C);
-    write_cell<DIM, FAST_A>(a.out, ncell, c, C);
+    return a.wl_count + list * 2 + (step & 1);
 }
 
+// Pass 1 over every cell: temperature, classification and the cells whose
+// stencil holds only liquid (no stored grain at any of the 7 points) or one
+// grain with zero liquid everywhere.  Everything else is queued for k_merge.
+//
+// A block owns an LTX x LTY column tile and marches up LZC z-planes.  The
+// cell's own column is carried in registers across planes (z-1, z, z+1); the
+// x/y neighbours of plane z come from an LDS tile with a one-cell halo.  The
+// queued cells of the whole block are collected in LDS and appended to the
+// work list with a single atomic.
+constexpr int LTX = 64, LTY = 8, LZC = 16;
+
+struct ColCell {
+    int cnt;
+    Real liq;
+    uint32_t sid;   // slot-0 grain (NO_ID when empty)
+    Real sv;        // slot-0 value
+};
+
+__device__ __forceinline__ ColCell load_col_cell(const SlotView<Real> &S, size_t c)
+{
+    ColCell r;
+    r.cnt = S.cnt[c];
+    r.liq = S.liq[c];
+    r.sid = NO_ID;
+    r.sv = Real(0.0);
+    if (r.cnt >= 1) {
+        r.sid = S.sid[c];
+        r.sv = S.sval[c];
+    }
+    return r;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
+{
+    if (a.halt_mode && a.stats->halt_step < a.step) return;
+    if (blockIdx.x == 0 && threadIdx.x < 3) a.wl_count[threadIdx.x * 2 + ((a.step + 1) & 1)] = 0;
+    const Consts<Real> &K = a.K;
+    constexpr int NP = (DIM == 3) ? 7 : 5;
+    constexpr int ZC = (DIM == 3) ? LZC : 1;
+    __shared__ uint8_t s_cnt[LTY + 2][LTX + 2];
+    __shared__ Real s_liq[LTY + 2][LTX + 2];
+    __shared__ uint16_t s_sid[LTY + 2][LTX + 2];
+    __shared__ Real s_sv[LTY + 2][LTX + 2];
+    __shared__ int s_q[LTX * LTY * ZC];
+    __shared__ int s_qn, s_qbase, s_any;
+
+    int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = lb % a.ntx;
+    const int rest = lb / a.ntx;
+    const int ty = rest % a.nty;
+    const int tz = rest / a.nty;
+    const int lx = threadIdx.x % LTX, ly = threadIdx.x / LTX;
+    const int l0 = tx * LTX, m0 = ty * LTY;
+    const int w = min(LTX, K.nx - l0), h = min(LTY, K.ny - m0);
+    const int l = l0 + lx, m = m0 + ly;
+    const bool valid = (lx < w) && (ly < h);
+    const int k0 = tz * ZC;
+    const int k1 = (DIM == 3) ? min(k0 + ZC, K.nz) : 1;
+    const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
+    if (threadIdx.x == 0) {
+        s_qn = 0;
+        s_any = 0;
+    }
+    // halo cells this thread loads each plane: x halo of its row, y halo of
+    // its column (x periodic; y periodic in 3D, mirror in 2D)
+    int hxl = -1, hxs = 0;
+    if (ly < h && lx < 2) {
+        hxs = (lx == 0) ? 0 : w + 1;
+        hxl = (lx == 0) ? bc_pm(l0, K.nx) : bc_pp(l0 + w - 1, K.nx);
+    }
+    int hym = -1, hys = 0;
+    if (lx < w && ly < 2) {
+        hys = (ly == 0) ? 0 : h + 1;
+        if (DIM == 3)
+            hym = (ly == 0) ? bc_pm(m0, K.ny) : bc_pp(m0 + h - 1, K.ny);
+        else
+            hym = (ly == 0) ? bc_mm(m0, K.ny) : bc_mp(m0 + h - 1, K.ny);
+    }
+
+    ColCell cp, cc, cn;
+    if (valid) {
+        cc = load_col_cell(a.in, cidx(l, m, k0, K.nx, K.ny));
+        if (DIM == 3) cp = load_col_cell(a.in, cidx(l, m, bc_mm(k0, K.nz), K.nx, K.ny));
+    }
+    for (int k = k0; k < k1; ++k) {
+        if (valid && DIM == 3) cn = load_col_cell(a.in, cidx(l, m, bc_mp(k, K.nz), K.nx, K.ny));
+        if (valid) {
+            s_cnt[ly + 1][lx + 1] = (uint8_t)cc.cnt;
+            s_liq[ly + 1][lx + 1] = cc.liq;
+            s_sid[ly + 1][lx + 1] = (uint16_t)cc.sid;
+            s_sv[ly + 1][lx + 1] = cc.sv;
+        }
+        if (hxl >= 0) {
+            ColCell t = load_col_cell(a.in, cidx(hxl, m, k, K.nx, K.ny));
+            s_cnt[ly + 1][hxs] = (uint8_t)t.cnt;
+            s_liq[ly + 1][hxs] = t.liq;
+            s_sid[ly + 1][hxs] = (uint16_t)t.sid;
+            s_sv[ly + 1][hxs] = t.sv;
+        }
+        if (hym >= 0) {
+            ColCell t = load_col_cell(a.in, cidx(l, hym, k, K.nx, K.ny));
+            s_cnt[hys][lx + 1] = (uint8_t)t.cnt;
+            s_liq[hys][lx + 1] = t.liq;
+            s_sid[hys][lx + 1] = (uint16_t)t.sid;
+            s_sv[hys][lx + 1] = t.sv;
+        }
+        __syncthreads();
+        bool queue = false;
+        size_t c = 0;
+        if (valid) {
+            c = cidx(l, m, k, K.nx, K.ny);
+            size_t ti = a.temp_dense ? c : (size_t)k;
+            if (a.temp_dense || (l == 0 && m == 0)) {
+                double Tin = a.T_in[ti];
+                a.T_out[ti] = a.temp_f32 ? (double)((float)Tin - (float)a.cooling) : Tin - a.cooling;
+            }
+            // stencil: c, x+, x-, y+, y-, z+, z-
+            int pcnt[NP];
+            Real pl[NP];
+            uint32_t psid[NP];
+            Real psv[NP];
+            const int yy = ly + 1, xx = lx + 1;
+            pcnt[P_C] = cc.cnt; pl[P_C] = cc.liq; psid[P_C] = cc.sid; psv[P_C] = cc.sv;
+            pcnt[P_XP] = s_cnt[yy][xx + 1]; pl[P_XP] = s_liq[yy][xx + 1];
+            psid[P_XP] = s_sid[yy][xx + 1]; psv[P_XP] = s_sv[yy][xx + 1];
+            pcnt[P_XM] = s_cnt[yy][xx - 1]; pl[P_XM] = s_liq[yy][xx - 1];
+            psid[P_XM] = s_sid[yy][xx - 1]; psv[P_XM] = s_sv[yy][xx - 1];
+            pcnt[P_YP] = s_cnt[yy + 1][xx]; pl[P_YP] = s_liq[yy + 1][xx];
+            psid[P_YP] = s_sid[yy + 1][xx]; psv[P_YP] = s_sv[yy + 1][xx];
+            pcnt[P_YM] = s_cnt[yy - 1][xx]; pl[P_YM] = s_liq[yy - 1][xx];
+            psid[P_YM] = s_sid[yy - 1][xx]; psv[P_YM] = s_sv[yy - 1][xx];
+            if constexpr (DIM == 3) {
+                pcnt[P_ZP] = cn.cnt; pl[P_ZP] = cn.liq; psid[P_ZP] = cn.sid; psv[P_ZP] = cn.sv;
+                pcnt[P_ZM] = cp.cnt; pl[P_ZM] = cp.liq; psid[P_ZM] = cp.sid; psv[P_ZM] = cp.sv;
+            }
+            int sum = 0;
+            bool single = true;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                sum += pcnt[p];
+                single = single && (pcnt[p] == 1) && (psid[p] == psid[0]) && (pl[p] == Real(0.0));
+            }
+            if (sum == 0) {
+                // only liquid in the stencil: active set {L} or empty
+                bool act = apt_active<NP>(pl);
+                Real ln, sn;
+                single_phase_update(pl[P_C], act, true, K.dt, ln, sn);
+                a.out.liq[c] = ln;
+                a.out.cnt[c] = 0;
+                if (act) s_any = 1;
+            } else if (single) {
+                // one grain, no liquid: active set {g} or empty
+                bool act = apt_active<NP>(psv);
+                Real ln, sn;
+                single_phase_update(psv[P_C], act, false, K.dt, ln, sn);
+                a.out.liq[c] = ln;
+                if (sn != Real(0.0)) {
+                    a.out.sid[c] = (uint16_t)psid[P_C];
+                    a.out.sval[c] = sn;
+                    a.out.cnt[c] = 1;
+                } else {
+                    a.out.cnt[c] = 0;
+                }
+                if (act) s_any = 1;
+            } else {
+                queue = true;
+            }
+        }
+        // append queued cells to the block's LDS list
+        unsigned long long ball = __ballot(queue);
+        if (ball != 0ull) {
+            int lane = threadIdx.x & 63;
+            int leader = __ffsll((long long)ball) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&s_qn, __popcll(ball));
+            base = __shfl(base, leader, 64);
+            if (queue) s_q[base + __popcll(ball & ((1ull << lane) - 1ull))] = (int)c;
+        }
+        __syncthreads();
+        cp = cc;
+        cc = cn;
+    }
+    if (threadIdx.x == 0) {
+        if (s_any && *((volatile int *)&a.stats->max_active) < 1) atomicMax(&a.stats->max_active, 1);
+        s_qbase = (s_qn > 0) ? atomicAdd(wl_counter(a, WL_MERGE, a.step), s_qn) : 0;
+    }
+    __syncthreads();
+    const int nq = s_qn;
+    int *dst = a.wl[WL_MERGE] + s_qbase;
+    for (int i = threadIdx.x; i < nq; i += LTX * LTY) dst[i] = s_q[i];
+    (void)ncell;
+}
+
+// Pass 2 over the queued cells: full APT merge.  Cells that need the
+// solid-liquid anisotropy terms go to k_front, cells with more active grains
+// than the register capacity to k_step_general; the rest are finished here.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
+{
+    if (a.halt_mode && a.stats->halt_step < a.step) return;
+    const Consts<Real> &K = a.K;
+    const int n = *wl_counter(a, WL_MERGE, a.step);
+    const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
+    const bool need_terms = K.aniso || DIM == 2;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        int i = base + threadIdx.x;
+        bool valid = i < n;
+        size_t c = valid ? (size_t)a.wl[WL_MERGE][i] : 0;
+        int l = (int)(c % K.nx);
+        int m = (int)((c / K.nx) % K.ny);
+        int k = (int)(c / ((size_t)K.nx * K.ny));
+        Cell<Real, DIM, FAST_A> C;
+        int ntot = 0;
+        bool fits = true, front = false;
+        if (valid) {
+            gather<Real, DIM, FAST_A>(K, a.in, ncell, l, m, k, C, &ntot, &fits);
+            note_active(a.stats, ntot, K.kmax, a.step, a.halt_mode);
+            front = fits && need_terms && C.liq && C.nsol > 0;
+            if (fits && !front) {
+                size_t ti = a.temp_dense ? c : (size_t)k;
+                SlotLiq<Real> L{a.in.liq, K.nx, K.ny};
+                cell_update<Real, DIM, FAST_A, false>(K, L, l, m, k, a.T_out[ti], C);
+                write_cell<Real, DIM, FAST_A>(a.out, ncell, c, C);
+            }
+        }
+        push_list(valid && front, (int)c, a.wl[WL_FRONT], wl_counter(a, WL_FRONT, a.step));
+        push_list(valid && !fits, (int)c, a.wl[WL_GENERAL], wl_counter(a, WL_GENERAL, a.step));
+    }
+}
+
+// Pass 3: solid-liquid interface cells, compacted so that every lane of a
+// wavefront evaluates the anisotropy terms.
+template <int DIM>
+__global__ __launch_bounds__(128) void k_front(StepArgs<Real> a)
+{
+    if (a.halt_mode && a.stats->halt_step < a.step) return;
+    const Consts<Real> &K = a.K;
+    const int n = *wl_counter(a, WL_FRONT, a.step);
+    const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        size_t c = (size_t)a.wl[WL_FRONT][i];
+        int l = (int)(c % K.nx);
+        int m = (int)((c / K.nx) % K.ny);
+        int k = (int)(c / ((size_t)K.nx * K.ny));
+        Cell<Real, DIM, FAST_A> C;
+        int ntot;
+        bool fits;
+        gather<Real, DIM, FAST_A>(K, a.in, ncell, l, m, k, C, &ntot, &fits);
+        size_t ti = a.temp_dense ? c : (size_t)k;
+        SlotLiq<Real> L{a.in.liq, K.nx, K.ny};
+        cell_update<Real, DIM, FAST_A, true>(K, L, l, m, k, a.T_out[ti], C);
+        write_cell<Real, DIM, FAST_A>(a.out, ncell, c, C);
+    }
+}
+
+// Pass 4: cells with more active grains than the merge pass holds.  The
+// arrays live in scratch memory with room for KMAX phases; the loops run over
+// the occupied entries only.
 template <int DIM>
 __global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
     const Consts<Real> &K = a.K;
-    const int n = a.wl_count[a.step & 1];
+    const int n = *wl_counter(a, WL_GENERAL, a.step);
     const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->slow_cells, (unsigned long long)n);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        size_t c = (size_t)a.worklist[i];
+        size_t c = (size_t)a.wl[WL_GENERAL][i];
         int l = (int)(c % K.nx);
         int m = (int)((c / K.nx) % K.ny);
         int k = (int)(c / ((size_t)K.nx * K.ny));
@@ -358,11 +366,10 @@ __global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
         Cell<Real, DIM, KCAP> C;
         int ntot;
         bool fits;
-        gather<DIM, KCAP>(K, a.in, ncell, l, m, k, C, &ntot, &fits);
-        SlotLiq L{a.in.liq, K.nx, K.ny};
+        gather<Real, DIM, KCAP>(K, a.in, ncell, l, m, k, C, &ntot, &fits);
+        SlotLiq<Real> L{a.in.liq, K.nx, K.ny};
         cell_update<Real, DIM, KCAP>(K, L, l, m, k, Tn, C);
-        write_cell<DIM, KCAP>(a.out, ncell, c, C);
-        if (i == 0) atomicAdd(&a.stats->slow_cells, (unsigned long long)n);
+        write_cell<Real, DIM, KCAP>(a.out, ncell, c, C);
     }
 }
 
@@ -381,8 +388,8 @@ __global__ void k_apt_slots(Consts<Real> K, SlotView<Real> S, int32_t *mf, int32
     int m = (int)((c / K.nx) % K.ny);
     int k = (int)(c / ((size_t)K.nx * K.ny));
     size_t r = (DIM == 3) ? ridx(l, m, k, K.ny, K.nz) : ((size_t)l * K.ny + m);
-    Merge<DIM> M;
-    stencil_cells<DIM>(K, l, m, k, M.pc);
+    Merge<Real, DIM> M;
+    stencil_cells<Real, DIM>(K, l, m, k, M.pc);
     Real lv[NP];
     for (int p = 0; p < NP; ++p) lv[p] = S.liq[M.pc[p]];
     int n = 0;
@@ -613,7 +620,7 @@ __device__ __forceinline__ void dense_cell(const Consts<Real> &K, const Real *ph
         }
         C.nsol = u;
     }
-    DenseLiq L{phi, K.ny, NZ};
+    DenseLiq<Real> L{phi, K.ny, NZ};
     cell_update<Real, DIM, A>(K, L, l, m, k, Tcell, C);
     for (int g = 0; g < K.N; ++g) phi_new[(size_t)g * ncell + r] = Real(0.0);
     phi_new[r] = C.out[0];
@@ -722,7 +729,7 @@ __global__ __launch_bounds__(256) void k_observe(SlotView<Real> S, int nx, int n
     for (int g = threadIdx.x; g < N; g += blockDim.x) atomicAdd(&gvol[g], vol[g]);
 }
 
-__global__ void k_front(const Real *liq, int nx, int ny, int nz, int32_t *front)
+__global__ void k_front_height(const Real *liq, int nx, int ny, int nz, int32_t *front)
 {
     size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= (size_t)nx * ny) return;
@@ -749,20 +756,23 @@ static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1
 
 extern "C" {
 
-hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, int general_blocks,
-                                     hipStream_t st)
+hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const int *grids, hipStream_t st)
 {
     StepArgs<Real> a = *args;
-    const int nz = dim == 3 ? a.K.nz : 1;
-    a.ntx = (a.K.nx + BX - 1) / BX;
-    a.nty = (a.K.ny + BY - 1) / BY;
-    unsigned grid = (unsigned)a.ntx * a.nty * nz;
+    a.ntx = (a.K.nx + LTX - 1) / LTX;
+    a.nty = (a.K.ny + LTY - 1) / LTY;
+    const int nzc = dim == 3 ? (a.K.nz + LZC - 1) / LZC : 1;
+    unsigned grid = (unsigned)a.ntx * a.nty * nzc;
     if (dim == 3) {
-        hipLaunchKernelGGL(k_step<3>, dim3(grid), dim3(BX * BY), 0, st, a);
-        hipLaunchKernelGGL(k_step_general<3>, dim3(general_blocks), dim3(64), 0, st, a);
+        hipLaunchKernelGGL(k_light<3>, dim3(grid), dim3(LTX * LTY), 0, st, a);
+        hipLaunchKernelGGL(k_merge<3>, dim3(grids[0]), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(k_front<3>, dim3(grids[1]), dim3(128), 0, st, a);
+        hipLaunchKernelGGL(k_step_general<3>, dim3(grids[2]), dim3(64), 0, st, a);
     } else {
-        hipLaunchKernelGGL(k_step<2>, dim3(grid), dim3(BX * BY), 0, st, a);
-        hipLaunchKernelGGL(k_step_general<2>, dim3(general_blocks), dim3(64), 0, st, a);
+        hipLaunchKernelGGL(k_light<2>, dim3(grid), dim3(LTX * LTY), 0, st, a);
+        hipLaunchKernelGGL(k_merge<2>, dim3(grids[0]), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(k_front<2>, dim3(grids[1]), dim3(128), 0, st, a);
+        hipLaunchKernelGGL(k_step_general<2>, dim3(grids[2]), dim3(64), 0, st, a);
     }
     return hipGetLastError();
 }
@@ -867,7 +877,7 @@ hipError_t MPF_NAME(mpf_launch_observe)(SlotView<Real> S, int nx, int ny, int nz
 {
     size_t shm = ((size_t)N + 256 * 6) * sizeof(double);
     hipLaunchKernelGGL(k_observe, dim3(nz), dim3(256), shm, st, S, nx, ny, nz, N, gvol, part);
-    hipLaunchKernelGGL(k_front, dim3(nblk((size_t)nx * ny, 256)), dim3(256), 0, st, S.liq, nx, ny, nz, front);
+    hipLaunchKernelGGL(k_front_height, dim3(nblk((size_t)nx * ny, 256)), dim3(256), 0, st, S.liq, nx, ny, nz, front);
     return hipGetLastError();
 }
 

@@ -25,6 +25,7 @@ namespace mpf {
 constexpr int KCAP = 64;        // largest KMAX supported
 constexpr uint32_t NO_ID = 0xFFFFu;
 constexpr int NO_HALT = 0x7fffffff; // halt_step value while no step has overflowed
+enum { WL_MERGE = 0, WL_FRONT = 1, WL_GENERAL = 2 };
 
 template <typename Real>
 struct SlotView {
@@ -52,8 +53,14 @@ struct StepArgs {
     int temp_dense;  // 1: temperature per cell; 0: one value per z-plane
     int temp_f32;    // 1: the temperature is a float32 quantity
     double cooling;
-    int *worklist;   // cells left for the general path
-    int *wl_count;   // [2] ping-pong counters
+    // Work lists of one step (cell indices).  A step runs four kernels: the
+    // classification pass over all cells finishes cells whose stencil holds
+    // only liquid or only one grain and queues the rest (WL_MERGE); the merge
+    // pass finishes cells that need no anisotropy terms and queues
+    // solid-liquid interface cells (WL_FRONT) and cells with more than the
+    // register capacity of active grains (WL_GENERAL).
+    int *wl[3];      // WL_MERGE, WL_FRONT, WL_GENERAL; ncell entries each
+    int *wl_count;   // [3][2] counters, ping-pong on the step parity
     int step;        // step index (for the ping-pong counters and halt)
     int halt_mode;   // stop at the first KMAX overflow
     Stats *stats;

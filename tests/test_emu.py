@@ -1,0 +1,87 @@
+"""The solver's per-cell device code compiled for the host (tests/emu) against
+the oracle, step by step: FP64 bit-exact on every kernel path (single-phase
+stencils, the 4-grain register path, the scratch path), FP32 within 1e-4."""
+import numpy as np
+import pytest
+
+import cases
+from emu import emu
+from oracle import oracle as O
+
+
+def _run(phi, temp, tabs, scal, steps, mode=0, precision="fp64", aniso=True, torque=True):
+    cur_ref = phi.copy()
+    cur = phi.copy()
+    used = np.zeros(4, int)
+    for _ in range(steps):
+        mf, nf, _ = O.apt(cur_ref, 50)
+        ref = O.update(cur_ref, temp, mf, nf, tabs, scal, aniso=aniso, torque=torque)
+        out, paths, _ = emu.step(cur if precision == "fp64" else cur_ref, temp, tabs, scal, mode=mode,
+                                 precision=precision, aniso=aniso, torque=torque)
+        used += paths
+        yield ref, out, used
+        cur_ref, cur = ref, out
+
+
+@pytest.mark.parametrize("seed,n_solid", [(17, 4), (3, 7), (11, 12)])
+@pytest.mark.parametrize("mode", [0, 2])
+def test_fp64_voronoi_paths_bit_exact(seed, n_solid, mode):
+    c = cases.voronoi_case(nx=10, ny=9, nz=12, n_solid=n_solid, seed=seed, qseed=seed + 1)
+    for ref, out, used in _run(c["phi"], c["temp"], c["tables"], c["scalars"], 3, mode=mode):
+        assert np.array_equal(out, ref)
+    if mode == 0:
+        assert used[0] > 0 and used[1] > 0
+
+
+def test_fp64_quad_junction_and_many_phase_cells():
+    """A four-grain junction meeting the front (liquid + 4 solids, the last
+    register slot) and a blob of up to 8 grains (scratch path)."""
+    p = cases.ref_params()
+    d, _ = cases.tables_for(p, np.tile([0, 0, 0, 1.0], (5, 1)))
+    nx, ny, nz = 8, 8, 12
+    gm = np.ones((nx, ny), np.int32)
+    gm[4:, :4], gm[:4, 4:], gm[4:, 4:] = 2, 3, 4
+    phi = cases.inputs.init_phi_from_grain_map_3d(gm, 4, nx, ny, nz, p["dx"], d["delta"], 5).astype(np.float64)
+    rng = np.random.default_rng(2)
+    q = np.vstack([[0, 0, 0, 1.0], rng.standard_normal((4, 4))])
+    q[1:] /= np.linalg.norm(q[1:], axis=1, keepdims=True)
+    d, tabs = cases.tables_for(p, q)
+    scal = cases.inputs.kernel_scalars(p, d)
+    temp = np.full((nx, ny, nz), p["T_melt"] - 2.0)
+    for ref, out, used in _run(phi, temp, tabs, scal, 4):
+        assert np.array_equal(out, ref)
+    _, nf, st = O.apt(ref, 50)
+    assert st[1] >= 5
+
+    N = 9
+    phi = np.zeros((N, 8, 8, 8))
+    phi[0] = 0.3
+    phi[1:, 2:6, 2:6, 2:6] = rng.uniform(0.0, 1.0, (N - 1, 4, 4, 4))
+    phi[1:] = np.where(phi[1:] < 0.35, 0.0, phi[1:])
+    phi /= phi.sum(0)
+    q = np.vstack([[0, 0, 0, 1.0], rng.standard_normal((N - 1, 4))])
+    q[1:] /= np.linalg.norm(q[1:], axis=1, keepdims=True)
+    tabs = cases.tables_for(p, q)[1]
+    temp = np.full((8, 8, 8), 1682.0)
+    for ref, out, used in _run(phi, temp, tabs, scal, 3):
+        assert np.array_equal(out, ref)
+    assert used[3] > 0
+
+
+@pytest.mark.parametrize("aniso,torque", [(True, False), (False, False)])
+def test_fp64_switchable_bit_exact(aniso, torque):
+    c = cases.voronoi_case()
+    for ref, out, _ in _run(c["phi"], c["temp"], c["tables"], c["scalars"], 3, aniso=aniso, torque=torque):
+        assert np.array_equal(out, ref)
+
+
+def test_fp64_2d_bit_exact():
+    c = cases.disk2d_case(n=32, radius=6.0)
+    for ref, out, _ in _run(c["phi"], c["temp"], c["tables"], c["scalars"], 5):
+        assert np.array_equal(out, ref)
+
+
+def test_fp32_step_error_small():
+    c = cases.voronoi_case()
+    for ref, out, _ in _run(c["phi"], c["temp"], c["tables"], c["scalars"], 3, precision="fp32"):
+        assert np.abs(out - ref).max() <= 1e-4

@@ -203,3 +203,40 @@ def test_observables_match_numpy():
         nz = phi.shape[3]
         pos = np.where(mask.any(2), nz - 1 - np.argmax(mask[:, :, ::-1], axis=2), np.nan)
         np.testing.assert_array_equal(ob["front"], pos)
+
+
+@pytest.mark.parametrize("seed,n_solid", [(3, 7), (11, 12)])
+def test_fp64_voronoi_seeds_bit_exact(seed, n_solid):
+    c = cases.voronoi_case(nx=10, ny=9, nz=12, n_solid=n_solid, seed=seed, qseed=seed + 1)
+    steps = 8
+    ref, _, _, st = O.run(c["phi"], c["temp"], steps, c["tables"], c["scalars"], kmax=50, temp_f32=True)
+    with _solver_for(c, "fp64", temp_f32=True) as s:
+        assert s.step(steps) == steps
+        assert np.array_equal(s.phi(), ref)
+        _assert_apt_equal(s, ref)
+
+
+def test_fp64_quad_junction_bit_exact():
+    """Four grains meeting at a column that crosses the front: cells with the
+    liquid and four solids active fill the last register slot."""
+    p = cases.ref_params()
+    d, _ = cases.tables_for(p, np.tile([0, 0, 0, 1.0], (5, 1)))
+    nx, ny, nz = 8, 8, 12
+    gm = np.ones((nx, ny), np.int32)
+    gm[4:, :4], gm[:4, 4:], gm[4:, 4:] = 2, 3, 4
+    phi = cases.inputs.init_phi_from_grain_map_3d(gm, 4, nx, ny, nz, p["dx"], d["delta"], 5).astype(np.float64)
+    rng = np.random.default_rng(2)
+    q = np.vstack([[0, 0, 0, 1.0], rng.standard_normal((4, 4))])
+    q[1:] /= np.linalg.norm(q[1:], axis=1, keepdims=True)
+    d, tabs = cases.tables_for(p, q)
+    scal = cases.inputs.kernel_scalars(p, d)
+    temp = np.full((nx, ny, nz), p["T_melt"] - 2.0)
+    ref, _, _, st = O.run(phi, temp, 6, tabs, scal, kmax=50)
+    assert st[1] >= 5
+    s = Solver((nx, ny, nz), 5, scal, precision="fp64")
+    s.set_tables(*tabs)
+    s.upload_phi(phi)
+    s.upload_temp(temp)
+    assert s.step(6) == 6
+    assert np.array_equal(s.phi(), ref)
+    s.close()
