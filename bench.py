@@ -118,12 +118,22 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the solver has no "
               "multi-GPU path in this build.", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        from mpf import dense
+        n = 256 << 20
+        src, dst = dense.DeviceArray((n,), np.uint8), dense.DeviceArray((n,), np.uint8)
+        from mpf import _lib
+        _lib.check(_lib.lib().mpf_memcpy_dtod(dst.ptr, src.ptr, n))
+        dense.synchronize()
+        log("copy probe: 256 MiB device-to-device copy done")
     W = workload()
     ncell = NX * NY * NZ
     log(f"workload ready: {NX}x{NY}x{NZ}, N={W['N']}")
@@ -147,7 +157,9 @@ def main():
                      "kernel": "k_step (fused temperature+APT+RHS+update) + k_step_general"},
         "wall_s_timed": round(r32["wall"], 3),
         "front_mean_cells": r32["ob1"]["front_mean"],
-        "general_cells": r32["stats"]["general_cells"],
+        "cells_per_step": {"merge": r32["stats"]["merge_cells"] // args.steps,
+                           "front": r32["stats"]["front_cells"] // args.steps,
+                           "general": r32["stats"]["general_cells"] // args.steps},
         "max_active": r32["stats"]["max_active"],
     }
     if not args.no_fp64:

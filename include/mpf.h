@@ -78,6 +78,8 @@ typedef struct mpf_stats {
     int halt_step;                  /* -1, or the step index that overflowed */
     int steps_done;                 /* total steps advanced by this handle */
     double last_step_ms;            /* GPU time of the last mpf_step call (hipEvents) */
+    unsigned long long merge_cells; /* cells that needed the full APT merge (summed over steps) */
+    unsigned long long front_cells; /* solid-liquid interface cells (summed over steps) */
 } mpf_stats;
 
 /* ---- resident solver ---------------------------------------------------- */
@@ -157,6 +159,7 @@ int mpf_malloc(void **ptr, size_t bytes);
 int mpf_free(void *ptr);
 int mpf_memcpy_htod(void *dst, const void *src, size_t bytes);
 int mpf_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int mpf_memcpy_dtod(void *dst, const void *src, size_t bytes);
 int mpf_memset(void *ptr, int value, size_t bytes);
 int mpf_device_synchronize(void);
 

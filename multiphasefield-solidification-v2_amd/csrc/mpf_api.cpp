@@ -495,6 +495,8 @@ extern "C" int mpf_get_stats(mpf_handle *h, mpf_stats *st)
     st->halt_step = s.halt_step == NO_HALT ? -1 : s.halt_step;
     st->steps_done = h->steps_done;
     st->last_step_ms = h->last_ms;
+    st->merge_cells = s.merge_cells;
+    st->front_cells = s.front_cells;
     return 0;
 }
 
@@ -646,6 +648,11 @@ extern "C" int mpf_memcpy_htod(void *dst, const void *src, size_t bytes)
 extern "C" int mpf_memcpy_dtoh(void *dst, const void *src, size_t bytes)
 {
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+extern "C" int mpf_memcpy_dtod(void *dst, const void *src, size_t bytes)
+{
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return 0;
 }
 extern "C" int mpf_memset(void *ptr, int value, size_t bytes)

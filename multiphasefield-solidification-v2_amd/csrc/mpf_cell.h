@@ -150,10 +150,8 @@ template <typename Real>
 struct LGrad {
     Real gx, gy, gz; // central differences (grad_phi_xyz)
     Real q;          // |grad|^2
-    Real inv_g;      // 1/sqrt(q)       (valid when q >= g2_floor)
+    Real inv_g;      // 1/sqrt(q)       (valid when q >= g2_floor, else 0)
     Real inv_g3;     // 1/(q*sqrt(q))
-    Real ux, uy, uz; // unit gradient
-    bool ok;         // q >= g2_floor
 };
 
 template <typename Real, int DIM>
@@ -163,17 +161,19 @@ MPF_DEV void lgrad_finish(const Consts<Real> &K, LGrad<Real> &G)
         G.q = G.gx * G.gx + G.gy * G.gy + G.gz * G.gz;
     else
         G.q = G.gx * G.gx + G.gy * G.gy;
-    G.ok = !(G.q < K.g2_floor);
-    if (G.ok) {
+    if (!(G.q < K.g2_floor)) {
         Real sq = m_sqrt(G.q);
         G.inv_g = Real(1.0) / sq;
         G.inv_g3 = Real(1.0) / (G.q * sq);
-        G.ux = G.gx * G.inv_g;
-        G.uy = G.gy * G.inv_g;
-        G.uz = G.gz * G.inv_g;
     } else {
-        G.inv_g = G.inv_g3 = G.ux = G.uy = G.uz = Real(0.0);
+        G.inv_g = G.inv_g3 = Real(0.0);
     }
+}
+
+template <typename Real>
+MPF_DEV bool lgrad_ok(const Consts<Real> &K, const LGrad<Real> &G)
+{
+    return !(G.q < K.g2_floor);
 }
 
 // A12 quantities at one point: a, coef and dc/dphi (da/dphi = mu_delta*coef*dc).
@@ -183,16 +183,16 @@ struct A12v {
 };
 
 // da_dphixyz_A12, src/gpu_kernels_3d.py:304-341 (2D: src/gpu_kernels.py:237-264)
+// a_at_point is calc_a(f.ca), already evaluated for eps^2 at the same point.
 template <typename Real, int DIM>
-MPF_DEV A12v<Real> a12_at(const Consts<Real> &K, const LGrad<Real> &G,
-                                             const Facet<Real> &f)
+MPF_DEV A12v<Real> a12_at(const Consts<Real> &K, const LGrad<Real> &G, const Facet<Real> &f, Real a_at_point)
 {
     A12v<Real> r;
-    if (!G.ok) {
+    if (!lgrad_ok(K, G)) {
         r.a = r.dax = r.day = r.daz = Real(0.0);
         return r;
     }
-    r.a = calc_a(K, f.ca);
+    r.a = a_at_point;
     Real udotg;
     if constexpr (DIM == 3)
         udotg = G.gx * f.nx + G.gy * f.ny + G.gz * f.nz;
@@ -296,42 +296,63 @@ MPF_DEV void build_liq_ctx(const Consts<Real> &K, const LiqAt &L, int l, int m, 
 
 // lap_sl and b_sl of one active solid s (src/gpu_kernels_3d.py:558-577,
 // :709-733; 2D src/gpu_kernels.py:394-408).  Requires K.aniso.
+//
+// The facet search runs once per stencil point; from it come eps^2 at the
+// point (aniso_term1) and the one component of da/dphi that A13 needs there,
+// so only two numbers per point stay live.  The full facet and A12 are kept
+// for the centre (torque).
 template <typename Real, int DIM>
-MPF_DEV void solid_liquid_terms(const Consts<Real> &K, const LiqCtx<Real, DIM> &X,
-                                                int s, Real &lap_sl, Real &b_sl)
+MPF_DEV void solid_liquid_terms(const Consts<Real> &K, const LiqCtx<Real, DIM> &X, int s, Real &lap_sl,
+                                Real &b_sl)
 {
     constexpr int NP = LiqCtx<Real, DIM>::NP;
     Real nrm[24];
 #pragma unroll
     for (int u = 0; u < 24; ++u) nrm[u] = K.n111[(size_t)s * 24 + u];
-    Facet<Real> f[NP];
-    Real e2[NP];
-#pragma unroll
-    for (int p = 0; p < NP; ++p) {
-        // eps2_at_cell_from_liquid_3d (:137-146): best_cos is 0 below the floor.
-        Real bc = Real(0.0);
-        if (X.G[p].ok) {
-            f[p] = facet_search<Real, DIM>(nrm, X.G[p].ux, X.G[p].uy, X.G[p].uz);
-            bc = f[p].ca;
-        } else {
-            f[p].cs = f[p].ca = Real(0.0);
-            f[p].nx = Real(1.0);
-            f[p].ny = f[p].nz = Real(0.0);
-        }
-        Real a = calc_a(K, bc);
+    // One stencil point: facet search, eps^2 and A12 (written out per point so
+    // that every access to the context is static).
+    auto point = [&](const LGrad<Real> &G, Real &e2p, Facet<Real> &f, A12v<Real> &r) {
+        f.cs = f.ca = f.ny = f.nz = Real(0.0);
+        f.nx = Real(1.0);
+        // eps2_at_cell_from_liquid_3d (:137-146): best_cos is 0 below the floor
+        if (lgrad_ok(K, G)) f = facet_search<Real, DIM>(nrm, G.gx * G.inv_g, G.gy * G.inv_g, G.gz * G.inv_g);
+        Real a = calc_a(K, f.ca);
         Real e = K.eps0_sl * a;
-        e2[p] = e * e;
+        e2p = e * e;
+        if (K.torque)
+            r = a12_at<Real, DIM>(K, G, f, a);
+        else
+            r.a = r.dax = r.day = r.daz = Real(0.0);
+    };
+    Facet<Real> fc, fo;
+    A12v<Real> ac, ro;
+    Real e2c, e2xp, e2xm, e2yp, e2ym, e2zp = Real(0.0), e2zm = Real(0.0);
+    Real dxp, dxm, dyp, dym, dzp = Real(0.0), dzm = Real(0.0);
+    point(X.G[P_C], e2c, fc, ac);
+    point(X.G[P_XP], e2xp, fo, ro);
+    dxp = ro.dax;
+    point(X.G[P_XM], e2xm, fo, ro);
+    dxm = ro.dax;
+    point(X.G[P_YP], e2yp, fo, ro);
+    dyp = ro.day;
+    point(X.G[P_YM], e2ym, fo, ro);
+    dym = ro.day;
+    if constexpr (DIM == 3) {
+        point(X.G[P_ZP], e2zp, fo, ro);
+        dzp = ro.daz;
+        point(X.G[P_ZM], e2zm, fo, ro);
+        dzm = ro.daz;
     }
     // aniso_term1_solid_3d (:149-204)
     Real pc = X.v[P_C];
-    Real Fx_p = (e2[P_C] + e2[P_XP]) * (X.v[P_XP] - pc) / K.two_dx;
-    Real Fx_m = (e2[P_C] + e2[P_XM]) * (pc - X.v[P_XM]) / K.two_dx;
-    Real Fy_p = (e2[P_C] + e2[P_YP]) * (X.v[P_YP] - pc) / K.two_dx;
-    Real Fy_m = (e2[P_C] + e2[P_YM]) * (pc - X.v[P_YM]) / K.two_dx;
+    Real Fx_p = (e2c + e2xp) * (X.v[P_XP] - pc) / K.two_dx;
+    Real Fx_m = (e2c + e2xm) * (pc - X.v[P_XM]) / K.two_dx;
+    Real Fy_p = (e2c + e2yp) * (X.v[P_YP] - pc) / K.two_dx;
+    Real Fy_m = (e2c + e2ym) * (pc - X.v[P_YM]) / K.two_dx;
     Real lap_val;
     if constexpr (DIM == 3) {
-        Real Fz_p = (e2[P_C] + e2[P_ZP]) * (X.v[P_ZP] - pc) / K.two_dx;
-        Real Fz_m = (e2[P_C] + e2[P_ZM]) * (pc - X.v[P_ZM]) / K.two_dx;
+        Real Fz_p = (e2c + e2zp) * (X.v[P_ZP] - pc) / K.two_dx;
+        Real Fz_m = (e2c + e2zm) * (pc - X.v[P_ZM]) / K.two_dx;
         lap_val = (Fx_p - Fx_m) / K.dx + (Fy_p - Fy_m) / K.dx + (Fz_p - Fz_m) / K.dx;
     } else {
         lap_val = (Fx_p - Fx_m) / K.dx + (Fy_p - Fy_m) / K.dx;
@@ -339,9 +360,9 @@ MPF_DEV void solid_liquid_terms(const Consts<Real> &K, const LiqCtx<Real, DIM> &
 
     // torque_A11_3d (:386-441); 2D torque_A11 (src/gpu_kernels.py:290-332).
     // Below the gradient floor the torque is 0.0 and is still added.
+    const LGrad<Real> &G = X.G[P_C];
     Real tq = Real(0.0);
-    if (K.torque && X.G[P_C].ok) {
-        const LGrad<Real> &G = X.G[P_C];
+    if (K.torque && lgrad_ok(K, G)) {
         Real phix = G.gx, phiy = G.gy, phiz = G.gz;
         Real q = G.q;
         Real q_x, q_y, q_z;
@@ -354,8 +375,6 @@ MPF_DEV void solid_liquid_terms(const Consts<Real> &K, const LiqCtx<Real, DIM> &
             q_y = Real(2.0) * (phix * X.dxy + phiy * X.dyy);
             q_z = Real(0.0);
         }
-        A12v<Real> ac = a12_at<Real, DIM>(K, G, f[P_C]);
-        const Facet<Real> &fc = f[P_C];
         Real udotg;
         if constexpr (DIM == 3)
             udotg = phix * fc.nx + phiy * fc.ny + phiz * fc.nz;
@@ -368,11 +387,9 @@ MPF_DEV void solid_liquid_terms(const Consts<Real> &K, const LiqCtx<Real, DIM> &
         Real S = m_sqrt(pymax(Real(1.0) - c * c, Real(0.0)) + K.p2);
         Real coef = (c / C) - K.tan_a0 * (c / S);
         Real a_val = ac.a;
-        // A13_divergence_3d (:344-383): da/dphi_p at the face neighbours
-        Real d_dx = (a12_at<Real, DIM>(K, X.G[P_XP], f[P_XP]).dax -
-                     a12_at<Real, DIM>(K, X.G[P_XM], f[P_XM]).dax) * K.half_over_dx;
-        Real d_dy = (a12_at<Real, DIM>(K, X.G[P_YP], f[P_YP]).day -
-                     a12_at<Real, DIM>(K, X.G[P_YM], f[P_YM]).day) * K.half_over_dx;
+        // A13_divergence_3d (:344-383)
+        Real d_dx = (dxp - dxm) * K.half_over_dx;
+        Real d_dy = (dyp - dym) * K.half_over_dx;
         if constexpr (DIM == 3) {
             Real dcz = fc.nz * G.inv_g - udotg * phiz * G.inv_g3;
             Real c_x = dcx * X.dxx + dcy * X.dxy + dcz * X.dxz;
@@ -381,8 +398,7 @@ MPF_DEV void solid_liquid_terms(const Consts<Real> &K, const LiqCtx<Real, DIM> &
             Real da_dx = K.mu_delta * coef * c_x;
             Real da_dy = K.mu_delta * coef * c_y;
             Real da_dz = K.mu_delta * coef * c_z;
-            Real d_dz = (a12_at<Real, DIM>(K, X.G[P_ZP], f[P_ZP]).daz -
-                         a12_at<Real, DIM>(K, X.G[P_ZM], f[P_ZM]).daz) * K.half_over_dx;
+            Real d_dz = (dzp - dzm) * K.half_over_dx;
             Real Ex = (da_dx * ac.dax * q) + (a_val * d_dx * q) + (a_val * ac.dax * q_x);
             Real Ey = (da_dy * ac.day * q) + (a_val * d_dy * q) + (a_val * ac.day * q_y);
             Real Ez = (da_dz * ac.daz * q) + (a_val * d_dz * q) + (a_val * ac.daz * q_z);
@@ -400,7 +416,7 @@ MPF_DEV void solid_liquid_terms(const Consts<Real> &K, const LiqCtx<Real, DIM> &
     if (K.torque) lap_val += tq;
     lap_sl = lap_val;
     // b_sl: kinetic anisotropy from the liquid gradient at the centre
-    b_sl = calc_b(K, X.G[P_C].ok ? f[P_C].ca : Real(0.0));
+    b_sl = calc_b(K, lgrad_ok(K, G) ? fc.ca : Real(0.0));
 }
 
 // Read/write a small register array at a run-time index without dynamic
@@ -486,28 +502,8 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
     // only when the liquid is active (the reference computes them regardless).
     Real w_sl = K.w0_sl;
     if (S.liq && S.nsol > 0) {
-        if (ANISO_TERMS && (K.aniso || DIM == 2)) {
-            if constexpr (ANISO_TERMS) {
-                LiqCtx<Real, DIM> X;
-                build_liq_ctx<Real, DIM>(K, L, l, m, k, X);
-                for (int t = 1; t <= S.nsol; ++t) {
-                    int g = sel_int<NA, SMALL>(S.gid, t);
-                    Real ls, bs;
-                    solid_liquid_terms<Real, DIM>(K, X, g, ls, bs);
-                    put_real<Real, NA, SMALL>(lap_sl, t, ls);
-                    put_real<Real, NA, SMALL>(b_sl, t, bs);
-                }
-            }
-        } else {
-            // anisotropy off (switchable kernel, :729-731): eps0^2 * lap(phi_L)
-#pragma unroll UNR_A
-            for (int t = 1; t < NA; ++t) {
-                if (t > S.nsol) break;
-                lap_sl[t] = K.eps0_sq * lap[0];
-                b_sl[t] = Real(1.0);
-            }
-        }
-
+        // The dominant solid's gradient is taken first so that the neighbour
+        // values of the other phases are dead before the liquid context.
         // dominant solid and anisotropic w_sl (:579-595; 2D :410-436)
         if constexpr (!ANISO_TERMS) {
         } else if constexpr (DIM == 3) {
@@ -578,6 +574,63 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
             }
             Real a_loc = calc_a(K, cmax);
             w_sl = K.w0_sl * (a_loc * a_loc);
+        }
+
+        if (ANISO_TERMS && (K.aniso || DIM == 2)) {
+            if constexpr (ANISO_TERMS) {
+                LiqCtx<Real, DIM> X;
+                build_liq_ctx<Real, DIM>(K, L, l, m, k, X);
+                if constexpr (SMALL) {
+                    // The solids pass through a rotating register queue: A
+                    // iterations (uniform trip count, body emitted once), each
+                    // taking the head id and appending its result, so every
+                    // array access is static.
+                    int gq[A];
+                    Real lq[A], bq[A];
+#pragma unroll
+                    for (int u = 0; u < A; ++u) {
+                        gq[u] = S.gid[u + 1];
+                        lq[u] = Real(0.0);
+                        bq[u] = Real(1.0);
+                    }
+#pragma unroll 1
+                    for (int it = 0; it < A; ++it) {
+                        Real ls = Real(0.0), bs = Real(1.0);
+                        if (it < S.nsol) solid_liquid_terms<Real, DIM>(K, X, gq[0], ls, bs);
+#pragma unroll
+                        for (int u = 0; u < A - 1; ++u) {
+                            gq[u] = gq[u + 1];
+                            lq[u] = lq[u + 1];
+                            bq[u] = bq[u + 1];
+                        }
+                        gq[A - 1] = 0;
+                        lq[A - 1] = ls;
+                        bq[A - 1] = bs;
+                    }
+#pragma unroll
+                    for (int u = 0; u < A; ++u) {
+                        if (u < S.nsol) {
+                            lap_sl[u + 1] = lq[u];
+                            b_sl[u + 1] = bq[u];
+                        }
+                    }
+                } else {
+                    for (int t = 1; t <= S.nsol; ++t) {
+                        Real ls, bs;
+                        solid_liquid_terms<Real, DIM>(K, X, S.gid[t], ls, bs);
+                        lap_sl[t] = ls;
+                        b_sl[t] = bs;
+                    }
+                }
+            }
+        } else {
+            // anisotropy off (switchable kernel, :729-731): eps0^2 * lap(phi_L)
+#pragma unroll UNR_A
+            for (int t = 1; t < NA; ++t) {
+                if (t > S.nsol) break;
+                lap_sl[t] = K.eps0_sq * lap[0];
+                b_sl[t] = Real(1.0);
+            }
         }
     }
 

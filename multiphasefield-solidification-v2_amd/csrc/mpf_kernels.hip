@@ -129,11 +129,12 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
     const Consts<Real> &K = a.K;
     constexpr int NP = (DIM == 3) ? 7 : 5;
     constexpr int ZC = (DIM == 3) ? LZC : 1;
-    __shared__ uint8_t s_cnt[LTY + 2][LTX + 2];
-    __shared__ Real s_liq[LTY + 2][LTX + 2];
-    __shared__ uint16_t s_sid[LTY + 2][LTX + 2];
-    __shared__ Real s_sv[LTY + 2][LTX + 2];
-    __shared__ int s_q[LTX * LTY * ZC];
+    // plane tiles, double-buffered so one barrier per plane suffices
+    __shared__ uint8_t s_cnt[2][LTY + 2][LTX + 2];
+    __shared__ Real s_liq[2][LTY + 2][LTX + 2];
+    __shared__ uint16_t s_sid[2][LTY + 2][LTX + 2];
+    __shared__ Real s_sv[2][LTY + 2][LTX + 2];
+    __shared__ uint16_t s_q[LTX * LTY * ZC]; // queued cells as (z-plane, row, column) in the tile
     __shared__ int s_qn, s_qbase, s_any;
 
     int lb = xcd_remap(blockIdx.x, gridDim.x);
@@ -148,7 +149,6 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
     const bool valid = (lx < w) && (ly < h);
     const int k0 = tz * ZC;
     const int k1 = (DIM == 3) ? min(k0 + ZC, K.nz) : 1;
-    const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
     if (threadIdx.x == 0) {
         s_qn = 0;
         s_any = 0;
@@ -169,38 +169,51 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
             hym = (ly == 0) ? bc_mm(m0, K.ny) : bc_mp(m0 + h - 1, K.ny);
     }
 
-    ColCell cp, cc, cn;
+    // Software pipeline: the own column (z-1, z, z+1) and the halo of the
+    // current plane are in registers; the loads of the next plane are issued
+    // before the current plane is processed.
+    ColCell cp, cc, cn, hx, hy;
     if (valid) {
         cc = load_col_cell(a.in, cidx(l, m, k0, K.nx, K.ny));
-        if (DIM == 3) cp = load_col_cell(a.in, cidx(l, m, bc_mm(k0, K.nz), K.nx, K.ny));
+        if (DIM == 3) {
+            cp = load_col_cell(a.in, cidx(l, m, bc_mm(k0, K.nz), K.nx, K.ny));
+            cn = load_col_cell(a.in, cidx(l, m, bc_mp(k0, K.nz), K.nx, K.ny));
+        }
     }
+    if (hxl >= 0) hx = load_col_cell(a.in, cidx(hxl, m, k0, K.nx, K.ny));
+    if (hym >= 0) hy = load_col_cell(a.in, cidx(l, hym, k0, K.nx, K.ny));
+
     for (int k = k0; k < k1; ++k) {
-        if (valid && DIM == 3) cn = load_col_cell(a.in, cidx(l, m, bc_mp(k, K.nz), K.nx, K.ny));
+        const int b = (k - k0) & 1;
+        ColCell cnn, nhx, nhy;
+        if (DIM == 3 && k + 1 < k1) {
+            const int kn = k + 1;
+            if (valid) cnn = load_col_cell(a.in, cidx(l, m, bc_mp(kn, K.nz), K.nx, K.ny));
+            if (hxl >= 0) nhx = load_col_cell(a.in, cidx(hxl, m, kn, K.nx, K.ny));
+            if (hym >= 0) nhy = load_col_cell(a.in, cidx(l, hym, kn, K.nx, K.ny));
+        }
         if (valid) {
-            s_cnt[ly + 1][lx + 1] = (uint8_t)cc.cnt;
-            s_liq[ly + 1][lx + 1] = cc.liq;
-            s_sid[ly + 1][lx + 1] = (uint16_t)cc.sid;
-            s_sv[ly + 1][lx + 1] = cc.sv;
+            s_cnt[b][ly + 1][lx + 1] = (uint8_t)cc.cnt;
+            s_liq[b][ly + 1][lx + 1] = cc.liq;
+            s_sid[b][ly + 1][lx + 1] = (uint16_t)cc.sid;
+            s_sv[b][ly + 1][lx + 1] = cc.sv;
         }
         if (hxl >= 0) {
-            ColCell t = load_col_cell(a.in, cidx(hxl, m, k, K.nx, K.ny));
-            s_cnt[ly + 1][hxs] = (uint8_t)t.cnt;
-            s_liq[ly + 1][hxs] = t.liq;
-            s_sid[ly + 1][hxs] = (uint16_t)t.sid;
-            s_sv[ly + 1][hxs] = t.sv;
+            s_cnt[b][ly + 1][hxs] = (uint8_t)hx.cnt;
+            s_liq[b][ly + 1][hxs] = hx.liq;
+            s_sid[b][ly + 1][hxs] = (uint16_t)hx.sid;
+            s_sv[b][ly + 1][hxs] = hx.sv;
         }
         if (hym >= 0) {
-            ColCell t = load_col_cell(a.in, cidx(l, hym, k, K.nx, K.ny));
-            s_cnt[hys][lx + 1] = (uint8_t)t.cnt;
-            s_liq[hys][lx + 1] = t.liq;
-            s_sid[hys][lx + 1] = (uint16_t)t.sid;
-            s_sv[hys][lx + 1] = t.sv;
+            s_cnt[b][hys][lx + 1] = (uint8_t)hy.cnt;
+            s_liq[b][hys][lx + 1] = hy.liq;
+            s_sid[b][hys][lx + 1] = (uint16_t)hy.sid;
+            s_sv[b][hys][lx + 1] = hy.sv;
         }
         __syncthreads();
         bool queue = false;
-        size_t c = 0;
         if (valid) {
-            c = cidx(l, m, k, K.nx, K.ny);
+            const size_t c = cidx(l, m, k, K.nx, K.ny);
             size_t ti = a.temp_dense ? c : (size_t)k;
             if (a.temp_dense || (l == 0 && m == 0)) {
                 double Tin = a.T_in[ti];
@@ -213,14 +226,14 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
             Real psv[NP];
             const int yy = ly + 1, xx = lx + 1;
             pcnt[P_C] = cc.cnt; pl[P_C] = cc.liq; psid[P_C] = cc.sid; psv[P_C] = cc.sv;
-            pcnt[P_XP] = s_cnt[yy][xx + 1]; pl[P_XP] = s_liq[yy][xx + 1];
-            psid[P_XP] = s_sid[yy][xx + 1]; psv[P_XP] = s_sv[yy][xx + 1];
-            pcnt[P_XM] = s_cnt[yy][xx - 1]; pl[P_XM] = s_liq[yy][xx - 1];
-            psid[P_XM] = s_sid[yy][xx - 1]; psv[P_XM] = s_sv[yy][xx - 1];
-            pcnt[P_YP] = s_cnt[yy + 1][xx]; pl[P_YP] = s_liq[yy + 1][xx];
-            psid[P_YP] = s_sid[yy + 1][xx]; psv[P_YP] = s_sv[yy + 1][xx];
-            pcnt[P_YM] = s_cnt[yy - 1][xx]; pl[P_YM] = s_liq[yy - 1][xx];
-            psid[P_YM] = s_sid[yy - 1][xx]; psv[P_YM] = s_sv[yy - 1][xx];
+            pcnt[P_XP] = s_cnt[b][yy][xx + 1]; pl[P_XP] = s_liq[b][yy][xx + 1];
+            psid[P_XP] = s_sid[b][yy][xx + 1]; psv[P_XP] = s_sv[b][yy][xx + 1];
+            pcnt[P_XM] = s_cnt[b][yy][xx - 1]; pl[P_XM] = s_liq[b][yy][xx - 1];
+            psid[P_XM] = s_sid[b][yy][xx - 1]; psv[P_XM] = s_sv[b][yy][xx - 1];
+            pcnt[P_YP] = s_cnt[b][yy + 1][xx]; pl[P_YP] = s_liq[b][yy + 1][xx];
+            psid[P_YP] = s_sid[b][yy + 1][xx]; psv[P_YP] = s_sv[b][yy + 1][xx];
+            pcnt[P_YM] = s_cnt[b][yy - 1][xx]; pl[P_YM] = s_liq[b][yy - 1][xx];
+            psid[P_YM] = s_sid[b][yy - 1][xx]; psv[P_YM] = s_sv[b][yy - 1][xx];
             if constexpr (DIM == 3) {
                 pcnt[P_ZP] = cn.cnt; pl[P_ZP] = cn.liq; psid[P_ZP] = cn.sid; psv[P_ZP] = cn.sv;
                 pcnt[P_ZM] = cp.cnt; pl[P_ZM] = cp.liq; psid[P_ZM] = cp.sid; psv[P_ZM] = cp.sv;
@@ -266,12 +279,17 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
             int base = 0;
             if (lane == leader) base = atomicAdd(&s_qn, __popcll(ball));
             base = __shfl(base, leader, 64);
-            if (queue) s_q[base + __popcll(ball & ((1ull << lane) - 1ull))] = (int)c;
+            if (queue)
+                s_q[base + __popcll(ball & ((1ull << lane) - 1ull))] =
+                    (uint16_t)(((k - k0) * LTY + ly) * LTX + lx);
         }
-        __syncthreads();
         cp = cc;
         cc = cn;
+        cn = cnn;
+        hx = nhx;
+        hy = nhy;
     }
+    __syncthreads();
     if (threadIdx.x == 0) {
         if (s_any && *((volatile int *)&a.stats->max_active) < 1) atomicMax(&a.stats->max_active, 1);
         s_qbase = (s_qn > 0) ? atomicAdd(wl_counter(a, WL_MERGE, a.step), s_qn) : 0;
@@ -279,8 +297,11 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
     __syncthreads();
     const int nq = s_qn;
     int *dst = a.wl[WL_MERGE] + s_qbase;
-    for (int i = threadIdx.x; i < nq; i += LTX * LTY) dst[i] = s_q[i];
-    (void)ncell;
+    for (int i = threadIdx.x; i < nq; i += LTX * LTY) {
+        int q = s_q[i];
+        int qx = q % LTX, qy = (q / LTX) % LTY, qz = q / (LTX * LTY);
+        dst[i] = (int)cidx(l0 + qx, m0 + qy, k0 + qz, K.nx, K.ny);
+    }
 }
 
 // Pass 2 over the queued cells: full APT merge.  Cells that need the
@@ -294,6 +315,7 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
     const int n = *wl_counter(a, WL_MERGE, a.step);
     const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
     const bool need_terms = K.aniso || DIM == 2;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->merge_cells, (unsigned long long)n);
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         int i = base + threadIdx.x;
         bool valid = i < n;
@@ -329,6 +351,7 @@ __global__ __launch_bounds__(128) void k_front(StepArgs<Real> a)
     const Consts<Real> &K = a.K;
     const int n = *wl_counter(a, WL_FRONT, a.step);
     const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->front_cells, (unsigned long long)n);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         size_t c = (size_t)a.wl[WL_FRONT][i];
         int l = (int)(c % K.nx);

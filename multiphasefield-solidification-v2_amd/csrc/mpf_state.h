@@ -42,6 +42,8 @@ struct Stats {
     unsigned long long slow_cells;     // cells sent to the general path
     int halt_step;             // first step with a KMAX overflow (halt mode), else NO_HALT
     int pad;
+    unsigned long long merge_cells;    // cells queued for the merge pass (summed over steps)
+    unsigned long long front_cells;    // solid-liquid interface cells (summed over steps)
 };
 
 template <typename Real>

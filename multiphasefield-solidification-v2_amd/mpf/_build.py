@@ -51,10 +51,15 @@ def build(verbose: bool = False) -> str:
     cc = hipcc()
     kern = os.path.join(CSRC, "mpf_kernels.hip")
     objs = []
-    for real, sfx, contract in (("double", "_f64", "off"), ("float", "_f32", "fast")):
+    # FP64: IEEE operations in source order (bit-exact with the oracle).
+    # FP32: contraction and the hardware's approximate (1-2 ulp) square root
+    # and division instead of the correctly rounded sequences.
+    variants = (("double", "_f64", ["-ffp-contract=off"]),
+                ("float", "_f32", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]))
+    for real, sfx, flags in variants:
         o = os.path.join(BUILD, f"mpf_kernels{sfx}.o")
-        _run([cc, *_common(), f"-DMPF_REAL={real}", f"-DMPF_SFX={sfx}",
-              f"-ffp-contract={contract}", "-c", kern, "-o", o], verbose)
+        _run([cc, *_common(), f"-DMPF_REAL={real}", f"-DMPF_SFX={sfx}", *flags, "-c", kern, "-o", o],
+             verbose)
         objs.append(o)
     api = os.path.join(BUILD, "mpf_api.o")
     _run([cc, *_common(), "-x", "hip", "-c", os.path.join(CSRC, "mpf_api.cpp"), "-o", api], verbose)

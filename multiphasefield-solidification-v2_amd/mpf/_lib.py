@@ -24,7 +24,7 @@ EXPORTS = (
     "mpf_upload_temp", "mpf_download_temp", "mpf_step", "mpf_download_phi", "mpf_download_apt",
     "mpf_get_stats", "mpf_reset_stats", "mpf_observables", "mpf_device_bytes",
     "mpf_dense_apt", "mpf_dense_update", "mpf_dense_temp",
-    "mpf_malloc", "mpf_free", "mpf_memcpy_htod", "mpf_memcpy_dtoh", "mpf_memset",
+    "mpf_malloc", "mpf_free", "mpf_memcpy_htod", "mpf_memcpy_dtoh", "mpf_memcpy_dtod", "mpf_memset",
     "mpf_device_synchronize", "mpf_last_error", "mpf_abi_version",
 )
 
@@ -59,6 +59,7 @@ class Stats(C.Structure):
         ("max_active", C.c_int), ("kmax_overflow", C.c_int),
         ("overflow_cells", C.c_ulonglong), ("general_cells", C.c_ulonglong),
         ("halt_step", C.c_int), ("steps_done", C.c_int), ("last_step_ms", C.c_double),
+        ("merge_cells", C.c_ulonglong), ("front_cells", C.c_ulonglong),
     ]
 
 
@@ -102,6 +103,7 @@ def lib():
         "mpf_free": (C.c_int, [vp]),
         "mpf_memcpy_htod": (C.c_int, [vp, vp, C.c_size_t]),
         "mpf_memcpy_dtoh": (C.c_int, [vp, vp, C.c_size_t]),
+        "mpf_memcpy_dtod": (C.c_int, [vp, vp, C.c_size_t]),
         "mpf_memset": (C.c_int, [vp, C.c_int, C.c_size_t]),
         "mpf_device_synchronize": (C.c_int, []),
         "mpf_last_error": (C.c_char_p, []),

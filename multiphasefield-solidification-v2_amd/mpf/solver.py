@@ -155,7 +155,8 @@ class Solver:
         return {"max_active": s.max_active, "kmax_overflow": s.kmax_overflow,
                 "overflow_cells": s.overflow_cells, "general_cells": s.general_cells,
                 "halt_step": s.halt_step, "steps_done": s.steps_done,
-                "last_step_ms": s.last_step_ms}
+                "last_step_ms": s.last_step_ms, "merge_cells": s.merge_cells,
+                "front_cells": s.front_cells}
 
     def reset_stats(self):
         check(_lib.lib().mpf_reset_stats(self._h))
