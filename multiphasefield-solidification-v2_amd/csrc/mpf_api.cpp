@@ -68,6 +68,7 @@ struct Buffers {
     uint8_t *cnt = nullptr;
     uint16_t *sid = nullptr;
     void *sval = nullptr;
+    uint16_t *occ = nullptr; // occupancy mask (occ_words entries)
 };
 
 struct mpf_handle {
@@ -75,6 +76,7 @@ struct mpf_handle {
     int dim = 3;
     int nz = 1;
     size_t ncell = 0;
+    size_t nocc = 0; // occupancy mask words per buffer
     int scap = 1;
     size_t rsz = 8; // sizeof(Real)
     hipStream_t stream = nullptr;
@@ -126,6 +128,7 @@ extern "C" void mpf_destroy(mpf_handle *h)
         hipFree(h->b[i].cnt);
         hipFree(h->b[i].sid);
         hipFree(h->b[i].sval);
+        hipFree(h->b[i].occ);
         hipFree(h->T[i]);
     }
     hipFree(h->tab);
@@ -171,6 +174,7 @@ extern "C" int mpf_create(const mpf_config *cfg, mpf_handle **out)
     h->nz = c.nz;
     h->ncell = ncell;
     h->rsz = c.precision == MPF_FP64 ? 8 : 4;
+    h->nocc = occ_words(c.nx, c.ny, c.nz, c.dim);
     int cap = c.kmax < c.n_phase - 1 ? c.kmax : c.n_phase - 1;
     if (c.slot_capacity > 0) {
         // a step may store up to min(KMAX, N-1) grains per cell
@@ -203,6 +207,7 @@ extern "C" int mpf_create(const mpf_config *cfg, mpf_handle **out)
         TRY_RC(dev_alloc(h, (void **)&h->b[i].cnt, ncell));
         TRY_RC(dev_alloc(h, (void **)&h->b[i].sid, ncell * h->scap * sizeof(uint16_t)));
         TRY_RC(dev_alloc(h, &h->b[i].sval, ncell * h->scap * h->rsz));
+        TRY_RC(dev_alloc(h, (void **)&h->b[i].occ, h->nocc * sizeof(uint16_t)));
     }
     h->nT = c.temp_dense ? ncell : (size_t)c.nz;
     for (int i = 0; i < 2; ++i) TRY_RC(dev_alloc(h, (void **)&h->T[i], h->nT * sizeof(double)));
@@ -215,6 +220,7 @@ extern "C" int mpf_create(const mpf_config *cfg, mpf_handle **out)
     for (int i = 0; i < 2; ++i) {
         e = hipMemsetAsync(h->b[i].cnt, 0, ncell, h->stream);
         if (e == hipSuccess) e = hipMemsetAsync(h->b[i].liq, 0, ncell * h->rsz, h->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(h->b[i].occ, 0xff, h->nocc * sizeof(uint16_t), h->stream);
         if (e == hipSuccess) e = hipMemsetAsync(h->T[i], 0, h->nT * sizeof(double), h->stream);
         if (e != hipSuccess) {
             mpf_destroy(h);
@@ -267,6 +273,8 @@ extern "C" int mpf_upload_phi(mpf_handle *h, const void *phi, int dtype)
     hipError_t e = hipMalloc((void **)&err, sizeof(int));
     if (e == hipSuccess) e = hipMemcpy(d, phi, bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(err, 0, sizeof(int));
+    // the new state may hold grains anywhere
+    if (e == hipSuccess) e = hipMemsetAsync(h->b[h->cur].occ, 0xff, h->nocc * sizeof(uint16_t), h->stream);
     if (e == hipSuccess) {
         const mpf_config &c = h->cfg;
         if (h->rsz == 8)
@@ -306,6 +314,7 @@ extern "C" int mpf_init_columns(mpf_handle *h, const int32_t *grain_map, const i
     if (e == hipSuccess) e = hipMemcpy(dp, profile, ncol * sizeof(int32_t), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ds, solid, (size_t)nprof * c.nz * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dl, liquid, (size_t)nprof * c.nz * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(h->b[h->cur].occ, 0xff, h->nocc * sizeof(uint16_t), h->stream);
     if (e == hipSuccess) {
         if (h->rsz == 8)
             e = mpf_launch_init_columns_f64(dg, dp, ds, dl, c.nx, c.ny, c.nz, view<double>(h, h->cur), h->stream);
@@ -366,6 +375,8 @@ static hipError_t launch_one_step(mpf_handle *h, const int *grids)
     if (c.dim == 2) a.K.aniso = 1;
     a.in = view<R>(h, h->cur);
     a.out = view<R>(h, h->cur ^ 1);
+    a.occ_in = h->b[h->cur].occ;
+    a.occ_out = h->b[h->cur ^ 1].occ;
     a.T_in = h->T[h->tcur];
     a.T_out = h->T[h->tcur ^ 1];
     a.temp_dense = c.temp_dense;

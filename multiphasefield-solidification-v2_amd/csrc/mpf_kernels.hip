@@ -98,8 +98,13 @@ __device__ __forceinline__ int *wl_counter(const StepArgs<Real> &a, int list, in
 // x/y neighbours of plane z come from an LDS tile with a one-cell halo.  The
 // queued cells of the whole block are collected in LDS and appended to the
 // work list with a single atomic.
-constexpr int LTX = 64, LTY = 8, LZC = 16;
-
+//
+// The loads of plane z+1 are issued before plane z is processed.  Every lane
+// issues the same loads on every plane (lanes without a cell, or planes whose
+// occupancy bit is clear, read a line that stays cached and discard it), so
+// the number of memory operations in flight is static and the compiler waits
+// only for the plane it needs.  Slot 0 is read together with the count and
+// dropped when the count is zero.
 struct ColCell {
     int cnt;
     Real liq;
@@ -107,21 +112,26 @@ struct ColCell {
     Real sv;        // slot-0 value
 };
 
-__device__ __forceinline__ ColCell load_col_cell(const SlotView<Real> &S, size_t c)
+// c: cell for count and liquid; cs: cell whose slot 0 is read (c itself, or
+// the cached line when the cell cannot hold a grain)
+__device__ __forceinline__ ColCell load_cell(const SlotView<Real> &S, size_t c, size_t cs)
 {
     ColCell r;
     r.cnt = S.cnt[c];
     r.liq = S.liq[c];
-    r.sid = NO_ID;
-    r.sv = Real(0.0);
-    if (r.cnt >= 1) {
-        r.sid = S.sid[c];
-        r.sv = S.sval[c];
-    }
+    r.sid = S.sid[cs];
+    r.sv = S.sval[cs];
     return r;
 }
 
-template <int DIM>
+__device__ __forceinline__ void finish_cell(ColCell &r)
+{
+    const bool e = r.cnt == 0;
+    r.sid = e ? NO_ID : r.sid;
+    r.sv = e ? Real(0.0) : r.sv;
+}
+
+template <int DIM, int TD>
 __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
@@ -136,12 +146,14 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
     __shared__ Real s_sv[2][LTY + 2][LTX + 2];
     __shared__ uint16_t s_q[LTX * LTY * ZC]; // queued cells as (z-plane, row, column) in the tile
     __shared__ int s_qn, s_qbase, s_any;
+    __shared__ uint32_t s_occ;
 
-    int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const int tx = lb % a.ntx;
     const int rest = lb / a.ntx;
     const int ty = rest % a.nty;
     const int tz = rest / a.nty;
+    const int nzc = (DIM == 3) ? (K.nz + LZC - 1) / LZC : 1;
     const int lx = threadIdx.x % LTX, ly = threadIdx.x / LTX;
     const int l0 = tx * LTX, m0 = ty * LTY;
     const int w = min(LTX, K.nx - l0), h = min(LTY, K.ny - m0);
@@ -149,10 +161,20 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
     const bool valid = (lx < w) && (ly < h);
     const int k0 = tz * ZC;
     const int k1 = (DIM == 3) ? min(k0 + ZC, K.nz) : 1;
+    const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
+    // the discarded loads go to the first 64 cells (one line per array)
+    const size_t cdum = min((size_t)(threadIdx.x & 63), ncell - 1);
     if (threadIdx.x == 0) {
         s_qn = 0;
         s_any = 0;
+        s_occ = 0;
     }
+    auto tile_of = [&](int ttx, int tty, int ttz) { return ((size_t)ttz * a.nty + tty) * a.ntx + ttx; };
+    // occupancy of the own column over planes k0-1 .. k0+LZC (bit kk-k0+1)
+    uint32_t colmask = (uint32_t)a.occ_in[tile_of(tx, ty, tz)] << 1;
+    if (DIM == 3 && tz > 0) colmask |= ((uint32_t)a.occ_in[tile_of(tx, ty, tz - 1)] >> (LZC - 1)) & 1u;
+    if (DIM == 3 && tz + 1 < nzc) colmask |= ((uint32_t)a.occ_in[tile_of(tx, ty, tz + 1)] & 1u) << (LZC + 1);
+
     // halo cells this thread loads each plane: x halo of its row, y halo of
     // its column (x periodic; y periodic in 3D, mirror in 2D)
     int hxl = -1, hxs = 0;
@@ -168,30 +190,65 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
         else
             hym = (ly == 0) ? bc_mm(m0, K.ny) : bc_mp(m0 + h - 1, K.ny);
     }
+    const uint32_t occx = hxl >= 0 ? a.occ_in[tile_of(hxl / LTX, ty, tz)] : 0u;
+    const uint32_t occy = hym >= 0 ? a.occ_in[tile_of(tx, hym / LTY, tz)] : 0u;
 
-    // Software pipeline: the own column (z-1, z, z+1) and the halo of the
-    // current plane are in registers; the loads of the next plane are issued
-    // before the current plane is processed.
-    ColCell cp, cc, cn, hx, hy;
-    if (valid) {
-        cc = load_col_cell(a.in, cidx(l, m, k0, K.nx, K.ny));
-        if (DIM == 3) {
-            cp = load_col_cell(a.in, cidx(l, m, bc_mm(k0, K.nz), K.nx, K.ny));
-            cn = load_col_cell(a.in, cidx(l, m, bc_mp(k0, K.nz), K.nx, K.ny));
+    auto own_cell = [&](int kk) {
+        const bool occ = valid && ((colmask >> (kk - k0 + 1)) & 1u);
+        const size_t c = valid ? cidx(l, m, kk, K.nx, K.ny) : cdum;
+        return load_cell(a.in, c, occ ? c : cdum);
+    };
+    auto halo_x = [&](int kk) {
+        const bool occ = hxl >= 0 && ((occx >> (kk - k0)) & 1u);
+        const size_t c = hxl >= 0 ? cidx(hxl, m, kk, K.nx, K.ny) : cdum;
+        return load_cell(a.in, c, occ ? c : cdum);
+    };
+    auto halo_y = [&](int kk) {
+        const bool occ = hym >= 0 && ((occy >> (kk - k0)) & 1u);
+        const size_t c = hym >= 0 ? cidx(l, hym, kk, K.nx, K.ny) : cdum;
+        return load_cell(a.in, c, occ ? c : cdum);
+    };
+    auto temp_at = [&](int kk) -> double {
+        return a.T_in[valid ? cidx(l, m, kk, K.nx, K.ny) : cdum];
+    };
+    // per-plane temperature: one lane of the tiles at x = y = 0 updates the
+    // planes of its z-range
+    if (!TD && tx == 0 && ty == 0 && threadIdx.x == 0) {
+        for (int k = k0; k < k1; ++k) {
+            const double Tin = a.T_in[k];
+            a.T_out[k] = a.temp_f32 ? (double)((float)Tin - (float)a.cooling) : Tin - a.cooling;
         }
     }
-    if (hxl >= 0) hx = load_col_cell(a.in, cidx(hxl, m, k0, K.nx, K.ny));
-    if (hym >= 0) hy = load_col_cell(a.in, cidx(l, hym, k0, K.nx, K.ny));
+
+    ColCell cp{}, cc, cn{}, hx, hy;
+    double Tc = 0.0;
+    cc = own_cell(k0);
+    if (DIM == 3) {
+        cp = own_cell(bc_mm(k0, K.nz));
+        cn = own_cell(bc_mp(k0, K.nz));
+    }
+    hx = halo_x(k0);
+    hy = halo_y(k0);
+    if (TD) Tc = temp_at(k0);
+    finish_cell(cp);
+    finish_cell(cc);
+    uint32_t occ_bits = 0;
 
     for (int k = k0; k < k1; ++k) {
         const int b = (k - k0) & 1;
-        ColCell cnn, nhx, nhy;
-        if (DIM == 3 && k + 1 < k1) {
-            const int kn = k + 1;
-            if (valid) cnn = load_col_cell(a.in, cidx(l, m, bc_mp(kn, K.nz), K.nx, K.ny));
-            if (hxl >= 0) nhx = load_col_cell(a.in, cidx(hxl, m, kn, K.nx, K.ny));
-            if (hym >= 0) nhy = load_col_cell(a.in, cidx(l, hym, kn, K.nx, K.ny));
+        // loads of the next plane (the last plane reloads itself; unused)
+        ColCell cnn{}, nhx{}, nhy{};
+        double Tn = 0.0;
+        if (DIM == 3) {
+            const int kn = min(k + 1, k1 - 1);
+            cnn = own_cell(bc_mp(kn, K.nz));
+            nhx = halo_x(kn);
+            nhy = halo_y(kn);
+            if (TD) Tn = temp_at(kn);
         }
+        finish_cell(cn);
+        finish_cell(hx);
+        finish_cell(hy);
         if (valid) {
             s_cnt[b][ly + 1][lx + 1] = (uint8_t)cc.cnt;
             s_liq[b][ly + 1][lx + 1] = cc.liq;
@@ -211,14 +268,10 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
             s_sv[b][hys][lx + 1] = hy.sv;
         }
         __syncthreads();
-        bool queue = false;
+        bool queue = false, grains = false;
         if (valid) {
             const size_t c = cidx(l, m, k, K.nx, K.ny);
-            size_t ti = a.temp_dense ? c : (size_t)k;
-            if (a.temp_dense || (l == 0 && m == 0)) {
-                double Tin = a.T_in[ti];
-                a.T_out[ti] = a.temp_f32 ? (double)((float)Tin - (float)a.cooling) : Tin - a.cooling;
-            }
+            if (TD) a.T_out[c] = a.temp_f32 ? (double)((float)Tc - (float)a.cooling) : Tc - a.cooling;
             // stencil: c, x+, x-, y+, y-, z+, z-
             int pcnt[NP];
             Real pl[NP];
@@ -245,6 +298,7 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
                 sum += pcnt[p];
                 single = single && (pcnt[p] == 1) && (psid[p] == psid[0]) && (pl[p] == Real(0.0));
             }
+            grains = sum > 0;
             if (sum == 0) {
                 // only liquid in the stencil: active set {L} or empty
                 bool act = apt_active<NP>(pl);
@@ -271,6 +325,8 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
                 queue = true;
             }
         }
+        // an output cell can only hold a grain where its stencil held one
+        if (__ballot(grains) != 0ull) occ_bits |= 1u << (k - k0);
         // append queued cells to the block's LDS list
         unsigned long long ball = __ballot(queue);
         if (ball != 0ull) {
@@ -288,11 +344,14 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
         cn = cnn;
         hx = nhx;
         hy = nhy;
+        Tc = Tn;
     }
+    if ((threadIdx.x & 63) == 0 && occ_bits) atomicOr(&s_occ, occ_bits);
     __syncthreads();
     if (threadIdx.x == 0) {
         if (s_any && *((volatile int *)&a.stats->max_active) < 1) atomicMax(&a.stats->max_active, 1);
         s_qbase = (s_qn > 0) ? atomicAdd(wl_counter(a, WL_MERGE, a.step), s_qn) : 0;
+        a.occ_out[tile_of(tx, ty, tz)] = (uint16_t)s_occ;
     }
     __syncthreads();
     const int nq = s_qn;
@@ -316,7 +375,16 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
     const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
     const bool need_terms = K.aniso || DIM == 2;
     if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->merge_cells, (unsigned long long)n);
-    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    // Same-address device atomics serialise (about 11 ns each on MI355X), so
+    // the queued lanes of the whole block are appended with one atomic per
+    // list and chunk.  s_off is double-buffered on the chunk parity: a wave
+    // can only overwrite a buffer after every wave passed the barrier of the
+    // chunk in between, by which time that buffer has been read.
+    __shared__ int s_off[2][2][4];
+    __shared__ int s_base[2][2];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int par = 0;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x, par ^= 1) {
         int i = base + threadIdx.x;
         bool valid = i < n;
         size_t c = valid ? (size_t)a.wl[WL_MERGE][i] : 0;
@@ -337,8 +405,28 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
                 write_cell<Real, DIM, FAST_A>(a.out, ncell, c, C);
             }
         }
-        push_list(valid && front, (int)c, a.wl[WL_FRONT], wl_counter(a, WL_FRONT, a.step));
-        push_list(valid && !fits, (int)c, a.wl[WL_GENERAL], wl_counter(a, WL_GENERAL, a.step));
+        const bool qf = valid && front, qg = valid && !fits;
+        const unsigned long long bf = __ballot(qf), bg = __ballot(qg);
+        if (lane == 0) {
+            s_off[par][0][wave] = __popcll(bf);
+            s_off[par][1][wave] = __popcll(bg);
+        }
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            int *o = s_off[par][threadIdx.x];
+            int tot = 0;
+            for (int w = 0; w < 4; ++w) {
+                int v = o[w];
+                o[w] = tot;
+                tot += v;
+            }
+            const int list = threadIdx.x == 0 ? WL_FRONT : WL_GENERAL;
+            s_base[par][threadIdx.x] = tot ? atomicAdd(wl_counter(a, list, a.step), tot) : 0;
+        }
+        __syncthreads();
+        const unsigned long long below = (1ull << lane) - 1ull;
+        if (qf) a.wl[WL_FRONT][s_base[par][0] + s_off[par][0][wave] + __popcll(bf & below)] = (int)c;
+        if (qg) a.wl[WL_GENERAL][s_base[par][1] + s_off[par][1][wave] + __popcll(bg & below)] = (int)c;
     }
 }
 
@@ -787,12 +875,18 @@ hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const 
     const int nzc = dim == 3 ? (a.K.nz + LZC - 1) / LZC : 1;
     unsigned grid = (unsigned)a.ntx * a.nty * nzc;
     if (dim == 3) {
-        hipLaunchKernelGGL(k_light<3>, dim3(grid), dim3(LTX * LTY), 0, st, a);
+        if (a.temp_dense)
+            hipLaunchKernelGGL((k_light<3, 1>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_light<3, 0>), dim3(grid), dim3(LTX * LTY), 0, st, a);
         hipLaunchKernelGGL(k_merge<3>, dim3(grids[0]), dim3(256), 0, st, a);
         hipLaunchKernelGGL(k_front<3>, dim3(grids[1]), dim3(128), 0, st, a);
         hipLaunchKernelGGL(k_step_general<3>, dim3(grids[2]), dim3(64), 0, st, a);
     } else {
-        hipLaunchKernelGGL(k_light<2>, dim3(grid), dim3(LTX * LTY), 0, st, a);
+        if (a.temp_dense)
+            hipLaunchKernelGGL((k_light<2, 1>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_light<2, 0>), dim3(grid), dim3(LTX * LTY), 0, st, a);
         hipLaunchKernelGGL(k_merge<2>, dim3(grids[0]), dim3(256), 0, st, a);
         hipLaunchKernelGGL(k_front<2>, dim3(grids[1]), dim3(128), 0, st, a);
         hipLaunchKernelGGL(k_step_general<2>, dim3(grids[2]), dim3(64), 0, st, a);

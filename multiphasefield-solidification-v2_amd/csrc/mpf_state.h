@@ -27,6 +27,20 @@ constexpr uint32_t NO_ID = 0xFFFFu;
 constexpr int NO_HALT = 0x7fffffff; // halt_step value while no step has overflowed
 enum { WL_MERGE = 0, WL_FRONT = 1, WL_GENERAL = 2 };
 
+// Tiles of the classification pass: LTX x LTY cells in x/y, LZC z-planes.
+constexpr int LTX = 64, LTY = 8, LZC = 16;
+
+// Occupancy mask: one uint16 per tile (index (tz*nty + ty)*ntx + tx); bit p
+// is set when plane tz*LZC + p of the tile may hold a stored grain.  A clear
+// bit guarantees cnt == 0 on every cell of that tile plane.  Each state
+// buffer has its own mask; states written by the host get all bits set.
+inline size_t occ_words(int nx, int ny, int nz, int dim)
+{
+    size_t ntx = (size_t)(nx + LTX - 1) / LTX, nty = (size_t)(ny + LTY - 1) / LTY;
+    size_t nzc = dim == 3 ? (size_t)(nz + LZC - 1) / LZC : 1;
+    return ntx * nty * nzc;
+}
+
 template <typename Real>
 struct SlotView {
     Real *liq;
@@ -50,6 +64,8 @@ template <typename Real>
 struct StepArgs {
     Consts<Real> K;
     SlotView<Real> in, out;
+    const uint16_t *occ_in; // occupancy mask of `in`
+    uint16_t *occ_out;      // occupancy mask of `out`, written by the classification pass
     const double *T_in;
     double *T_out;
     int temp_dense;  // 1: temperature per cell; 0: one value per z-plane
