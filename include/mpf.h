@@ -80,6 +80,10 @@ typedef struct mpf_stats {
     double last_step_ms;            /* GPU time of the last mpf_step call (hipEvents) */
     unsigned long long merge_cells; /* cells that needed the full APT merge (summed over steps) */
     unsigned long long front_cells; /* solid-liquid interface cells (summed over steps) */
+    int class_tables;               /* 1: the tables have the class form of
+                                       build_interaction_matrices (src/seed_modes.py:333-380)
+                                       and the step selects their values instead of loading them */
+    int pad_;
 } mpf_stats;
 
 /* ---- resident solver ---------------------------------------------------- */

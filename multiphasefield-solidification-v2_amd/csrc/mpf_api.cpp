@@ -91,6 +91,8 @@ struct mpf_handle {
     int step_counter = 0;
     int steps_done = 0;
     int tables_set = 0;
+    int tab_class = 0;    // the tables have the class form (class_tables)
+    double cls[3][2] = {}; // its values, exact in the solver precision
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     size_t bytes = 0;
@@ -252,10 +254,16 @@ extern "C" int mpf_set_tables(mpf_handle *h, const double *wij, const double *ai
     memcpy(all.data() + 3 * nn, n111, (size_t)N * 24 * sizeof(double));
     if (h->rsz == 8) {
         HIP_TRY(hipMemcpy(h->tab, all.data(), tot * sizeof(double), hipMemcpyHostToDevice));
+        double cls[3][2];
+        h->tab_class = class_tables<double>(all.data(), N, cls);
+        for (int t = 0; t < 3; ++t) h->cls[t][0] = cls[t][0], h->cls[t][1] = cls[t][1];
     } else {
         std::vector<float> f(tot);
         for (size_t i = 0; i < tot; ++i) f[i] = (float)all[i];
         HIP_TRY(hipMemcpy(h->tab, f.data(), tot * sizeof(float), hipMemcpyHostToDevice));
+        float cls[3][2];
+        h->tab_class = class_tables<float>(f.data(), N, cls);
+        for (int t = 0; t < 3; ++t) h->cls[t][0] = cls[t][0], h->cls[t][1] = cls[t][1];
     }
     h->tables_set = 1;
     return 0;
@@ -386,6 +394,11 @@ static hipError_t launch_one_step(mpf_handle *h, const int *grids)
     a.wl_count = h->wl_count;
     a.step = h->step_counter;
     a.halt_mode = c.halt_on_kmax;
+    a.tab_class = h->tab_class;
+    for (int t = 0; t < 3; ++t) {
+        a.K.cls[t][0] = (R)h->cls[t][0];
+        a.K.cls[t][1] = (R)h->cls[t][1];
+    }
     a.stats = h->stats;
     if (sizeof(R) == 8)
         return mpf_launch_step_f64((const StepArgs<double> *)&a, c.dim, grids, h->stream);
@@ -508,6 +521,8 @@ extern "C" int mpf_get_stats(mpf_handle *h, mpf_stats *st)
     st->last_step_ms = h->last_ms;
     st->merge_cells = s.merge_cells;
     st->front_cells = s.front_cells;
+    st->class_tables = h->tab_class;
+    st->pad_ = 0;
     return 0;
 }
 

@@ -49,8 +49,35 @@ struct Consts {
     Real tan_a0, delta_a, mu_a, mu_delta, p2; // mu_a*delta_a, p_round*p_round
     Real g2_floor, ksi, theta_c;
     double Sf, T_melt;
-    const Real *wij, *aij, *mij, *n111; // [N][N] x3, [N][8][3]
+    // wij, aij, mij [N][N] and n111 [N][8][3], contiguous in that order.  One
+    // pointer instead of four keeps the kernels' scalar registers free.
+    const Real *tab;
+    uint32_t nn; // N*N (N <= 65535)
+    MPF_DEV const Real *n111() const { return tab + 3 * (size_t)nn; }
+    // Values of the class-form tables (used by kernels built with TC = 1):
+    // cls[table][0] for solid-liquid pairs, cls[table][1] for grain-grain
+    // pairs; table 0 = wij, 1 = aij, 2 = mij.
+    Real cls[3][2];
 };
+
+// Interaction-table lookup.  TC = 1: the tables have the class form that
+// build_interaction_matrices gives them (src/seed_modes.py:333-380), with +0
+// on the diagonal, one value for every solid-liquid pair and one for every
+// grain-grain pair.  The host checks that form bit for bit before selecting
+// these kernels, so the value is selected instead of loaded and the result
+// is identical.
+enum { TAB_W = 0, TAB_A = 1, TAB_M = 2 };
+
+template <int TC, int T, typename Real>
+MPF_DEV Real tab_at(const Consts<Real> &K, int i, int j)
+{
+    if constexpr (TC != 0) {
+        if (i == j) return Real(0.0);
+        return (i == 0 || j == 0) ? K.cls[T][0] : K.cls[T][1];
+    } else {
+        return K.tab[(size_t)T * K.nn + (size_t)i * K.N + j];
+    }
+}
 
 // ---------------------------------------------------------------- math glue
 MPF_DEV double m_sqrt(double x) { return sqrt(x); }
@@ -308,7 +335,7 @@ MPF_DEV void solid_liquid_terms(const Consts<Real> &K, const LiqCtx<Real, DIM> &
     constexpr int NP = LiqCtx<Real, DIM>::NP;
     Real nrm[24];
 #pragma unroll
-    for (int u = 0; u < 24; ++u) nrm[u] = K.n111[(size_t)s * 24 + u];
+    for (int u = 0; u < 24; ++u) nrm[u] = K.n111()[(size_t)s * 24 + u];
     // One stencil point: facet search, eps^2 and A12 (written out per point so
     // that every access to the context is static).
     auto point = [&](const LGrad<Real> &G, Real &e2p, Facet<Real> &f, A12v<Real> &r) {
@@ -462,7 +489,7 @@ struct Cell {
 // Cells with many active phases (A large) keep the arrays in scratch memory.
 // ANISO_TERMS = false compiles the anisotropy terms out; the caller then
 // guarantees the cell has no active solid-liquid pair or anisotropy is off.
-template <typename Real, int DIM, int A, bool ANISO_TERMS = true, typename LiqAt>
+template <typename Real, int DIM, int A, bool ANISO_TERMS = true, int TC = 0, typename LiqAt>
 MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, int k,
                                             double Tcell, Cell<Real, DIM, A> &S)
 {
@@ -534,7 +561,7 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
                     Real cmax = Real(0.0);
                     if (!(g2 < K.g2_floor)) {
                         Real inv_g = Real(1.0) / m_sqrt(g2);
-                        Facet<Real> f = facet_search<Real, 3>(K.n111 + (size_t)gs * 24, sx * inv_g,
+                        Facet<Real> f = facet_search<Real, 3>(K.n111() + (size_t)gs * 24, sx * inv_g,
                                                               sy * inv_g, sz * inv_g);
                         cmax = f.ca;
                     }
@@ -568,7 +595,7 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
             Real cmax = Real(0.0);
             if (gn2 >= K.g2_floor) {
                 Real inv_gn = Real(1.0) / m_sqrt(gn2);
-                Facet<Real> f = facet_search<Real, 2>(K.n111 + (size_t)gs * 24, phix_ * inv_gn,
+                Facet<Real> f = facet_search<Real, 2>(K.n111() + (size_t)gs * 24, phix_ * inv_gn,
                                                       phiy_ * inv_gn, Real(0.0));
                 cmax = f.ca;
             }
@@ -638,7 +665,6 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
     const double dG = K.Sf * (Tcell - K.T_melt);
     const Real df_sl = Real(-dG), df_ls = Real(dG);
     const Real n_real = Real(n_act);
-    const int N = K.N;
 #pragma unroll UNR_NA
     for (int t1 = 0; t1 < TMAX; ++t1) {
         S.out[t1] = Real(0.0);
@@ -663,8 +689,8 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
                 if (!p3) continue;
                 const int pk = S.gid[t3];
                 const Real phi_k = S.v[t3][P_C];
-                Real wik = K.wij[(size_t)i * N + pk];
-                Real wjk = K.wij[(size_t)j * N + pk];
+                Real wik = tab_at<TC, TAB_W>(K, i, pk);
+                Real wjk = tab_at<TC, TAB_W>(K, j, pk);
                 if (pk == 0) {
                     if (i != 0) wik = w_sl;
                     if (j != 0) wjk = w_sl;
@@ -676,7 +702,7 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
                 } else if (i == 0 && j != 0) {
                     if (pk == j) term2 = Real(-0.5) * lap_sl[t2];
                 } else {
-                    Real aik = K.aij[(size_t)i * N + pk], ajk = K.aij[(size_t)j * N + pk];
+                    Real aik = tab_at<TC, TAB_A>(K, i, pk), ajk = tab_at<TC, TAB_A>(K, j, pk);
                     Real eps2ik = aik * aik;
                     Real eps2jk = ajk * ajk;
                     term2 = Real(0.5) * (eps2ik - eps2jk) * lap[t3];
@@ -685,11 +711,12 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
             }
             Real phii_phij = S.v[t1][P_C] * S.v[t2][P_C];
             Real term_force = Real(8.0 / 3.1415926535) * m_sqrt(pymax(phii_phij, Real(0.0))) * driving_force;
-            Real mij_eff = K.mij[(size_t)i * N + j];
+            const Real mij_tab = tab_at<TC, TAB_M>(K, i, j);
+            Real mij_eff = mij_tab;
             if (i != 0 && j == 0)
-                mij_eff = K.mij[(size_t)i * N + j] * b_sl[t1];
+                mij_eff = mij_tab * b_sl[t1];
             else if (i == 0 && j != 0)
-                mij_eff = K.mij[(size_t)i * N + j] * b_sl[t2];
+                mij_eff = mij_tab * b_sl[t2];
             dpi -= Real(2.0) * mij_eff / n_real * (ppp - term_force);
         }
         S.out[t1] = S.v[t1][P_C] + K.dt * dpi;

@@ -2,6 +2,8 @@
 // Shared by the host API and the CPU emulation tests.
 #pragma once
 
+#include <string.h>
+
 #include "mpf.h"
 #include "mpf_cell.h"
 #include "mpf_math.h"
@@ -45,13 +47,43 @@ inline Consts<R> make_consts(const mpf_scalars &s, int nx, int ny, int nz, int N
     K.theta_c = (R)s.theta_c_rad;
     K.Sf = s.Sf;
     K.T_melt = s.T_melt;
-    const R *t = (const R *)tab;
-    size_t nn = (size_t)N * N;
-    K.wij = t;
-    K.aij = t + nn;
-    K.mij = t + 2 * nn;
-    K.n111 = t + 3 * nn;
+    K.tab = (const R *)tab;
+    K.nn = (uint32_t)N * (uint32_t)N;
+    for (int t = 0; t < 3; ++t) K.cls[t][0] = K.cls[t][1] = R(0);
     return K;
+}
+
+// True when the tables tab = {wij, aij, mij} ([N][N] each, in the solver
+// precision) have the class form of build_interaction_matrices
+// (src/seed_modes.py:333-380): +0 on the diagonal, one bit pattern for all
+// solid-liquid pairs and one for all grain-grain pairs.  The class values go
+// to cls[table][0] (solid-liquid) and cls[table][1] (grain-grain).
+template <typename R>
+inline bool class_tables(const R *tab, int N, R cls[3][2])
+{
+    const size_t nn = (size_t)N * N;
+    const R zero = R(0);
+    for (int t = 0; t < 3; ++t) {
+        const R *a = tab + t * nn;
+        bool have[2] = {false, false};
+        cls[t][0] = cls[t][1] = zero;
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) {
+                const R v = a[(size_t)i * N + j];
+                if (i == j) {
+                    if (memcmp(&v, &zero, sizeof(R)) != 0) return false;
+                    continue;
+                }
+                const int c = (i == 0 || j == 0) ? 0 : 1;
+                if (!have[c]) {
+                    cls[t][c] = v;
+                    have[c] = true;
+                } else if (memcmp(&v, &cls[t][c], sizeof(R)) != 0) {
+                    return false;
+                }
+            }
+    }
+    return true;
 }
 
 

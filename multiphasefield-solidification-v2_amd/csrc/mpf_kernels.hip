@@ -366,7 +366,7 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
 // Pass 2 over the queued cells: full APT merge.  Cells that need the
 // solid-liquid anisotropy terms go to k_front, cells with more active grains
 // than the register capacity to k_step_general; the rest are finished here.
-template <int DIM>
+template <int DIM, int TC>
 __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
             if (fits && !front) {
                 size_t ti = a.temp_dense ? c : (size_t)k;
                 SlotLiq<Real> L{a.in.liq, K.nx, K.ny};
-                cell_update<Real, DIM, FAST_A, false>(K, L, l, m, k, a.T_out[ti], C);
+                cell_update<Real, DIM, FAST_A, false, TC>(K, L, l, m, k, a.T_out[ti], C);
                 write_cell<Real, DIM, FAST_A>(a.out, ncell, c, C);
             }
         }
@@ -432,7 +432,7 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
 
 // Pass 3: solid-liquid interface cells, compacted so that every lane of a
 // wavefront evaluates the anisotropy terms.
-template <int DIM>
+template <int DIM, int TC>
 __global__ __launch_bounds__(128) void k_front(StepArgs<Real> a)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
@@ -451,7 +451,7 @@ __global__ __launch_bounds__(128) void k_front(StepArgs<Real> a)
         gather<Real, DIM, FAST_A>(K, a.in, ncell, l, m, k, C, &ntot, &fits);
         size_t ti = a.temp_dense ? c : (size_t)k;
         SlotLiq<Real> L{a.in.liq, K.nx, K.ny};
-        cell_update<Real, DIM, FAST_A, true>(K, L, l, m, k, a.T_out[ti], C);
+        cell_update<Real, DIM, FAST_A, true, TC>(K, L, l, m, k, a.T_out[ti], C);
         write_cell<Real, DIM, FAST_A>(a.out, ncell, c, C);
     }
 }
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(128) void k_front(StepArgs<Real> a)
 // Pass 4: cells with more active grains than the merge pass holds.  The
 // arrays live in scratch memory with room for KMAX phases; the loops run over
 // the occupied entries only.
-template <int DIM>
+template <int DIM, int TC>
 __global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
@@ -479,7 +479,7 @@ __global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
         bool fits;
         gather<Real, DIM, KCAP>(K, a.in, ncell, l, m, k, C, &ntot, &fits);
         SlotLiq<Real> L{a.in.liq, K.nx, K.ny};
-        cell_update<Real, DIM, KCAP>(K, L, l, m, k, Tn, C);
+        cell_update<Real, DIM, KCAP, true, TC>(K, L, l, m, k, Tn, C);
         write_cell<Real, DIM, KCAP>(a.out, ncell, c, C);
     }
 }
@@ -865,6 +865,18 @@ using namespace mpf::MPF_CAT(impl, MPF_SFX);
 
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
+template <int DIM, int TC>
+static void launch_step_kernels(const StepArgs<Real> &a, unsigned grid, const int *grids, hipStream_t st)
+{
+    if (a.temp_dense)
+        hipLaunchKernelGGL((k_light<DIM, 1>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_light<DIM, 0>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+    hipLaunchKernelGGL((k_merge<DIM, TC>), dim3(grids[0]), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_front<DIM, TC>), dim3(grids[1]), dim3(128), 0, st, a);
+    hipLaunchKernelGGL((k_step_general<DIM, TC>), dim3(grids[2]), dim3(64), 0, st, a);
+}
+
 extern "C" {
 
 hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const int *grids, hipStream_t st)
@@ -873,23 +885,17 @@ hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const 
     a.ntx = (a.K.nx + LTX - 1) / LTX;
     a.nty = (a.K.ny + LTY - 1) / LTY;
     const int nzc = dim == 3 ? (a.K.nz + LZC - 1) / LZC : 1;
-    unsigned grid = (unsigned)a.ntx * a.nty * nzc;
+    const unsigned grid = (unsigned)a.ntx * a.nty * nzc;
     if (dim == 3) {
-        if (a.temp_dense)
-            hipLaunchKernelGGL((k_light<3, 1>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+        if (a.tab_class)
+            launch_step_kernels<3, 1>(a, grid, grids, st);
         else
-            hipLaunchKernelGGL((k_light<3, 0>), dim3(grid), dim3(LTX * LTY), 0, st, a);
-        hipLaunchKernelGGL(k_merge<3>, dim3(grids[0]), dim3(256), 0, st, a);
-        hipLaunchKernelGGL(k_front<3>, dim3(grids[1]), dim3(128), 0, st, a);
-        hipLaunchKernelGGL(k_step_general<3>, dim3(grids[2]), dim3(64), 0, st, a);
+            launch_step_kernels<3, 0>(a, grid, grids, st);
     } else {
-        if (a.temp_dense)
-            hipLaunchKernelGGL((k_light<2, 1>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+        if (a.tab_class)
+            launch_step_kernels<2, 1>(a, grid, grids, st);
         else
-            hipLaunchKernelGGL((k_light<2, 0>), dim3(grid), dim3(LTX * LTY), 0, st, a);
-        hipLaunchKernelGGL(k_merge<2>, dim3(grids[0]), dim3(256), 0, st, a);
-        hipLaunchKernelGGL(k_front<2>, dim3(grids[1]), dim3(128), 0, st, a);
-        hipLaunchKernelGGL(k_step_general<2>, dim3(grids[2]), dim3(64), 0, st, a);
+            launch_step_kernels<2, 0>(a, grid, grids, st);
     }
     return hipGetLastError();
 }

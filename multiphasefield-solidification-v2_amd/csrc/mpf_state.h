@@ -81,6 +81,7 @@ struct StepArgs {
     int *wl_count;   // [3][2] counters, ping-pong on the step parity
     int step;        // step index (for the ping-pong counters and halt)
     int halt_mode;   // stop at the first KMAX overflow
+    int tab_class;   // 1: the tables have the class form (K.cls holds the values)
     Stats *stats;
     int ntx, nty;    // tiles in x and y
 };

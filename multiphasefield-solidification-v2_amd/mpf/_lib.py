@@ -60,6 +60,7 @@ class Stats(C.Structure):
         ("overflow_cells", C.c_ulonglong), ("general_cells", C.c_ulonglong),
         ("halt_step", C.c_int), ("steps_done", C.c_int), ("last_step_ms", C.c_double),
         ("merge_cells", C.c_ulonglong), ("front_cells", C.c_ulonglong),
+        ("class_tables", C.c_int), ("pad_", C.c_int),
     ]
 
 

@@ -156,7 +156,7 @@ class Solver:
                 "overflow_cells": s.overflow_cells, "general_cells": s.general_cells,
                 "halt_step": s.halt_step, "steps_done": s.steps_done,
                 "last_step_ms": s.last_step_ms, "merge_cells": s.merge_cells,
-                "front_cells": s.front_cells}
+                "front_cells": s.front_cells, "class_tables": bool(s.class_tables)}
 
     def reset_stats(self):
         check(_lib.lib().mpf_reset_stats(self._h))

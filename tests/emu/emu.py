@@ -61,9 +61,12 @@ def unpack(liq, cnt, sid, sval, N, grid):
     return np.ascontiguousarray(np.transpose(out, (0,) + tuple(range(len(grid), 0, -1))))
 
 
-def step(phi, T_cooled, tables, scalars, kmax=50, aniso=True, torque=True, precision="fp64", mode=0):
+def step(phi, T_cooled, tables, scalars, kmax=50, aniso=True, torque=True, precision="fp64", mode=0,
+         use_class=False):
     """One step of the device code on the CPU.  T_cooled is the dense [nx, ny, nz]
-    temperature after this step's cooling.  Returns (phi_new, path counts)."""
+    temperature after this step's cooling.  use_class selects the class-form
+    table code (the tables must have that form).  Returns (phi_new, path
+    counts, per-cell path)."""
     from mpf import _lib as L
 
     dt = np.float64 if precision == "fp64" else np.float32
@@ -86,9 +89,11 @@ def step(phi, T_cooled, tables, scalars, kmax=50, aniso=True, torque=True, preci
     path_of = np.zeros(liq.size, np.int8)
     s = L.scalars_struct(scalars)
     vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
-    lib().emu_step(C.c_int(1 if precision == "fp64" else 0), C.c_int(dim), C.byref(s), nx, ny, nz, N, kmax,
-                   int(aniso), int(torque), vp(tab), vp(liq), vp(cnt), vp(sid), vp(sval), vp(lo), vp(co),
-                   vp(so), vp(vo), T.ctypes.data_as(C.POINTER(C.c_double)), mode,
-                   paths.ctypes.data_as(C.POINTER(C.c_int)), vp(path_of))
+    rc = lib().emu_step(C.c_int(1 if precision == "fp64" else 0), C.c_int(dim), C.byref(s), nx, ny, nz, N, kmax,
+                        int(aniso), int(torque), vp(tab), vp(liq), vp(cnt), vp(sid), vp(sval), vp(lo), vp(co),
+                        vp(so), vp(vo), T.ctypes.data_as(C.POINTER(C.c_double)), mode,
+                        paths.ctypes.data_as(C.POINTER(C.c_int)), vp(path_of), int(bool(use_class)))
+    if rc != 0:
+        raise ValueError("the tables do not have the class form")
     po = np.ascontiguousarray(np.transpose(path_of.reshape(tuple(reversed(grid))), tuple(range(dim - 1, -1, -1))))
     return unpack(lo, co, so, vo, N, grid), paths, po

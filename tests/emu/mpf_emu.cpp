@@ -16,7 +16,7 @@
 
 using namespace mpf;
 
-template <typename R, int DIM, int A>
+template <typename R, int DIM, int A, int TC>
 static bool run_cell(const Consts<R> &K, const SlotView<R> &in, const SlotView<R> &out, size_t ncell, int l, int m,
                      int k, double T)
 {
@@ -26,12 +26,12 @@ static bool run_cell(const Consts<R> &K, const SlotView<R> &in, const SlotView<R
     gather<R, DIM, A>(K, in, ncell, l, m, k, C, &ntot, &fits);
     if (!fits) return false;
     SlotLiq<R> L{in.liq, K.nx, K.ny};
-    cell_update<R, DIM, A>(K, L, l, m, k, T, C);
+    cell_update<R, DIM, A, true, TC>(K, L, l, m, k, T, C);
     write_cell<R, DIM, A>(out, ncell, cidx(l, m, k, K.nx, K.ny), C);
     return true;
 }
 
-template <typename R, int DIM>
+template <typename R, int DIM, int TC>
 static int step(const Consts<R> &K, SlotView<R> in, SlotView<R> out, const double *T, int mode, int *paths,
                 int8_t *path_of)
 {
@@ -75,39 +75,59 @@ static int step(const Consts<R> &K, SlotView<R> in, SlotView<R> out, const doubl
                         if (path_of) path_of[c] = 0;
                         continue;
                     }
-                    if (run_cell<R, DIM, 4>(K, in, out, ncell, l, m, k, T[c])) {
+                    if (run_cell<R, DIM, 4, TC>(K, in, out, ncell, l, m, k, T[c])) {
                         paths[1]++;
                         if (path_of) path_of[c] = 1;
                         continue;
                     }
                 }
-                if (mode == 1 && run_cell<R, DIM, 8>(K, in, out, ncell, l, m, k, T[c])) {
+                if (mode == 1 && run_cell<R, DIM, 8, TC>(K, in, out, ncell, l, m, k, T[c])) {
                     paths[2]++;
                     if (path_of) path_of[c] = 2;
                     continue;
                 }
-                run_cell<R, DIM, KCAP>(K, in, out, ncell, l, m, k, T[c]);
+                run_cell<R, DIM, KCAP, TC>(K, in, out, ncell, l, m, k, T[c]);
                 paths[3]++;
                 if (path_of) path_of[c] = 3;
             }
     return 0;
 }
 
+template <typename R>
+static int step_any(Consts<R> &K, int dim, int use_class, SlotView<R> in, SlotView<R> out, const double *T, int mode,
+                    int *paths, int8_t *path_of)
+{
+    R cls[3][2];
+    const bool tc = use_class && class_tables<R>(K.tab, K.N, cls);
+    if (tc)
+        for (int t = 0; t < 3; ++t) K.cls[t][0] = cls[t][0], K.cls[t][1] = cls[t][1];
+    if (dim == 3)
+        return tc ? step<R, 3, 1>(K, in, out, T, mode, paths, path_of) : step<R, 3, 0>(K, in, out, T, mode, paths, path_of);
+    return tc ? step<R, 2, 1>(K, in, out, T, mode, paths, path_of) : step<R, 2, 0>(K, in, out, T, mode, paths, path_of);
+}
+
+// use_class: select the class-form table kernels when the tables have that
+// form (as the solver does); returns -1 if asked for but the form is absent.
 extern "C" int emu_step(int precision, int dim, const mpf_scalars *s, int nx, int ny, int nz, int N, int kmax,
                         int aniso, int torque, const void *tables, const void *liq_in, const uint8_t *cnt_in,
                         const uint16_t *sid_in, const void *sval_in, void *liq_out, uint8_t *cnt_out,
-                        uint16_t *sid_out, void *sval_out, const double *T, int mode, int *paths, int8_t *path_of)
+                        uint16_t *sid_out, void *sval_out, const double *T, int mode, int *paths, int8_t *path_of,
+                        int use_class)
 {
     if (precision == MPF_FP64) {
         Consts<double> K = make_consts<double>(*s, nx, ny, nz, N, kmax, dim == 2 ? 1 : aniso, dim == 2 ? 1 : torque,
                                                tables);
+        double cls[3][2];
+        if (use_class && !class_tables<double>(K.tab, N, cls)) return -1;
         SlotView<double> in{(double *)liq_in, (uint8_t *)cnt_in, (uint16_t *)sid_in, (double *)sval_in};
         SlotView<double> out{(double *)liq_out, cnt_out, sid_out, (double *)sval_out};
-        return dim == 3 ? step<double, 3>(K, in, out, T, mode, paths, path_of) : step<double, 2>(K, in, out, T, mode, paths, path_of);
+        return step_any<double>(K, dim, use_class, in, out, T, mode, paths, path_of);
     }
     Consts<float> K = make_consts<float>(*s, nx, ny, nz, N, kmax, dim == 2 ? 1 : aniso, dim == 2 ? 1 : torque,
                                          tables);
+    float cls[3][2];
+    if (use_class && !class_tables<float>(K.tab, N, cls)) return -1;
     SlotView<float> in{(float *)liq_in, (uint8_t *)cnt_in, (uint16_t *)sid_in, (float *)sval_in};
     SlotView<float> out{(float *)liq_out, cnt_out, sid_out, (float *)sval_out};
-    return dim == 3 ? step<float, 3>(K, in, out, T, mode, paths, path_of) : step<float, 2>(K, in, out, T, mode, paths, path_of);
+    return step_any<float>(K, dim, use_class, in, out, T, mode, paths, path_of);
 }

@@ -9,7 +9,7 @@ from emu import emu
 from oracle import oracle as O
 
 
-def _run(phi, temp, tabs, scal, steps, mode=0, precision="fp64", aniso=True, torque=True):
+def _run(phi, temp, tabs, scal, steps, mode=0, precision="fp64", aniso=True, torque=True, use_class=False):
     cur_ref = phi.copy()
     cur = phi.copy()
     used = np.zeros(4, int)
@@ -17,7 +17,7 @@ def _run(phi, temp, tabs, scal, steps, mode=0, precision="fp64", aniso=True, tor
         mf, nf, _ = O.apt(cur_ref, 50)
         ref = O.update(cur_ref, temp, mf, nf, tabs, scal, aniso=aniso, torque=torque)
         out, paths, _ = emu.step(cur if precision == "fp64" else cur_ref, temp, tabs, scal, mode=mode,
-                                 precision=precision, aniso=aniso, torque=torque)
+                                 precision=precision, aniso=aniso, torque=torque, use_class=use_class)
         used += paths
         yield ref, out, used
         cur_ref, cur = ref, out
@@ -25,9 +25,11 @@ def _run(phi, temp, tabs, scal, steps, mode=0, precision="fp64", aniso=True, tor
 
 @pytest.mark.parametrize("seed,n_solid", [(17, 4), (3, 7), (11, 12)])
 @pytest.mark.parametrize("mode", [0, 2])
-def test_fp64_voronoi_paths_bit_exact(seed, n_solid, mode):
+@pytest.mark.parametrize("use_class", [False, True])
+def test_fp64_voronoi_paths_bit_exact(seed, n_solid, mode, use_class):
     c = cases.voronoi_case(nx=10, ny=9, nz=12, n_solid=n_solid, seed=seed, qseed=seed + 1)
-    for ref, out, used in _run(c["phi"], c["temp"], c["tables"], c["scalars"], 3, mode=mode):
+    for ref, out, used in _run(c["phi"], c["temp"], c["tables"], c["scalars"], 3, mode=mode,
+                               use_class=use_class):
         assert np.array_equal(out, ref)
     if mode == 0:
         assert used[0] > 0 and used[1] > 0
@@ -75,13 +77,36 @@ def test_fp64_switchable_bit_exact(aniso, torque):
         assert np.array_equal(out, ref)
 
 
-def test_fp64_2d_bit_exact():
+@pytest.mark.parametrize("use_class", [False, True])
+def test_fp64_2d_bit_exact(use_class):
     c = cases.disk2d_case(n=32, radius=6.0)
-    for ref, out, _ in _run(c["phi"], c["temp"], c["tables"], c["scalars"], 5):
+    for ref, out, _ in _run(c["phi"], c["temp"], c["tables"], c["scalars"], 5, use_class=use_class):
         assert np.array_equal(out, ref)
 
 
-def test_fp32_step_error_small():
+def test_class_form_detection():
+    """Tables from build_interaction_matrices have the class form; changing one
+    off-diagonal entry, or a non-zero diagonal, removes it (the general lookup
+    path then runs, and still matches the oracle)."""
+    c = cases.voronoi_case(n_solid=5)
+    w, a, m, n111 = c["tables"]
+    emu.step(c["phi"], c["temp"], c["tables"], c["scalars"], use_class=True)
+    for t, (i, j) in ((0, (2, 3)), (1, (0, 4)), (2, (3, 3))):
+        tabs = [x.copy() for x in (w, a, m)] + [n111]
+        tabs[t][i, j] *= 1.5 if i != j else 0.0
+        if i == j:
+            tabs[t][i, j] = 1e-3
+        with pytest.raises(ValueError):
+            emu.step(c["phi"], c["temp"], tabs, c["scalars"], use_class=True)
+        mf, nf, _ = O.apt(c["phi"], 50)
+        ref = O.update(c["phi"], c["temp"], mf, nf, tabs, c["scalars"])
+        out, _, _ = emu.step(c["phi"], c["temp"], tabs, c["scalars"])
+        assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("use_class", [False, True])
+def test_fp32_step_error_small(use_class):
     c = cases.voronoi_case()
-    for ref, out, _ in _run(c["phi"], c["temp"], c["tables"], c["scalars"], 3, precision="fp32"):
+    for ref, out, _ in _run(c["phi"], c["temp"], c["tables"], c["scalars"], 3, precision="fp32",
+                            use_class=use_class):
         assert np.abs(out - ref).max() <= 1e-4

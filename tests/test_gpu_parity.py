@@ -240,3 +240,25 @@ def test_fp64_quad_junction_bit_exact():
     assert s.step(6) == 6
     assert np.array_equal(s.phi(), ref)
     s.close()
+
+
+def test_fp64_general_and_class_tables_bit_exact():
+    """Reference tables take the class-form kernels; a table with one entry
+    changed takes the general lookup kernels.  Both bit-identical to the oracle."""
+    c = cases.voronoi_case(n_solid=5)
+    steps = 6
+    with _solver_for(c, "fp64", temp_f32=True) as s:
+        assert s.stats()["class_tables"]
+        s.step(steps)
+        ref, _, _, _ = O.run(c["phi"], c["temp"], steps, c["tables"], c["scalars"], kmax=50, temp_f32=True)
+        assert np.array_equal(s.phi(), ref)
+    w, a, m, n111 = (t.copy() for t in c["tables"])
+    w[2, 3] *= 1.25
+    a[0, 4] *= 0.9
+    tabs = (w, a, m, n111)
+    c2 = dict(c, tables=tabs)
+    with _solver_for(c2, "fp64", temp_f32=True) as s:
+        assert not s.stats()["class_tables"]
+        s.step(steps)
+        ref, _, _, _ = O.run(c["phi"], c["temp"], steps, tabs, c["scalars"], kmax=50, temp_f32=True)
+        assert np.array_equal(s.phi(), ref)
