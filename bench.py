@@ -53,7 +53,10 @@ def workload(nx=NX, ny=NY, nz=NZ, n_solid=N_SOLID, seed_height=SEED_H):
             "cooling": float(np.float32(d["cooling_rate"])), "N": n_solid + 1}
 
 
-def run_gpu(W, precision, steps, warmup):
+def run_gpu(W, precision, steps, warmup, kernel_steps=0):
+    """Warm-up, then `steps` timed steps (HIP events on the solver's stream,
+    inputs resident).  With kernel_steps, a further run of that many steps with
+    every kernel bracketed by HIP events gives each kernel's live duration."""
     s = Solver((NX, NY, NZ), W["N"], W["scalars"], precision=precision, kmax=50, cooling_rate=W["cooling"])
     try:
         s.set_tables(*W["tables"])
@@ -68,8 +71,16 @@ def run_gpu(W, precision, steps, warmup):
         wall = time.perf_counter() - t0
         st = s.stats()
         ob1 = s.observables(dx=W["p"]["dx"])
+        kern = None
+        if kernel_steps:
+            s.set_kernel_timing(True)
+            s.reset_stats()
+            s.step(kernel_steps)
+            ks = s.stats()
+            kern = {k: v / ks["kernel_steps"] for k, v in ks["kernel_ms"].items()}
+            kern["step"] = ks["last_step_ms"] / ks["kernel_steps"]
         return {"ms": st["last_step_ms"], "wall": wall, "done": done, "ob0": ob0, "ob1": ob1, "stats": st,
-                "bytes": s.device_bytes}
+                "bytes": s.device_bytes, "kernel_ms": kern}
     finally:
         s.close()
 
@@ -80,6 +91,17 @@ def alg_bytes_per_step(ob0, ob1, bv, ncell, nz):
     (id + value) read once and written once, the per-plane temperature."""
     slots = 0.5 * (ob0["stored_slots"] + ob1["stored_slots"])
     return 2.0 * ncell * (bv + 1) + 2.0 * slots * (bv + 2) + 2.0 * nz * 8, slots / ncell
+
+
+def measured_traffic(path):
+    """HBM bytes per step of the step's kernels from the committed rocprofv3
+    PMC summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by what the
+    same counters read for a device copy of known size)."""
+    if not path or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return {"bytes_per_step": t["bytes_per_step"], "source": os.path.relpath(path, ROOT)}
 
 
 def cpu_baseline(W, steps=6):
@@ -120,6 +142,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy (reference for HBM counters)")
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "pmc_traffic_512x512x256_fp32.json"),
+                    help="rocprofv3 PMC traffic summary to report as roofline.traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the solver has no "
@@ -137,11 +161,13 @@ def main():
     W = workload()
     ncell = NX * NY * NZ
     log(f"workload ready: {NX}x{NY}x{NZ}, N={W['N']}")
-    r32 = run_gpu(W, "fp32", args.steps, args.warmup)
+    r32 = run_gpu(W, "fp32", args.steps, args.warmup, kernel_steps=min(args.steps, 100))
     ms32 = r32["ms"] / args.steps
     log(f"fp32: {ms32:.3f} ms/step, stats {r32['stats']}")
+    log(f"fp32 per-kernel ms (live HIP events): {r32['kernel_ms']}")
     B32, slot32 = alg_bytes_per_step(r32["ob0"], r32["ob1"], 4, ncell, NZ)
     achieved = B32 / (ms32 * 1e-3) / 1e9
+    traffic = measured_traffic(args.traffic_json)
 
     out = {
         "metric": METRIC, "value": round(ncell * args.steps / (r32["ms"] * 1e-3) / 1e6, 2), "unit": "MLUPS",
@@ -152,9 +178,13 @@ def main():
                    "nx": NX, "ny": NY, "nz": NZ, "n_solid": N_SOLID, "seed_height": SEED_H, "kmax": 50,
                    "precision": "fp32", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic["bytes_per_step"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
                      "alg_bytes_per_step": round(B32), "mean_stored_slots_per_cell": round(slot32, 4),
-                     "kernel": "k_step (fused temperature+APT+RHS+update) + k_step_general"},
+                     "kernel": "one step: k_light, k_merge, k_front on the solver stream, k_step_general "
+                               "on a side stream concurrent with k_front",
+                     "kernel_ms_live": {k: round(v, 4) for k, v in r32["kernel_ms"].items()}},
         "wall_s_timed": round(r32["wall"], 3),
         "front_mean_cells": r32["ob1"]["front_mean"],
         "cells_per_step": {"merge": r32["stats"]["merge_cells"] // args.steps,

@@ -83,7 +83,9 @@ typedef struct mpf_stats {
     int class_tables;               /* 1: the tables have the class form of
                                        build_interaction_matrices (src/seed_modes.py:333-380)
                                        and the step selects their values instead of loading them */
-    int pad_;
+    int kernel_steps;               /* steps timed per kernel (mpf_set_kernel_timing) */
+    double kernel_ms[4];            /* summed GPU time of the step's kernels over those steps:
+                                       classification, merge, interface, many-phase */
 } mpf_stats;
 
 /* ---- resident solver ---------------------------------------------------- */
@@ -125,6 +127,10 @@ int mpf_download_apt(mpf_handle *h, int32_t *mf, int depth, int32_t *nf, int32_t
 
 int mpf_get_stats(mpf_handle *h, mpf_stats *st);
 int mpf_reset_stats(mpf_handle *h);
+/* Time every kernel of every step with HIP events (on: 1) and sum the times
+ * into mpf_stats.kernel_ms.  Adds a few microseconds per step.  No
+ * counterpart in the reference (it has no timing). */
+int mpf_set_kernel_timing(mpf_handle *h, int on);
 
 /* FP64 observables of the current state (tests/verification_3d.py:409-416,
  * :612-654): grain_volume[N] = sum of phi_g over cells (times dx^3 by the

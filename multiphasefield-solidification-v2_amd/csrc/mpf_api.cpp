@@ -18,7 +18,8 @@ using namespace mpf;
 
 // ------------------------------------------------ launchers of both builds
 #define MPF_DECLARE_LAUNCHERS(R, SFX)                                                                    \
-    extern "C" hipError_t mpf_launch_step##SFX(const StepArgs<R> *, int, const int *, hipStream_t);      \
+    extern "C" hipError_t mpf_launch_step##SFX(const StepArgs<R> *, int, const int *, hipStream_t, hipStream_t, \
+                                               hipEvent_t, hipEvent_t, hipEvent_t *);                       \
     extern "C" hipError_t mpf_launch_apt_slots##SFX(const Consts<R> *, SlotView<R>, int, int32_t *,      \
                                                     int32_t *, int32_t *, int, hipStream_t);             \
     extern "C" hipError_t mpf_launch_pack##SFX(const void *, int, int, int, int, int, SlotView<R>, int,  \
@@ -80,12 +81,18 @@ struct mpf_handle {
     int scap = 1;
     size_t rsz = 8; // sizeof(Real)
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;            // many-phase kernel, concurrent with the interface kernel
+    hipEvent_t ev_merged = nullptr, ev_general = nullptr;
+    int kernel_timing = 0;                 // per-kernel events (mpf_set_kernel_timing)
+    std::vector<hipEvent_t> tev;           // 6 per step of the current call
+    double kern_ms[4] = {0, 0, 0, 0};      // k_light, k_merge, k_front, k_step_general
+    int kern_steps = 0;
     Buffers b[2];
     double *T[2] = {nullptr, nullptr};
     size_t nT = 0;
     void *tab = nullptr; // wij, aij, mij, n111 contiguous
-    int *wl[3] = {nullptr, nullptr, nullptr}; // merge, front, general work lists
-    int *wl_count = nullptr;                   // [3][2]
+    int *wl = nullptr;        // NWL work lists of ncell entries (mpf_state.h)
+    int *wl_count = nullptr;  // [NWL][2]
     Stats *stats = nullptr;
     int cur = 0, tcur = 0;
     int step_counter = 0;
@@ -134,9 +141,14 @@ extern "C" void mpf_destroy(mpf_handle *h)
         hipFree(h->T[i]);
     }
     hipFree(h->tab);
-    for (int i = 0; i < 3; ++i) hipFree(h->wl[i]);
+    hipFree(h->wl);
     hipFree(h->wl_count);
     hipFree(h->stats);
+    if (h->side) hipStreamSynchronize(h->side);
+    for (hipEvent_t e : h->tev) hipEventDestroy(e);
+    if (h->ev_merged) hipEventDestroy(h->ev_merged);
+    if (h->ev_general) hipEventDestroy(h->ev_general);
+    if (h->side) hipStreamDestroy(h->side);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->stream) hipStreamDestroy(h->stream);
@@ -149,7 +161,7 @@ static int reset_stats_dev(mpf_handle *h)
     memset(&s, 0, sizeof(s));
     s.halt_step = NO_HALT;
     HIP_TRY(hipMemcpyAsync(h->stats, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemsetAsync(h->wl_count, 0, 6 * sizeof(int), h->stream));
+    HIP_TRY(hipMemsetAsync(h->wl_count, 0, 2 * NWL * sizeof(int), h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     return 0;
 }
@@ -215,8 +227,8 @@ extern "C" int mpf_create(const mpf_config *cfg, mpf_handle **out)
     for (int i = 0; i < 2; ++i) TRY_RC(dev_alloc(h, (void **)&h->T[i], h->nT * sizeof(double)));
     size_t nn = (size_t)c.n_phase * c.n_phase;
     TRY_RC(dev_alloc(h, &h->tab, (3 * nn + (size_t)c.n_phase * 24) * h->rsz));
-    for (int i = 0; i < 3; ++i) TRY_RC(dev_alloc(h, (void **)&h->wl[i], ncell * sizeof(int)));
-    TRY_RC(dev_alloc(h, (void **)&h->wl_count, 6 * sizeof(int)));
+    TRY_RC(dev_alloc(h, (void **)&h->wl, (size_t)NWL * ncell * sizeof(int)));
+    TRY_RC(dev_alloc(h, (void **)&h->wl_count, 2 * NWL * sizeof(int)));
     TRY_RC(dev_alloc(h, (void **)&h->stats, sizeof(Stats)));
     // empty state: all liquid, no grains
     for (int i = 0; i < 2; ++i) {
@@ -229,7 +241,10 @@ extern "C" int mpf_create(const mpf_config *cfg, mpf_handle **out)
             return fail(MPF_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
         }
     }
-    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
+        hipStreamCreate(&h->side) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_merged, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_general, hipEventDisableTiming) != hipSuccess) {
         mpf_destroy(h);
         return fail(MPF_EHIP, "hipEventCreate failed");
     }
@@ -373,7 +388,7 @@ extern "C" int mpf_download_temp(mpf_handle *h, double *temp)
 }
 
 template <typename R>
-static hipError_t launch_one_step(mpf_handle *h, const int *grids)
+static hipError_t launch_one_step(mpf_handle *h, const int *grids, hipEvent_t *tev)
 {
     const mpf_config &c = h->cfg;
     StepArgs<R> a;
@@ -390,7 +405,7 @@ static hipError_t launch_one_step(mpf_handle *h, const int *grids)
     a.temp_dense = c.temp_dense;
     a.temp_f32 = c.temp_f32;
     a.cooling = c.cooling_rate;
-    for (int i = 0; i < 3; ++i) a.wl[i] = h->wl[i];
+    a.wl_base = h->wl;
     a.wl_count = h->wl_count;
     a.step = h->step_counter;
     a.halt_mode = c.halt_on_kmax;
@@ -401,8 +416,10 @@ static hipError_t launch_one_step(mpf_handle *h, const int *grids)
     }
     a.stats = h->stats;
     if (sizeof(R) == 8)
-        return mpf_launch_step_f64((const StepArgs<double> *)&a, c.dim, grids, h->stream);
-    return mpf_launch_step_f32((const StepArgs<float> *)&a, c.dim, grids, h->stream);
+        return mpf_launch_step_f64((const StepArgs<double> *)&a, c.dim, grids, h->stream, h->side, h->ev_merged,
+                                   h->ev_general, tev);
+    return mpf_launch_step_f32((const StepArgs<float> *)&a, c.dim, grids, h->stream, h->side, h->ev_merged,
+                               h->ev_general, tev);
 }
 
 extern "C" int mpf_step(mpf_handle *h, int nsteps)
@@ -416,7 +433,7 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
     HIP_TRY(hipStreamSynchronize(h->stream));
     int m1 = NO_HALT;
     HIP_TRY(hipMemcpy(&h->stats->halt_step, &m1, sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(h->wl_count, 0, 6 * sizeof(int)));
+    HIP_TRY(hipMemset(h->wl_count, 0, 2 * NWL * sizeof(int)));
     const int s0 = h->step_counter, cur0 = h->cur, tcur0 = h->tcur;
     // grid-stride work-list kernels: enough waves to fill the GPU, fewer for
     // small grids
@@ -425,10 +442,19 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
         return (int)(b < (size_t)cap ? (b ? b : 1) : cap);
     };
     const int grids[3] = {blocks(256, 2048), blocks(128, 2048), blocks(64, 256)};
+    if (h->kernel_timing) {
+        const size_t need = (size_t)nsteps * 6;
+        while (h->tev.size() < need) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            h->tev.push_back(e);
+        }
+    }
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
     for (int i = 0; i < nsteps; ++i) {
-        hipError_t e = (h->rsz == 8) ? launch_one_step<double>(h, grids)
-                                     : launch_one_step<float>(h, grids);
+        hipEvent_t *tev = h->kernel_timing ? h->tev.data() + (size_t)i * 6 : nullptr;
+        hipError_t e = (h->rsz == 8) ? launch_one_step<double>(h, grids, tev)
+                                     : launch_one_step<float>(h, grids, tev);
         if (e != hipSuccess) return fail(MPF_EHIP, "step launch: %s", hipGetErrorString(e));
         h->cur ^= 1;
         h->tcur ^= 1;
@@ -439,6 +465,19 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->last_ms = ms;
+    if (h->kernel_timing) {
+        HIP_TRY(hipStreamSynchronize(h->side));
+        for (int i = 0; i < nsteps; ++i) {
+            hipEvent_t *t = h->tev.data() + (size_t)i * 6;
+            float k[4];
+            HIP_TRY(hipEventElapsedTime(&k[0], t[0], t[1]));
+            HIP_TRY(hipEventElapsedTime(&k[1], t[1], t[2]));
+            HIP_TRY(hipEventElapsedTime(&k[2], t[2], t[3]));
+            HIP_TRY(hipEventElapsedTime(&k[3], t[4], t[5]));
+            for (int j = 0; j < 4; ++j) h->kern_ms[j] += k[j];
+        }
+        h->kern_steps += nsteps;
+    }
     int done = nsteps;
     if (c.halt_on_kmax) {
         int hs = -1;
@@ -522,13 +561,23 @@ extern "C" int mpf_get_stats(mpf_handle *h, mpf_stats *st)
     st->merge_cells = s.merge_cells;
     st->front_cells = s.front_cells;
     st->class_tables = h->tab_class;
-    st->pad_ = 0;
+    for (int j = 0; j < 4; ++j) st->kernel_ms[j] = h->kern_ms[j];
+    st->kernel_steps = h->kern_steps;
+    return 0;
+}
+
+extern "C" int mpf_set_kernel_timing(mpf_handle *h, int on)
+{
+    if (!h) return fail(MPF_EINVAL, "mpf_set_kernel_timing: null handle");
+    h->kernel_timing = on ? 1 : 0;
     return 0;
 }
 
 extern "C" int mpf_reset_stats(mpf_handle *h)
 {
     if (!h) return fail(MPF_EINVAL, "mpf_reset_stats: null handle");
+    for (int j = 0; j < 4; ++j) h->kern_ms[j] = 0.0;
+    h->kern_steps = 0;
     return reset_stats_dev(h);
 }
 

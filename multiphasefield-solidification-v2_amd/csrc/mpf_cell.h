@@ -68,13 +68,57 @@ struct Consts {
 // is identical.
 enum { TAB_W = 0, TAB_A = 1, TAB_M = 2 };
 
+// A kernel-uniform value moved into a vector register.  The unrolled pair
+// loop selects among the class values many times; kept in scalar registers,
+// which the kernel has run out of, they would be reloaded from spill lanes at
+// every use.
+MPF_DEV float uniform_in_vgpr(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    float r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+#else
+    return x;
+#endif
+}
+
+MPF_DEV double uniform_in_vgpr(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r;
+    asm volatile("v_mov_b64 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+#else
+    return x;
+#endif
+}
+
+template <typename Real>
+struct ClassVals {
+    Real v[3][2];
+};
+
+template <int TC, typename Real>
+MPF_DEV ClassVals<Real> class_vals(const Consts<Real> &K)
+{
+    ClassVals<Real> c;
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) c.v[t][u] = TC ? uniform_in_vgpr(K.cls[t][u]) : Real(0.0);
+    return c;
+}
+
 template <int TC, int T, typename Real>
-MPF_DEV Real tab_at(const Consts<Real> &K, int i, int j)
+MPF_DEV Real tab_at(const Consts<Real> &K, const ClassVals<Real> &cv, int i, int j)
 {
     if constexpr (TC != 0) {
+        (void)K;
         if (i == j) return Real(0.0);
-        return (i == 0 || j == 0) ? K.cls[T][0] : K.cls[T][1];
+        return (i == 0 || j == 0) ? cv.v[T][0] : cv.v[T][1];
     } else {
+        (void)cv;
         return K.tab[(size_t)T * K.nn + (size_t)i * K.N + j];
     }
 }
@@ -665,6 +709,7 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
     const double dG = K.Sf * (Tcell - K.T_melt);
     const Real df_sl = Real(-dG), df_ls = Real(dG);
     const Real n_real = Real(n_act);
+    const ClassVals<Real> cv = class_vals<TC>(K);
 #pragma unroll UNR_NA
     for (int t1 = 0; t1 < TMAX; ++t1) {
         S.out[t1] = Real(0.0);
@@ -689,8 +734,8 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
                 if (!p3) continue;
                 const int pk = S.gid[t3];
                 const Real phi_k = S.v[t3][P_C];
-                Real wik = tab_at<TC, TAB_W>(K, i, pk);
-                Real wjk = tab_at<TC, TAB_W>(K, j, pk);
+                Real wik = tab_at<TC, TAB_W>(K, cv, i, pk);
+                Real wjk = tab_at<TC, TAB_W>(K, cv, j, pk);
                 if (pk == 0) {
                     if (i != 0) wik = w_sl;
                     if (j != 0) wjk = w_sl;
@@ -702,7 +747,7 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
                 } else if (i == 0 && j != 0) {
                     if (pk == j) term2 = Real(-0.5) * lap_sl[t2];
                 } else {
-                    Real aik = tab_at<TC, TAB_A>(K, i, pk), ajk = tab_at<TC, TAB_A>(K, j, pk);
+                    Real aik = tab_at<TC, TAB_A>(K, cv, i, pk), ajk = tab_at<TC, TAB_A>(K, cv, j, pk);
                     Real eps2ik = aik * aik;
                     Real eps2jk = ajk * ajk;
                     term2 = Real(0.5) * (eps2ik - eps2jk) * lap[t3];
@@ -711,7 +756,7 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
             }
             Real phii_phij = S.v[t1][P_C] * S.v[t2][P_C];
             Real term_force = Real(8.0 / 3.1415926535) * m_sqrt(pymax(phii_phij, Real(0.0))) * driving_force;
-            const Real mij_tab = tab_at<TC, TAB_M>(K, i, j);
+            const Real mij_tab = tab_at<TC, TAB_M>(K, cv, i, j);
             Real mij_eff = mij_tab;
             if (i != 0 && j == 0)
                 mij_eff = mij_tab * b_sl[t1];

@@ -70,28 +70,37 @@ struct DenseLiq {
 // k-way merge of the sorted slot lists of the stencil points.  Yields the
 // union of grain ids present at any of the points in ascending order with the
 // value at every point (0 where absent).
+//
+// Each list keeps its head and the slot after it in registers: both are read
+// in one round trip after the counts, and a refill (lists of 3 or more
+// slots) is needed only two merge steps after it is issued, so the merge does
+// not wait on memory once per step.
 template <typename Real, int DIM>
 struct Merge {
     static constexpr int NP = (DIM == 3) ? 7 : 5;
     size_t pc[NP];
-    int cnt[NP], pos[NP];
-    uint32_t hid[NP];
-    Real hv[NP];
+    int cnt[NP], pos[NP];   // pos: slot index of the head
+    uint32_t hid[NP], nid[NP];
+    Real hv[NP], nv[NP];
 
     MPF_DEV void init(const SlotView<Real> &S, size_t ncell)
     {
 #pragma unroll
+        for (int p = 0; p < NP; ++p) cnt[p] = S.cnt[pc[p]];
+#pragma unroll
         for (int p = 0; p < NP; ++p) {
-            cnt[p] = S.cnt[pc[p]];
             pos[p] = 0;
-            hid[p] = NO_ID;
-            hv[p] = Real(0.0);
+            hid[p] = nid[p] = NO_ID;
+            hv[p] = nv[p] = Real(0.0);
             if (cnt[p] > 0) {
                 hid[p] = S.sid[pc[p]];
                 hv[p] = S.sval[pc[p]];
             }
+            if (cnt[p] > 1) {
+                nid[p] = S.sid[ncell + pc[p]];
+                nv[p] = S.sval[ncell + pc[p]];
+            }
         }
-        (void)ncell;
     }
 
     // Next candidate; returns NO_ID when all lists are exhausted.
@@ -106,12 +115,13 @@ struct Merge {
             if (hid[p] == g) {
                 vals[p] = hv[p];
                 pos[p] += 1;
-                if (pos[p] < cnt[p]) {
-                    size_t o = (size_t)pos[p] * ncell + pc[p];
-                    hid[p] = S.sid[o];
-                    hv[p] = S.sval[o];
-                } else {
-                    hid[p] = NO_ID;
+                hid[p] = nid[p];
+                hv[p] = nv[p];
+                nid[p] = NO_ID;
+                if (pos[p] + 1 < cnt[p]) {
+                    size_t o = (size_t)(pos[p] + 1) * ncell + pc[p];
+                    nid[p] = S.sid[o];
+                    nv[p] = S.sval[o];
                 }
             } else {
                 vals[p] = Real(0.0);

@@ -89,6 +89,11 @@ __device__ __forceinline__ int *wl_counter(const StepArgs<Real> &a, int list, in
     return a.wl_count + list * 2 + (step & 1);
 }
 
+__device__ __forceinline__ int *wl_list(const StepArgs<Real> &a, int list)
+{
+    return a.wl_base + (size_t)list * ((size_t)a.K.nx * a.K.ny * a.K.nz);
+}
+
 // Pass 1 over every cell: temperature, classification and the cells whose
 // stencil holds only liquid (no stored grain at any of the 7 points) or one
 // grain with zero liquid everywhere.  Everything else is queued for k_merge.
@@ -99,50 +104,65 @@ __device__ __forceinline__ int *wl_counter(const StepArgs<Real> &a, int list, in
 // queued cells of the whole block are collected in LDS and appended to the
 // work list with a single atomic.
 //
-// The loads of plane z+1 are issued before plane z is processed.  Every lane
-// issues the same loads on every plane (lanes without a cell, or planes whose
-// occupancy bit is clear, read a line that stays cached and discard it), so
-// the number of memory operations in flight is static and the compiler waits
-// only for the plane it needs.  Slot 0 is read together with the count and
-// dropped when the count is zero.
+// The loads of the next plane are issued before the current plane is
+// processed.  Every lane issues the same loads on every plane (lanes outside
+// the grid read a clamped cell, lanes without a halo role read their own
+// cell, and planes whose occupancy bit is clear read slot 0 of the first 64
+// cells, a line that stays cached), so the number of memory operations in
+// flight is static and the compiler waits only for the plane it needs.  The
+// march is unrolled by four so that the column ring (z-1, z, z+1, z+2) and
+// the halo double buffer are fixed registers.  Slot 0 is read together with
+// the count and dropped when the count is zero.  Offsets are cell indices of
+// type Off (32-bit when every byte offset of the arrays fits in 32 bits).
 struct ColCell {
-    int cnt;
+    uint8_t cnt;
     Real liq;
-    uint32_t sid;   // slot-0 grain (NO_ID when empty)
+    uint16_t sid;   // slot-0 grain (NO_ID when empty)
     Real sv;        // slot-0 value
 };
 
-// c: cell for count and liquid; cs: cell whose slot 0 is read (c itself, or
-// the cached line when the cell cannot hold a grain)
-__device__ __forceinline__ ColCell load_cell(const SlotView<Real> &S, size_t c, size_t cs)
+// base[i] with a 32-bit byte offset when Off is 32-bit (lets the compiler use
+// a scalar base with a 32-bit vector offset)
+template <typename T, typename Off>
+__device__ __forceinline__ T &at(T *base, Off i)
+{
+    if constexpr (sizeof(Off) == 4)
+        return *(T *)((char *)base + (uint32_t)(i * (uint32_t)sizeof(T)));
+    else
+        return base[i];
+}
+
+// c: cell for count and liquid; cs: cell whose slot 0 is read
+template <typename Off>
+__device__ __forceinline__ ColCell load_cell(const SlotView<Real> &S, Off c, Off cs)
 {
     ColCell r;
-    r.cnt = S.cnt[c];
-    r.liq = S.liq[c];
-    r.sid = S.sid[cs];
-    r.sv = S.sval[cs];
+    r.cnt = at(S.cnt, c);
+    r.liq = at(S.liq, c);
+    r.sid = at(S.sid, cs);
+    r.sv = at(S.sval, cs);
     return r;
 }
 
 __device__ __forceinline__ void finish_cell(ColCell &r)
 {
     const bool e = r.cnt == 0;
-    r.sid = e ? NO_ID : r.sid;
+    r.sid = e ? (uint16_t)NO_ID : r.sid;
     r.sv = e ? Real(0.0) : r.sv;
 }
 
-template <int DIM, int TD>
+template <int DIM, int TD, typename Off>
 __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
-    if (blockIdx.x == 0 && threadIdx.x < 3) a.wl_count[threadIdx.x * 2 + ((a.step + 1) & 1)] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < NWL) a.wl_count[threadIdx.x * 2 + ((a.step + 1) & 1)] = 0;
     const Consts<Real> &K = a.K;
     constexpr int NP = (DIM == 3) ? 7 : 5;
     constexpr int ZC = (DIM == 3) ? LZC : 1;
     // plane tiles, double-buffered so one barrier per plane suffices
-    __shared__ uint8_t s_cnt[2][LTY + 2][LTX + 2];
+    // key = count | slot-0 grain << 8
+    __shared__ uint32_t s_key[2][LTY + 2][LTX + 2];
     __shared__ Real s_liq[2][LTY + 2][LTX + 2];
-    __shared__ uint16_t s_sid[2][LTY + 2][LTX + 2];
     __shared__ Real s_sv[2][LTY + 2][LTX + 2];
     __shared__ uint16_t s_q[LTX * LTY * ZC]; // queued cells as (z-plane, row, column) in the tile
     __shared__ int s_qn, s_qbase, s_any;
@@ -161,9 +181,7 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
     const bool valid = (lx < w) && (ly < h);
     const int k0 = tz * ZC;
     const int k1 = (DIM == 3) ? min(k0 + ZC, K.nz) : 1;
-    const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
-    // the discarded loads go to the first 64 cells (one line per array)
-    const size_t cdum = min((size_t)(threadIdx.x & 63), ncell - 1);
+    const Off plane = (Off)K.nx * (Off)K.ny;
     if (threadIdx.x == 0) {
         s_qn = 0;
         s_any = 0;
@@ -192,24 +210,27 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
     }
     const uint32_t occx = hxl >= 0 ? a.occ_in[tile_of(hxl / LTX, ty, tz)] : 0u;
     const uint32_t occy = hym >= 0 ? a.occ_in[tile_of(tx, hym / LTY, tz)] : 0u;
+    // in-plane cell offsets: own (clamped into the grid), halo x, halo y
+    // (lanes without a halo role use their own cell)
+    const Off own_xy = (Off)min(m, K.ny - 1) * (Off)K.nx + (Off)min(l, K.nx - 1);
+    const Off hx_xy = hxl >= 0 ? (Off)m * (Off)K.nx + (Off)hxl : own_xy;
+    const Off hy_xy = hym >= 0 ? (Off)hym * (Off)K.nx + (Off)l : own_xy;
+    const Off dum = (Off)(threadIdx.x & 63) < plane * (Off)(DIM == 3 ? K.nz : 1) ? (Off)(threadIdx.x & 63) : (Off)0;
 
     auto own_cell = [&](int kk) {
-        const bool occ = valid && ((colmask >> (kk - k0 + 1)) & 1u);
-        const size_t c = valid ? cidx(l, m, kk, K.nx, K.ny) : cdum;
-        return load_cell(a.in, c, occ ? c : cdum);
+        const bool occ = (colmask >> (kk - k0 + 1)) & 1u;
+        const Off c = own_xy + (Off)kk * plane;
+        return load_cell<Off>(a.in, c, occ ? c : dum);
     };
     auto halo_x = [&](int kk) {
-        const bool occ = hxl >= 0 && ((occx >> (kk - k0)) & 1u);
-        const size_t c = hxl >= 0 ? cidx(hxl, m, kk, K.nx, K.ny) : cdum;
-        return load_cell(a.in, c, occ ? c : cdum);
+        const bool occ = (occx >> (kk - k0)) & 1u;
+        const Off c = hx_xy + (Off)kk * plane;
+        return load_cell<Off>(a.in, c, occ ? c : dum);
     };
     auto halo_y = [&](int kk) {
-        const bool occ = hym >= 0 && ((occy >> (kk - k0)) & 1u);
-        const size_t c = hym >= 0 ? cidx(l, hym, kk, K.nx, K.ny) : cdum;
-        return load_cell(a.in, c, occ ? c : cdum);
-    };
-    auto temp_at = [&](int kk) -> double {
-        return a.T_in[valid ? cidx(l, m, kk, K.nx, K.ny) : cdum];
+        const bool occ = (occy >> (kk - k0)) & 1u;
+        const Off c = hy_xy + (Off)kk * plane;
+        return load_cell<Off>(a.in, c, occ ? c : dum);
     };
     // per-plane temperature: one lane of the tiles at x = y = 0 updates the
     // planes of its z-range
@@ -220,145 +241,143 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
         }
     }
 
-    ColCell cp{}, cc, cn{}, hx, hy;
-    double Tc = 0.0;
-    cc = own_cell(k0);
-    if (DIM == 3) {
-        cp = own_cell(bc_mm(k0, K.nz));
-        cn = own_cell(bc_mp(k0, K.nz));
-    }
-    hx = halo_x(k0);
-    hy = halo_y(k0);
-    if (TD) Tc = temp_at(k0);
-    finish_cell(cp);
-    finish_cell(cc);
+    // ring[r & 3] holds plane k0 + r - 1 of the own column; hr/yr/tr[r & 1]
+    // hold the halo and temperature of plane k0 + r
+    ColCell ring[4], hr[2], yr[2];
+    double tr[2] = {0.0, 0.0};
+    ring[0] = (DIM == 3) ? own_cell(bc_mm(k0, K.nz)) : ColCell{};
+    ring[1] = own_cell(k0);
+    ring[2] = (DIM == 3) ? own_cell(bc_mp(k0, K.nz)) : ColCell{};
+    ring[3] = ColCell{};
+    hr[0] = halo_x(k0);
+    yr[0] = halo_y(k0);
+    hr[1] = yr[1] = ColCell{};
+    if (TD) tr[0] = at(a.T_in, own_xy + (Off)k0 * plane);
+    finish_cell(ring[0]);
+    finish_cell(ring[1]);
     uint32_t occ_bits = 0;
 
-    for (int k = k0; k < k1; ++k) {
-        const int b = (k - k0) & 1;
-        // loads of the next plane (the last plane reloads itself; unused)
-        ColCell cnn{}, nhx{}, nhy{};
-        double Tn = 0.0;
-        if (DIM == 3) {
-            const int kn = min(k + 1, k1 - 1);
-            cnn = own_cell(bc_mp(kn, K.nz));
-            nhx = halo_x(kn);
-            nhy = halo_y(kn);
-            if (TD) Tn = temp_at(kn);
-        }
-        finish_cell(cn);
-        finish_cell(hx);
-        finish_cell(hy);
-        if (valid) {
-            s_cnt[b][ly + 1][lx + 1] = (uint8_t)cc.cnt;
-            s_liq[b][ly + 1][lx + 1] = cc.liq;
-            s_sid[b][ly + 1][lx + 1] = (uint16_t)cc.sid;
-            s_sv[b][ly + 1][lx + 1] = cc.sv;
-        }
-        if (hxl >= 0) {
-            s_cnt[b][ly + 1][hxs] = (uint8_t)hx.cnt;
-            s_liq[b][ly + 1][hxs] = hx.liq;
-            s_sid[b][ly + 1][hxs] = (uint16_t)hx.sid;
-            s_sv[b][ly + 1][hxs] = hx.sv;
-        }
-        if (hym >= 0) {
-            s_cnt[b][hys][lx + 1] = (uint8_t)hy.cnt;
-            s_liq[b][hys][lx + 1] = hy.liq;
-            s_sid[b][hys][lx + 1] = (uint16_t)hy.sid;
-            s_sv[b][hys][lx + 1] = hy.sv;
-        }
-        __syncthreads();
-        bool queue = false, grains = false;
-        if (valid) {
-            const size_t c = cidx(l, m, k, K.nx, K.ny);
-            if (TD) a.T_out[c] = a.temp_f32 ? (double)((float)Tc - (float)a.cooling) : Tc - a.cooling;
-            // stencil: c, x+, x-, y+, y-, z+, z-
-            int pcnt[NP];
-            Real pl[NP];
-            uint32_t psid[NP];
-            Real psv[NP];
-            const int yy = ly + 1, xx = lx + 1;
-            pcnt[P_C] = cc.cnt; pl[P_C] = cc.liq; psid[P_C] = cc.sid; psv[P_C] = cc.sv;
-            pcnt[P_XP] = s_cnt[b][yy][xx + 1]; pl[P_XP] = s_liq[b][yy][xx + 1];
-            psid[P_XP] = s_sid[b][yy][xx + 1]; psv[P_XP] = s_sv[b][yy][xx + 1];
-            pcnt[P_XM] = s_cnt[b][yy][xx - 1]; pl[P_XM] = s_liq[b][yy][xx - 1];
-            psid[P_XM] = s_sid[b][yy][xx - 1]; psv[P_XM] = s_sv[b][yy][xx - 1];
-            pcnt[P_YP] = s_cnt[b][yy + 1][xx]; pl[P_YP] = s_liq[b][yy + 1][xx];
-            psid[P_YP] = s_sid[b][yy + 1][xx]; psv[P_YP] = s_sv[b][yy + 1][xx];
-            pcnt[P_YM] = s_cnt[b][yy - 1][xx]; pl[P_YM] = s_liq[b][yy - 1][xx];
-            psid[P_YM] = s_sid[b][yy - 1][xx]; psv[P_YM] = s_sv[b][yy - 1][xx];
-            if constexpr (DIM == 3) {
-                pcnt[P_ZP] = cn.cnt; pl[P_ZP] = cn.liq; psid[P_ZP] = cn.sid; psv[P_ZP] = cn.sv;
-                pcnt[P_ZM] = cp.cnt; pl[P_ZM] = cp.liq; psid[P_ZM] = cp.sid; psv[P_ZM] = cp.sv;
-            }
-            int sum = 0;
-            bool single = true;
+    for (int kb = k0; kb < k1; kb += 4) {
 #pragma unroll
-            for (int p = 0; p < NP; ++p) {
-                sum += pcnt[p];
-                single = single && (pcnt[p] == 1) && (psid[p] == psid[0]) && (pl[p] == Real(0.0));
+        for (int j = 0; j < 4; ++j) {
+            const int k = kb + j;
+            if (k >= k1) break;
+            const int b = j & 1;
+            ColCell &cp = ring[j & 3], &cc = ring[(j + 1) & 3], &cn = ring[(j + 2) & 3];
+            ColCell &hx = hr[j & 1], &hy = yr[j & 1];
+            // loads of the next plane (the last plane reloads itself; unused)
+            if (DIM == 3) {
+                const int kn = min(k + 1, k1 - 1);
+                ring[(j + 3) & 3] = own_cell(bc_mp(kn, K.nz));
+                hr[(j + 1) & 1] = halo_x(kn);
+                yr[(j + 1) & 1] = halo_y(kn);
+                if (TD) tr[(j + 1) & 1] = at(a.T_in, own_xy + (Off)kn * plane);
             }
-            grains = sum > 0;
-            if (sum == 0) {
-                // only liquid in the stencil: active set {L} or empty
-                bool act = apt_active<NP>(pl);
-                Real ln, sn;
-                single_phase_update(pl[P_C], act, true, K.dt, ln, sn);
-                a.out.liq[c] = ln;
-                a.out.cnt[c] = 0;
-                if (act) s_any = 1;
-            } else if (single) {
-                // one grain, no liquid: active set {g} or empty
-                bool act = apt_active<NP>(psv);
-                Real ln, sn;
-                single_phase_update(psv[P_C], act, false, K.dt, ln, sn);
-                a.out.liq[c] = ln;
-                if (sn != Real(0.0)) {
-                    a.out.sid[c] = (uint16_t)psid[P_C];
-                    a.out.sval[c] = sn;
-                    a.out.cnt[c] = 1;
-                } else {
-                    a.out.cnt[c] = 0;
+            finish_cell(cn);
+            finish_cell(hx);
+            finish_cell(hy);
+            if (valid) {
+                s_key[b][ly + 1][lx + 1] = (uint32_t)cc.cnt | ((uint32_t)cc.sid << 8);
+                s_liq[b][ly + 1][lx + 1] = cc.liq;
+                s_sv[b][ly + 1][lx + 1] = cc.sv;
+            }
+            if (hxl >= 0) {
+                s_key[b][ly + 1][hxs] = (uint32_t)hx.cnt | ((uint32_t)hx.sid << 8);
+                s_liq[b][ly + 1][hxs] = hx.liq;
+                s_sv[b][ly + 1][hxs] = hx.sv;
+            }
+            if (hym >= 0) {
+                s_key[b][hys][lx + 1] = (uint32_t)hy.cnt | ((uint32_t)hy.sid << 8);
+                s_liq[b][hys][lx + 1] = hy.liq;
+                s_sv[b][hys][lx + 1] = hy.sv;
+            }
+            __syncthreads();
+            bool queue = false, grains = false;
+            if (valid) {
+                const Off c = own_xy + (Off)k * plane;
+                if (TD) {
+                    const double Tc = tr[j & 1];
+                    at(a.T_out, c) = a.temp_f32 ? (double)((float)Tc - (float)a.cooling) : Tc - a.cooling;
                 }
-                if (act) s_any = 1;
-            } else {
-                queue = true;
+                // stencil: c, x+, x-, y+, y-, z+, z-
+                uint32_t pkey[NP];
+                Real pl[NP];
+                Real psv[NP];
+                const int yy = ly + 1, xx = lx + 1;
+                pkey[P_C] = (uint32_t)cc.cnt | ((uint32_t)cc.sid << 8); pl[P_C] = cc.liq; psv[P_C] = cc.sv;
+                pkey[P_XP] = s_key[b][yy][xx + 1]; pl[P_XP] = s_liq[b][yy][xx + 1]; psv[P_XP] = s_sv[b][yy][xx + 1];
+                pkey[P_XM] = s_key[b][yy][xx - 1]; pl[P_XM] = s_liq[b][yy][xx - 1]; psv[P_XM] = s_sv[b][yy][xx - 1];
+                pkey[P_YP] = s_key[b][yy + 1][xx]; pl[P_YP] = s_liq[b][yy + 1][xx]; psv[P_YP] = s_sv[b][yy + 1][xx];
+                pkey[P_YM] = s_key[b][yy - 1][xx]; pl[P_YM] = s_liq[b][yy - 1][xx]; psv[P_YM] = s_sv[b][yy - 1][xx];
+                if constexpr (DIM == 3) {
+                    pkey[P_ZP] = (uint32_t)cn.cnt | ((uint32_t)cn.sid << 8); pl[P_ZP] = cn.liq; psv[P_ZP] = cn.sv;
+                    pkey[P_ZM] = (uint32_t)cp.cnt | ((uint32_t)cp.sid << 8); pl[P_ZM] = cp.liq; psv[P_ZM] = cp.sv;
+                }
+                // one grain everywhere: every key equals the centre's, with count 1
+                int sum = 0;
+                bool single = (pkey[P_C] & 0xffu) == 1u;
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    sum += (int)(pkey[p] & 0xffu);
+                    single = single && (pkey[p] == pkey[P_C]) && (pl[p] == Real(0.0));
+                }
+                grains = sum > 0;
+                if (sum == 0) {
+                    // only liquid in the stencil: active set {L} or empty
+                    bool act = apt_active<NP>(pl);
+                    Real ln, sn;
+                    single_phase_update(pl[P_C], act, true, K.dt, ln, sn);
+                    at(a.out.liq, c) = ln;
+                    at(a.out.cnt, c) = 0;
+                    if (act) s_any = 1;
+                } else if (single) {
+                    // one grain, no liquid: active set {g} or empty
+                    bool act = apt_active<NP>(psv);
+                    Real ln, sn;
+                    single_phase_update(psv[P_C], act, false, K.dt, ln, sn);
+                    at(a.out.liq, c) = ln;
+                    if (sn != Real(0.0)) {
+                        at(a.out.sid, c) = (uint16_t)cc.sid;
+                        at(a.out.sval, c) = sn;
+                        at(a.out.cnt, c) = (uint8_t)1;
+                    } else {
+                        at(a.out.cnt, c) = (uint8_t)0;
+                    }
+                    if (act) s_any = 1;
+                } else {
+                    queue = true;
+                }
+            }
+            // an output cell can only hold a grain where its stencil held one
+            if (__ballot(grains) != 0ull) occ_bits |= 1u << (k - k0);
+            // append queued cells to the block's LDS list
+            unsigned long long ball = __ballot(queue);
+            if (ball != 0ull) {
+                int lane = threadIdx.x & 63;
+                int leader = __ffsll((long long)ball) - 1;
+                int base = 0;
+                if (lane == leader) base = atomicAdd(&s_qn, __popcll(ball));
+                base = __shfl(base, leader, 64);
+                if (queue)
+                    s_q[base + __popcll(ball & ((1ull << lane) - 1ull))] =
+                        (uint16_t)(((k - k0) * LTY + ly) * LTX + lx);
             }
         }
-        // an output cell can only hold a grain where its stencil held one
-        if (__ballot(grains) != 0ull) occ_bits |= 1u << (k - k0);
-        // append queued cells to the block's LDS list
-        unsigned long long ball = __ballot(queue);
-        if (ball != 0ull) {
-            int lane = threadIdx.x & 63;
-            int leader = __ffsll((long long)ball) - 1;
-            int base = 0;
-            if (lane == leader) base = atomicAdd(&s_qn, __popcll(ball));
-            base = __shfl(base, leader, 64);
-            if (queue)
-                s_q[base + __popcll(ball & ((1ull << lane) - 1ull))] =
-                    (uint16_t)(((k - k0) * LTY + ly) * LTX + lx);
-        }
-        cp = cc;
-        cc = cn;
-        cn = cnn;
-        hx = nhx;
-        hy = nhy;
-        Tc = Tn;
     }
     if ((threadIdx.x & 63) == 0 && occ_bits) atomicOr(&s_occ, occ_bits);
     __syncthreads();
+    __syncthreads();
+    const int nq = s_qn;
     if (threadIdx.x == 0) {
         if (s_any && *((volatile int *)&a.stats->max_active) < 1) atomicMax(&a.stats->max_active, 1);
-        s_qbase = (s_qn > 0) ? atomicAdd(wl_counter(a, WL_MERGE, a.step), s_qn) : 0;
+        s_qbase = nq > 0 ? atomicAdd(wl_counter(a, WL_MERGE, a.step), nq) : 0;
         a.occ_out[tile_of(tx, ty, tz)] = (uint16_t)s_occ;
     }
     __syncthreads();
-    const int nq = s_qn;
-    int *dst = a.wl[WL_MERGE] + s_qbase;
+    int *dst = wl_list(a, WL_MERGE) + s_qbase;
     for (int i = threadIdx.x; i < nq; i += LTX * LTY) {
-        int q = s_q[i];
-        int qx = q % LTX, qy = (q / LTX) % LTY, qz = q / (LTX * LTY);
+        const int q = s_q[i];
+        const int qx = q % LTX, qy = (q / LTX) % LTY, qz = q / (LTX * LTY);
         dst[i] = (int)cidx(l0 + qx, m0 + qy, k0 + qz, K.nx, K.ny);
     }
 }
@@ -380,14 +399,15 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
     // list and chunk.  s_off is double-buffered on the chunk parity: a wave
     // can only overwrite a buffer after every wave passed the barrier of the
     // chunk in between, by which time that buffer has been read.
-    __shared__ int s_off[2][2][4];
-    __shared__ int s_base[2][2];
+    constexpr int NQ = 1 + NFB; // pushed lists: WL_GENERAL, then the front buckets
+    __shared__ int s_off[2][NQ][4];
+    __shared__ int s_base[2][NQ];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int par = 0;
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x, par ^= 1) {
         int i = base + threadIdx.x;
         bool valid = i < n;
-        size_t c = valid ? (size_t)a.wl[WL_MERGE][i] : 0;
+        size_t c = valid ? (size_t)wl_list(a, WL_MERGE)[i] : 0;
         int l = (int)(c % K.nx);
         int m = (int)((c / K.nx) % K.ny);
         int k = (int)(c / ((size_t)K.nx * K.ny));
@@ -396,8 +416,9 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
         bool fits = true, front = false;
         if (valid) {
             gather<Real, DIM, FAST_A>(K, a.in, ncell, l, m, k, C, &ntot, &fits);
-            note_active(a.stats, ntot, K.kmax, a.step, a.halt_mode);
             front = fits && need_terms && C.liq && C.nsol > 0;
+            // interface cells are counted by the interface pass
+            if (!front) note_active(a.stats, ntot, K.kmax, a.step, a.halt_mode);
             if (fits && !front) {
                 size_t ti = a.temp_dense ? c : (size_t)k;
                 SlotLiq<Real> L{a.in.liq, K.nx, K.ny};
@@ -405,14 +426,17 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
                 write_cell<Real, DIM, FAST_A>(a.out, ncell, c, C);
             }
         }
-        const bool qf = valid && front, qg = valid && !fits;
-        const unsigned long long bf = __ballot(qf), bg = __ballot(qg);
-        if (lane == 0) {
-            s_off[par][0][wave] = __popcll(bf);
-            s_off[par][1][wave] = __popcll(bg);
+        // q = 0: the many-phase list; q = 1..NFB: interface cells with q solids
+        const int myq = !valid ? -1 : (!fits ? 0 : (front ? min(C.nsol, NFB) : -1));
+        unsigned long long bq = 0ull;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const unsigned long long b = __ballot(myq == q);
+            if (q == myq) bq = b;
+            if (lane == 0) s_off[par][q][wave] = __popcll(b);
         }
         __syncthreads();
-        if (threadIdx.x < 2) {
+        if (threadIdx.x < NQ) {
             int *o = s_off[par][threadIdx.x];
             int tot = 0;
             for (int w = 0; w < 4; ++w) {
@@ -420,13 +444,15 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
                 o[w] = tot;
                 tot += v;
             }
-            const int list = threadIdx.x == 0 ? WL_FRONT : WL_GENERAL;
+            const int list = threadIdx.x == 0 ? WL_GENERAL : WL_FRONT + threadIdx.x - 1;
             s_base[par][threadIdx.x] = tot ? atomicAdd(wl_counter(a, list, a.step), tot) : 0;
         }
         __syncthreads();
-        const unsigned long long below = (1ull << lane) - 1ull;
-        if (qf) a.wl[WL_FRONT][s_base[par][0] + s_off[par][0][wave] + __popcll(bf & below)] = (int)c;
-        if (qg) a.wl[WL_GENERAL][s_base[par][1] + s_off[par][1][wave] + __popcll(bg & below)] = (int)c;
+        if (myq >= 0) {
+            const int list = myq == 0 ? WL_GENERAL : WL_FRONT + myq - 1;
+            const unsigned long long below = (1ull << lane) - 1ull;
+            wl_list(a, list)[s_base[par][myq] + s_off[par][myq][wave] + __popcll(bq & below)] = (int)c;
+        }
     }
 }
 
@@ -437,11 +463,26 @@ __global__ __launch_bounds__(128) void k_front(StepArgs<Real> a)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
     const Consts<Real> &K = a.K;
-    const int n = *wl_counter(a, WL_FRONT, a.step);
+    // the buckets are processed as one sequence, fewest solids first
+    int nb[NFB];
+    int n = 0;
+#pragma unroll
+    for (int b = 0; b < NFB; ++b) {
+        nb[b] = *wl_counter(a, WL_FRONT + b, a.step);
+        n += nb[b];
+    }
     const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
     if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->front_cells, (unsigned long long)n);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        size_t c = (size_t)a.wl[WL_FRONT][i];
+        int b = 0, off = i;
+#pragma unroll
+        for (int q = 0; q < NFB - 1; ++q) {
+            if (b == q && off >= nb[q]) {
+                off -= nb[q];
+                b = q + 1;
+            }
+        }
+        size_t c = (size_t)wl_list(a, WL_FRONT + b)[off];
         int l = (int)(c % K.nx);
         int m = (int)((c / K.nx) % K.ny);
         int k = (int)(c / ((size_t)K.nx * K.ny));
@@ -449,6 +490,7 @@ __global__ __launch_bounds__(128) void k_front(StepArgs<Real> a)
         int ntot;
         bool fits;
         gather<Real, DIM, FAST_A>(K, a.in, ncell, l, m, k, C, &ntot, &fits);
+        note_active(a.stats, ntot, K.kmax, a.step, a.halt_mode);
         size_t ti = a.temp_dense ? c : (size_t)k;
         SlotLiq<Real> L{a.in.liq, K.nx, K.ny};
         cell_update<Real, DIM, FAST_A, true, TC>(K, L, l, m, k, a.T_out[ti], C);
@@ -468,7 +510,7 @@ __global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
     const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
     if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->slow_cells, (unsigned long long)n);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        size_t c = (size_t)a.wl[WL_GENERAL][i];
+        size_t c = (size_t)wl_list(a, WL_GENERAL)[i];
         int l = (int)(c % K.nx);
         int m = (int)((c / K.nx) % K.ny);
         int k = (int)(c / ((size_t)K.nx * K.ny));
@@ -865,21 +907,56 @@ using namespace mpf::MPF_CAT(impl, MPF_SFX);
 
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
+// One step.  With a side stream, the many-phase kernel runs there,
+// concurrently with the interface kernel (both only read the input state and
+// write disjoint cells); the main stream waits for it before the step ends.
+// tev (optional): 6 timing events, recorded around the kernels: [0] before
+// k_light, [1] after it, [2] after k_merge, [3] after k_front, and [4]/[5]
+// around k_step_general.
 template <int DIM, int TC>
-static void launch_step_kernels(const StepArgs<Real> &a, unsigned grid, const int *grids, hipStream_t st)
+static void launch_step_kernels(const StepArgs<Real> &a, unsigned grid, const int *grids, hipStream_t st,
+                                hipStream_t side, hipEvent_t ev_merged, hipEvent_t ev_general, hipEvent_t *tev)
 {
-    if (a.temp_dense)
-        hipLaunchKernelGGL((k_light<DIM, 1>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+    hipStream_t gs = side ? side : st;
+    if (tev) hipEventRecord(tev[0], st);
+    // 32-bit offsets when every byte offset of the light pass fits (8-byte
+    // temperature per cell is the widest element it indexes per cell)
+    const size_t ncell = (size_t)a.K.nx * a.K.ny * (DIM == 3 ? a.K.nz : 1);
+    const bool off32 = ncell * 8 <= 0xffffffffull;
+    if (a.temp_dense && off32)
+        hipLaunchKernelGGL((k_light<DIM, 1, uint32_t>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+    else if (a.temp_dense)
+        hipLaunchKernelGGL((k_light<DIM, 1, size_t>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+    else if (off32)
+        hipLaunchKernelGGL((k_light<DIM, 0, uint32_t>), dim3(grid), dim3(LTX * LTY), 0, st, a);
     else
-        hipLaunchKernelGGL((k_light<DIM, 0>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+        hipLaunchKernelGGL((k_light<DIM, 0, size_t>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+    if (tev) hipEventRecord(tev[1], st);
     hipLaunchKernelGGL((k_merge<DIM, TC>), dim3(grids[0]), dim3(256), 0, st, a);
-    hipLaunchKernelGGL((k_front<DIM, TC>), dim3(grids[1]), dim3(128), 0, st, a);
-    hipLaunchKernelGGL((k_step_general<DIM, TC>), dim3(grids[2]), dim3(64), 0, st, a);
+    if (tev) hipEventRecord(tev[2], st);
+    if (side) {
+        hipEventRecord(ev_merged, st);
+        hipStreamWaitEvent(side, ev_merged, 0);
+    } else {
+        hipLaunchKernelGGL((k_front<DIM, TC>), dim3(grids[1]), dim3(128), 0, st, a);
+        if (tev) hipEventRecord(tev[3], st);
+    }
+    if (tev) hipEventRecord(tev[4], gs);
+    hipLaunchKernelGGL((k_step_general<DIM, TC>), dim3(grids[2]), dim3(64), 0, gs, a);
+    if (tev) hipEventRecord(tev[5], gs);
+    if (side) {
+        hipEventRecord(ev_general, side);
+        hipLaunchKernelGGL((k_front<DIM, TC>), dim3(grids[1]), dim3(128), 0, st, a);
+        if (tev) hipEventRecord(tev[3], st);
+        hipStreamWaitEvent(st, ev_general, 0);
+    }
 }
 
 extern "C" {
 
-hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const int *grids, hipStream_t st)
+hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const int *grids, hipStream_t st,
+                                     hipStream_t side, hipEvent_t ev_merged, hipEvent_t ev_general,
+                                     hipEvent_t *tev)
 {
     StepArgs<Real> a = *args;
     a.ntx = (a.K.nx + LTX - 1) / LTX;
@@ -888,14 +965,14 @@ hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const 
     const unsigned grid = (unsigned)a.ntx * a.nty * nzc;
     if (dim == 3) {
         if (a.tab_class)
-            launch_step_kernels<3, 1>(a, grid, grids, st);
+            launch_step_kernels<3, 1>(a, grid, grids, st, side, ev_merged, ev_general, tev);
         else
-            launch_step_kernels<3, 0>(a, grid, grids, st);
+            launch_step_kernels<3, 0>(a, grid, grids, st, side, ev_merged, ev_general, tev);
     } else {
         if (a.tab_class)
-            launch_step_kernels<2, 1>(a, grid, grids, st);
+            launch_step_kernels<2, 1>(a, grid, grids, st, side, ev_merged, ev_general, tev);
         else
-            launch_step_kernels<2, 0>(a, grid, grids, st);
+            launch_step_kernels<2, 0>(a, grid, grids, st, side, ev_merged, ev_general, tev);
     }
     return hipGetLastError();
 }

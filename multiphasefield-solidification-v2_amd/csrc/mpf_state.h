@@ -25,7 +25,14 @@ namespace mpf {
 constexpr int KCAP = 64;        // largest KMAX supported
 constexpr uint32_t NO_ID = 0xFFFFu;
 constexpr int NO_HALT = 0x7fffffff; // halt_step value while no step has overflowed
-enum { WL_MERGE = 0, WL_FRONT = 1, WL_GENERAL = 2 };
+// Work lists of a step: cells for the merge pass, cells with more active
+// grains than the register path holds, and solid-liquid interface cells in
+// NFB buckets by their number of active solids (WL_FRONT + n - 1), so that a
+// wavefront of the interface pass evaluates the same number of solid-liquid
+// pairs on every lane.
+constexpr int NFB = 4;
+enum { WL_MERGE = 0, WL_GENERAL = 1, WL_FRONT = 2 };
+constexpr int NWL = WL_FRONT + NFB;
 
 // Tiles of the classification pass: LTX x LTY cells in x/y, LZC z-planes.
 constexpr int LTX = 64, LTY = 8, LZC = 16;
@@ -77,8 +84,8 @@ struct StepArgs {
     // pass finishes cells that need no anisotropy terms and queues
     // solid-liquid interface cells (WL_FRONT) and cells with more than the
     // register capacity of active grains (WL_GENERAL).
-    int *wl[3];      // WL_MERGE, WL_FRONT, WL_GENERAL; ncell entries each
-    int *wl_count;   // [3][2] counters, ping-pong on the step parity
+    int *wl_base;    // NWL lists of ncell entries each, list i at wl_base + i*ncell
+    int *wl_count;   // [NWL][2] counters, ping-pong on the step parity
     int step;        // step index (for the ping-pong counters and halt)
     int halt_mode;   // stop at the first KMAX overflow
     int tab_class;   // 1: the tables have the class form (K.cls holds the values)

@@ -156,7 +156,13 @@ class Solver:
                 "overflow_cells": s.overflow_cells, "general_cells": s.general_cells,
                 "halt_step": s.halt_step, "steps_done": s.steps_done,
                 "last_step_ms": s.last_step_ms, "merge_cells": s.merge_cells,
-                "front_cells": s.front_cells, "class_tables": bool(s.class_tables)}
+                "front_cells": s.front_cells, "class_tables": bool(s.class_tables),
+                "kernel_steps": s.kernel_steps,
+                "kernel_ms": dict(zip(("light", "merge", "front", "general"), list(s.kernel_ms)))}
+
+    def set_kernel_timing(self, on: bool = True):
+        """Time each kernel of every step with HIP events (summed in stats()["kernel_ms"])."""
+        check(_lib.lib().mpf_set_kernel_timing(self._h, int(bool(on))))
 
     def reset_stats(self):
         check(_lib.lib().mpf_reset_stats(self._h))
