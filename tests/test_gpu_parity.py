@@ -262,3 +262,20 @@ def test_fp64_general_and_class_tables_bit_exact():
         s.step(steps)
         ref, _, _, _ = O.run(c["phi"], c["temp"], steps, tabs, c["scalars"], kmax=50, temp_f32=True)
         assert np.array_equal(s.phi(), ref)
+
+
+@pytest.mark.parametrize("shape", [(10, 8, 14), (12, 17, 14), (68, 18, 20)])
+def test_fp64_classification_kernels_bit_exact(shape):
+    """Both classification kernels: four cells per thread (nx % 4 == 0 with
+    distinct halo threads) and one cell per thread (the other shapes), on
+    partial tiles in x and y."""
+    nx, ny, nz = shape
+    c = cases.voronoi_case(nx=nx, ny=ny, nz=nz, n_solid=5, seed=5, qseed=6, seed_height=5)
+    steps = 6
+    ref, _, _, st = O.run(c["phi"], c["temp"], steps, c["tables"], c["scalars"], kmax=50, temp_f32=True)
+    with _solver_for(c, "fp64", temp_f32=True) as s:
+        assert s.step(steps) == steps
+        got = s.phi()
+        print(shape, "max |diff| =", np.abs(got - ref).max())
+        assert np.array_equal(got, ref)
+        _assert_apt_equal(s, ref)
