@@ -127,6 +127,29 @@ int mpf_download_apt(mpf_handle *h, int32_t *mf, int depth, int32_t *nf, int32_t
 
 int mpf_get_stats(mpf_handle *h, mpf_stats *st);
 int mpf_reset_stats(mpf_handle *h);
+/* ---- batches of independent 3D grids (no counterpart in the reference's
+ * kernels; replaces the per-case loops of the bicrystal and orientation
+ * sweeps, run_twomode_3d.py:200-322 and run_orientation_scan_3d.py:495-841,
+ * which step one grid at a time).  The grids share the configuration's
+ * physics, N and KMAX; each has its own size, tables and temperature.
+ * mpf_step advances all of them together, one launch per kernel.  The
+ * single-grid entry points above that move grid data (upload, download,
+ * observables) refuse a batch handle; mpf_set_tables sets every grid.
+ * dims: [n_inst][3] = nx, ny, nz.  cfg->nx/ny/nz are ignored; cfg->dim must
+ * be 3, temp_dense and halt_on_kmax 0. */
+int mpf_batch_create(const mpf_config *cfg, int n_inst, const int32_t *dims, mpf_handle **out);
+int mpf_batch_size(mpf_handle *h); /* number of grids (1 for a single-grid handle) */
+int mpf_batch_set_tables(mpf_handle *h, int inst, const double *wij, const double *aij, const double *mij,
+                         const double *n111);
+int mpf_batch_upload_phi(mpf_handle *h, int inst, const void *phi, int dtype);
+int mpf_batch_init_columns(mpf_handle *h, int inst, const int32_t *grain_map, const int32_t *profile,
+                           int nprofiles, const double *solid, const double *liquid);
+int mpf_batch_upload_temp(mpf_handle *h, int inst, const double *temp); /* [nz] of that grid */
+int mpf_batch_download_temp(mpf_handle *h, int inst, double *temp);
+int mpf_batch_download_phi(mpf_handle *h, int inst, void *phi, int dtype);
+int mpf_batch_download_apt(mpf_handle *h, int inst, int32_t *mf, int depth, int32_t *nf, int32_t *status);
+int mpf_batch_observables(mpf_handle *h, int inst, double *grain_volume, int32_t *front, double *out);
+
 /* Time every kernel of every step with HIP events (on: 1) and sum the times
  * into mpf_stats.kernel_ms.  Adds a few microseconds per step.  No
  * counterpart in the reference (it has no timing). */

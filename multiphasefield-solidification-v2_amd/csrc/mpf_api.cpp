@@ -75,11 +75,18 @@ struct Buffers {
 struct mpf_handle {
     mpf_config cfg;
     int dim = 3;
-    int nz = 1;
-    size_t ncell = 0;
-    size_t nocc = 0; // occupancy mask words per buffer
+    size_t ncell = 0; // cells over all instances
+    size_t nocc = 0;  // occupancy mask words per buffer
     int scap = 1;
     size_t rsz = 8; // sizeof(Real)
+    int batch = 0;                  // created by mpf_batch_create
+    std::vector<Inst> inst;         // one entry per grid; a single grid is instance 0
+    Inst *inst_d = nullptr;         // device copy (batches)
+    int n_tiles = 0;                // classification tiles over all instances
+    size_t tab_stride = 0;          // table elements per instance (batches)
+    std::vector<int> tab_set;       // per instance: tables uploaded
+    std::vector<int> tab_cls;       // per instance: tables have the class form
+    std::vector<double> cls_vals;   // per instance: 6 class values
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;            // many-phase kernel, concurrent with the interface kernel
     hipEvent_t ev_merged = nullptr, ev_general = nullptr;
@@ -90,7 +97,7 @@ struct mpf_handle {
     Buffers b[2];
     double *T[2] = {nullptr, nullptr};
     size_t nT = 0;
-    void *tab = nullptr; // wij, aij, mij, n111 contiguous
+    void *tab = nullptr; // per instance: wij, aij, mij, n111 contiguous
     int *wl = nullptr;        // NWL work lists of ncell entries (mpf_state.h)
     int *wl_count = nullptr;  // [NWL][2]
     Stats *stats = nullptr;
@@ -98,12 +105,14 @@ struct mpf_handle {
     int step_counter = 0;
     int steps_done = 0;
     int tables_set = 0;
-    int tab_class = 0;    // the tables have the class form (class_tables)
+    int tab_class = 0;    // every instance's tables have the same class form
     double cls[3][2] = {}; // its values, exact in the solver precision
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     size_t bytes = 0;
 };
+
+static size_t inst_cells(const Inst &d) { return (size_t)d.nx * d.ny * d.nz; }
 
 template <typename R>
 static SlotView<R> view(mpf_handle *h, int i)
@@ -113,6 +122,21 @@ static SlotView<R> view(mpf_handle *h, int i)
     v.cnt = h->b[i].cnt;
     v.sid = h->b[i].sid;
     v.sval = (R *)h->b[i].sval;
+    v.stride = h->ncell;
+    return v;
+}
+
+// The buffer seen by one instance: its first cell at index 0, slot stride
+// of the whole buffer.
+template <typename R>
+static SlotView<R> iview(mpf_handle *h, int i, int ii)
+{
+    SlotView<R> v = view<R>(h, i);
+    const size_t o = h->inst[ii].cell_off;
+    v.liq += o;
+    v.cnt += o;
+    v.sid += o;
+    v.sval += o;
     return v;
 }
 
@@ -144,6 +168,7 @@ extern "C" void mpf_destroy(mpf_handle *h)
     hipFree(h->wl);
     hipFree(h->wl_count);
     hipFree(h->stats);
+    hipFree(h->inst_d);
     if (h->side) hipStreamSynchronize(h->side);
     for (hipEvent_t e : h->tev) hipEventDestroy(e);
     if (h->ev_merged) hipEventDestroy(h->ev_merged);
@@ -166,29 +191,56 @@ static int reset_stats_dev(mpf_handle *h)
     return 0;
 }
 
-extern "C" int mpf_create(const mpf_config *cfg, mpf_handle **out)
+// Common creation of a single grid (batch = 0, one instance with the
+// configuration's grid) and of a batch of 3D grids.
+static int create_handle(const mpf_config &c, const std::vector<Inst> &grids, int batch, mpf_handle **out)
 {
-    if (!cfg || !out) return fail(MPF_EINVAL, "mpf_create: null argument");
-    *out = nullptr;
-    const mpf_config &c = *cfg;
     if (c.dim != 2 && c.dim != 3) return fail(MPF_EINVAL, "dim must be 2 or 3 (got %d)", c.dim);
-    if (c.nx < 1 || c.ny < 1 || c.nz < 1) return fail(MPF_EINVAL, "grid must be positive");
-    if (c.dim == 2 && c.nz != 1) return fail(MPF_EINVAL, "2D grids need nz == 1");
     if (c.n_phase < 1 || c.n_phase > 65535) return fail(MPF_EINVAL, "n_phase must be in [1, 65535]");
     if (c.kmax < 1 || c.kmax > KCAP) return fail(MPF_EINVAL, "kmax must be in [1, %d]", KCAP);
     if (c.precision != MPF_FP32 && c.precision != MPF_FP64) return fail(MPF_EINVAL, "bad precision");
-    size_t ncell = (size_t)c.nx * c.ny * c.nz;
-    if (ncell > (size_t)0x7fffffff) return fail(MPF_EINVAL, "grid has more than 2^31-1 cells");
     if (c.dim == 2 && c.n_phase < 2) return fail(MPF_EINVAL, "the 2D update needs n_phase >= 2");
-
+    if (grids.empty()) return fail(MPF_EINVAL, "no grid");
+    if (batch) {
+        if (c.dim != 3) return fail(MPF_EINVAL, "a batch holds 3D grids");
+        if ((int)grids.size() > MAX_INST) return fail(MPF_EINVAL, "a batch holds at most %d grids", MAX_INST);
+        if (c.temp_dense) return fail(MPF_EINVAL, "a batch keeps the temperature per z-plane (temp_dense = 0)");
+        if (c.halt_on_kmax) return fail(MPF_EINVAL, "halt_on_kmax is not available for a batch");
+    }
+    size_t ncell = 0, ntile = 0, nplane = 0;
+    std::vector<Inst> inst = grids;
+    for (size_t i = 0; i < inst.size(); ++i) {
+        Inst &d = inst[i];
+        if (d.nx < 1 || d.ny < 1 || d.nz < 1) return fail(MPF_EINVAL, "grid %zu: sizes must be positive", i);
+        if (c.dim == 2 && d.nz != 1) return fail(MPF_EINVAL, "2D grids need nz == 1");
+        d.pad = 0;
+        d.cell_off = (uint32_t)ncell;
+        d.tile_off = (uint32_t)ntile;
+        d.occ_off = (uint32_t)ntile;
+        d.t_off = (uint32_t)nplane;
+        ncell += inst_cells(d);
+        ntile += occ_words(d.nx, d.ny, d.nz, c.dim);
+        nplane += (size_t)d.nz;
+        if (ncell > (size_t)0x7fffffff) return fail(MPF_EINVAL, "more than 2^31-1 cells in total");
+    }
     mpf_handle *h = new (std::nothrow) mpf_handle();
     if (!h) return fail(MPF_ENOMEM, "host allocation failed");
     h->cfg = c;
+    h->cfg.nx = inst[0].nx;
+    h->cfg.ny = inst[0].ny;
+    h->cfg.nz = inst[0].nz;
     h->dim = c.dim;
-    h->nz = c.nz;
     h->ncell = ncell;
+    h->batch = batch;
+    h->inst = inst;
+    h->n_tiles = (int)ntile;
+    h->nocc = ntile;
     h->rsz = c.precision == MPF_FP64 ? 8 : 4;
-    h->nocc = occ_words(c.nx, c.ny, c.nz, c.dim);
+    const size_t nn = (size_t)c.n_phase * c.n_phase;
+    h->tab_stride = 3 * nn + (size_t)c.n_phase * 24;
+    h->tab_set.assign(inst.size(), 0);
+    h->tab_cls.assign(inst.size(), 0);
+    h->cls_vals.assign(inst.size() * 6, 0.0);
     int cap = c.kmax < c.n_phase - 1 ? c.kmax : c.n_phase - 1;
     if (c.slot_capacity > 0) {
         // a step may store up to min(KMAX, N-1) grains per cell
@@ -223,13 +275,20 @@ extern "C" int mpf_create(const mpf_config *cfg, mpf_handle **out)
         TRY_RC(dev_alloc(h, &h->b[i].sval, ncell * h->scap * h->rsz));
         TRY_RC(dev_alloc(h, (void **)&h->b[i].occ, h->nocc * sizeof(uint16_t)));
     }
-    h->nT = c.temp_dense ? ncell : (size_t)c.nz;
+    h->nT = c.temp_dense ? ncell : nplane;
     for (int i = 0; i < 2; ++i) TRY_RC(dev_alloc(h, (void **)&h->T[i], h->nT * sizeof(double)));
-    size_t nn = (size_t)c.n_phase * c.n_phase;
-    TRY_RC(dev_alloc(h, &h->tab, (3 * nn + (size_t)c.n_phase * 24) * h->rsz));
+    TRY_RC(dev_alloc(h, &h->tab, inst.size() * h->tab_stride * h->rsz));
     TRY_RC(dev_alloc(h, (void **)&h->wl, (size_t)NWL * ncell * sizeof(int)));
     TRY_RC(dev_alloc(h, (void **)&h->wl_count, 2 * NWL * sizeof(int)));
     TRY_RC(dev_alloc(h, (void **)&h->stats, sizeof(Stats)));
+    if (batch) {
+        TRY_RC(dev_alloc(h, (void **)&h->inst_d, inst.size() * sizeof(Inst)));
+        e = hipMemcpy(h->inst_d, inst.data(), inst.size() * sizeof(Inst), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            mpf_destroy(h);
+            return fail(MPF_EHIP, "hipMemcpy: %s", hipGetErrorString(e));
+        }
+    }
     // empty state: all liquid, no grains
     for (int i = 0; i < 2; ++i) {
         e = hipMemsetAsync(h->b[i].cnt, 0, ncell, h->stream);
@@ -254,42 +313,111 @@ extern "C" int mpf_create(const mpf_config *cfg, mpf_handle **out)
     return 0;
 }
 
+extern "C" int mpf_create(const mpf_config *cfg, mpf_handle **out)
+{
+    if (!cfg || !out) return fail(MPF_EINVAL, "mpf_create: null argument");
+    *out = nullptr;
+    Inst d{};
+    d.nx = cfg->nx;
+    d.ny = cfg->ny;
+    d.nz = cfg->nz;
+    return create_handle(*cfg, std::vector<Inst>(1, d), 0, out);
+}
+
+extern "C" int mpf_batch_create(const mpf_config *cfg, int n_inst, const int32_t *dims, mpf_handle **out)
+{
+    if (!cfg || !dims || !out || n_inst < 1) return fail(MPF_EINVAL, "mpf_batch_create: bad argument");
+    *out = nullptr;
+    std::vector<Inst> g((size_t)n_inst);
+    for (int i = 0; i < n_inst; ++i) {
+        g[i] = Inst{};
+        g[i].nx = dims[3 * i];
+        g[i].ny = dims[3 * i + 1];
+        g[i].nz = dims[3 * i + 2];
+    }
+    return create_handle(*cfg, g, 1, out);
+}
+
+extern "C" int mpf_batch_size(mpf_handle *h) { return h ? (int)h->inst.size() : 0; }
+
 extern "C" size_t mpf_device_bytes(mpf_handle *h) { return h ? h->bytes : 0; }
 
-extern "C" int mpf_set_tables(mpf_handle *h, const double *wij, const double *aij, const double *mij,
-                              const double *n111)
+static int check_inst(mpf_handle *h, int ii, const char *fn)
 {
-    if (!h || !wij || !aij || !mij || !n111) return fail(MPF_EINVAL, "mpf_set_tables: null argument");
-    int N = h->cfg.n_phase;
-    size_t nn = (size_t)N * N, tot = 3 * nn + (size_t)N * 24;
+    if (!h) return fail(MPF_EINVAL, "%s: null handle", fn);
+    if (ii < 0 || ii >= (int)h->inst.size()) return fail(MPF_EINVAL, "%s: no instance %d", fn, ii);
+    return 0;
+}
+
+// Tables of one instance, converted to the solver precision; the class form
+// is checked per instance and used only when every instance has it with the
+// same values.
+static int set_tables_inst(mpf_handle *h, int ii, const double *wij, const double *aij, const double *mij,
+                           const double *n111)
+{
+    const int N = h->cfg.n_phase;
+    const size_t nn = (size_t)N * N, tot = h->tab_stride;
     std::vector<double> all(tot);
     memcpy(all.data(), wij, nn * sizeof(double));
     memcpy(all.data() + nn, aij, nn * sizeof(double));
     memcpy(all.data() + 2 * nn, mij, nn * sizeof(double));
     memcpy(all.data() + 3 * nn, n111, (size_t)N * 24 * sizeof(double));
+    char *dst = (char *)h->tab + (size_t)ii * tot * h->rsz;
+    int cls_ok = 0;
+    double cv[6];
     if (h->rsz == 8) {
-        HIP_TRY(hipMemcpy(h->tab, all.data(), tot * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dst, all.data(), tot * sizeof(double), hipMemcpyHostToDevice));
         double cls[3][2];
-        h->tab_class = class_tables<double>(all.data(), N, cls);
-        for (int t = 0; t < 3; ++t) h->cls[t][0] = cls[t][0], h->cls[t][1] = cls[t][1];
+        cls_ok = class_tables<double>(all.data(), N, cls);
+        for (int t = 0; t < 6; ++t) cv[t] = cls[t / 2][t % 2];
     } else {
         std::vector<float> f(tot);
         for (size_t i = 0; i < tot; ++i) f[i] = (float)all[i];
-        HIP_TRY(hipMemcpy(h->tab, f.data(), tot * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dst, f.data(), tot * sizeof(float), hipMemcpyHostToDevice));
         float cls[3][2];
-        h->tab_class = class_tables<float>(f.data(), N, cls);
-        for (int t = 0; t < 3; ++t) h->cls[t][0] = cls[t][0], h->cls[t][1] = cls[t][1];
+        cls_ok = class_tables<float>(f.data(), N, cls);
+        for (int t = 0; t < 6; ++t) cv[t] = cls[t / 2][t % 2];
     }
-    h->tables_set = 1;
+    h->tab_set[ii] = 1;
+    h->tab_cls[ii] = cls_ok;
+    for (int t = 0; t < 6; ++t) h->cls_vals[(size_t)ii * 6 + t] = cv[t];
+    int all_set = 1, all_cls = 1;
+    for (size_t i = 0; i < h->inst.size(); ++i) {
+        all_set &= h->tab_set[i];
+        all_cls &= h->tab_set[i] && h->tab_cls[i] &&
+                   memcmp(&h->cls_vals[i * 6], &h->cls_vals[0], 6 * sizeof(double)) == 0;
+    }
+    h->tables_set = all_set;
+    h->tab_class = all_cls;
+    for (int t = 0; t < 6; ++t) h->cls[t / 2][t % 2] = h->cls_vals[t];
     return 0;
 }
 
-extern "C" int mpf_upload_phi(mpf_handle *h, const void *phi, int dtype)
+extern "C" int mpf_set_tables(mpf_handle *h, const double *wij, const double *aij, const double *mij,
+                              const double *n111)
 {
-    if (!h || !phi) return fail(MPF_EINVAL, "mpf_upload_phi: null argument");
+    if (!h || !wij || !aij || !mij || !n111) return fail(MPF_EINVAL, "mpf_set_tables: null argument");
+    for (int ii = 0; ii < (int)h->inst.size(); ++ii) {
+        int rc = set_tables_inst(h, ii, wij, aij, mij, n111);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+extern "C" int mpf_batch_set_tables(mpf_handle *h, int ii, const double *wij, const double *aij,
+                                    const double *mij, const double *n111)
+{
+    if (!wij || !aij || !mij || !n111) return fail(MPF_EINVAL, "mpf_batch_set_tables: null argument");
+    if (int rc = check_inst(h, ii, "mpf_batch_set_tables")) return rc;
+    return set_tables_inst(h, ii, wij, aij, mij, n111);
+}
+
+static int upload_phi_inst(mpf_handle *h, int ii, const void *phi, int dtype)
+{
     if (dtype != MPF_FP32 && dtype != MPF_FP64) return fail(MPF_EINVAL, "bad dtype");
+    const Inst &g = h->inst[ii];
     size_t esz = dtype == MPF_FP64 ? 8 : 4;
-    size_t bytes = h->ncell * (size_t)h->cfg.n_phase * esz;
+    size_t bytes = inst_cells(g) * (size_t)h->cfg.n_phase * esz;
     void *d = nullptr;
     HIP_TRY(hipMalloc(&d, bytes));
     int *err = nullptr;
@@ -301,10 +429,10 @@ extern "C" int mpf_upload_phi(mpf_handle *h, const void *phi, int dtype)
     if (e == hipSuccess) {
         const mpf_config &c = h->cfg;
         if (h->rsz == 8)
-            e = mpf_launch_pack_f64(d, dtype == MPF_FP64, c.n_phase, c.nx, c.ny, c.nz, view<double>(h, h->cur),
+            e = mpf_launch_pack_f64(d, dtype == MPF_FP64, c.n_phase, g.nx, g.ny, g.nz, iview<double>(h, h->cur, ii),
                                     h->scap, err, h->stream);
         else
-            e = mpf_launch_pack_f32(d, dtype == MPF_FP64, c.n_phase, c.nx, c.ny, c.nz, view<float>(h, h->cur),
+            e = mpf_launch_pack_f32(d, dtype == MPF_FP64, c.n_phase, g.nx, g.ny, g.nz, iview<float>(h, h->cur, ii),
                                     h->scap, err, h->stream);
     }
     int herr = 0;
@@ -312,50 +440,82 @@ extern "C" int mpf_upload_phi(mpf_handle *h, const void *phi, int dtype)
     if (e == hipSuccess) e = hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost);
     hipFree(d);
     hipFree(err);
-    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_upload_phi: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(MPF_EHIP, "upload_phi: %s", hipGetErrorString(e));
     if (herr) return fail(MPF_ECAPACITY, "a cell holds more than %d non-zero solid phases", h->scap);
+    return 0;
+}
+
+extern "C" int mpf_upload_phi(mpf_handle *h, const void *phi, int dtype)
+{
+    if (!h || !phi) return fail(MPF_EINVAL, "mpf_upload_phi: null argument");
+    if (h->batch) return fail(MPF_EINVAL, "mpf_upload_phi: use mpf_batch_upload_phi for a batch");
+    return upload_phi_inst(h, 0, phi, dtype);
+}
+
+extern "C" int mpf_batch_upload_phi(mpf_handle *h, int ii, const void *phi, int dtype)
+{
+    if (!phi) return fail(MPF_EINVAL, "mpf_batch_upload_phi: null argument");
+    if (int rc = check_inst(h, ii, "mpf_batch_upload_phi")) return rc;
+    return upload_phi_inst(h, ii, phi, dtype);
+}
+
+static int init_columns_inst(mpf_handle *h, int ii, const int32_t *grain_map, const int32_t *profile, int nprof,
+                             const double *solid, const double *liquid)
+{
+    if (!grain_map || !profile || !solid || !liquid || nprof < 1)
+        return fail(MPF_EINVAL, "init_columns: bad argument");
+    const mpf_config &c = h->cfg;
+    const Inst &gd = h->inst[ii];
+    size_t ncol = (size_t)gd.nx * gd.ny;
+    for (size_t i = 0; i < ncol; ++i) {
+        if (profile[i] < 0 || profile[i] >= nprof) return fail(MPF_EINVAL, "profile index out of range");
+        if (grain_map[i] < 0 || grain_map[i] >= c.n_phase) return fail(MPF_EINVAL, "grain id out of range");
+    }
+    void *d = nullptr;
+    size_t bytes = ncol * 2 * sizeof(int32_t) + (size_t)nprof * gd.nz * 2 * sizeof(double);
+    HIP_TRY(hipMalloc(&d, bytes));
+    int32_t *dg = (int32_t *)d, *dp = dg + ncol;
+    double *ds = (double *)(dp + ncol), *dl = ds + (size_t)nprof * gd.nz;
+    hipError_t e = hipMemcpy(dg, grain_map, ncol * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dp, profile, ncol * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ds, solid, (size_t)nprof * gd.nz * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dl, liquid, (size_t)nprof * gd.nz * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(h->b[h->cur].occ, 0xff, h->nocc * sizeof(uint16_t), h->stream);
+    if (e == hipSuccess) {
+        if (h->rsz == 8)
+            e = mpf_launch_init_columns_f64(dg, dp, ds, dl, gd.nx, gd.ny, gd.nz, iview<double>(h, h->cur, ii),
+                                            h->stream);
+        else
+            e = mpf_launch_init_columns_f32(dg, dp, ds, dl, gd.nx, gd.ny, gd.nz, iview<float>(h, h->cur, ii),
+                                            h->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    hipFree(d);
+    if (e != hipSuccess) return fail(MPF_EHIP, "init_columns: %s", hipGetErrorString(e));
     return 0;
 }
 
 extern "C" int mpf_init_columns(mpf_handle *h, const int32_t *grain_map, const int32_t *profile, int nprof,
                                 const double *solid, const double *liquid)
 {
-    if (!h || !grain_map || !profile || !solid || !liquid || nprof < 1)
-        return fail(MPF_EINVAL, "mpf_init_columns: bad argument");
-    const mpf_config &c = h->cfg;
-    size_t ncol = (size_t)c.nx * c.ny;
-    for (size_t i = 0; i < ncol; ++i) {
-        if (profile[i] < 0 || profile[i] >= nprof) return fail(MPF_EINVAL, "profile index out of range");
-        if (grain_map[i] < 0 || grain_map[i] >= c.n_phase) return fail(MPF_EINVAL, "grain id out of range");
-    }
-    void *d = nullptr;
-    size_t bytes = ncol * 2 * sizeof(int32_t) + (size_t)nprof * c.nz * 2 * sizeof(double);
-    HIP_TRY(hipMalloc(&d, bytes));
-    int32_t *dg = (int32_t *)d, *dp = dg + ncol;
-    double *ds = (double *)(dp + ncol), *dl = ds + (size_t)nprof * c.nz;
-    hipError_t e = hipMemcpy(dg, grain_map, ncol * sizeof(int32_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dp, profile, ncol * sizeof(int32_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(ds, solid, (size_t)nprof * c.nz * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dl, liquid, (size_t)nprof * c.nz * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemsetAsync(h->b[h->cur].occ, 0xff, h->nocc * sizeof(uint16_t), h->stream);
-    if (e == hipSuccess) {
-        if (h->rsz == 8)
-            e = mpf_launch_init_columns_f64(dg, dp, ds, dl, c.nx, c.ny, c.nz, view<double>(h, h->cur), h->stream);
-        else
-            e = mpf_launch_init_columns_f32(dg, dp, ds, dl, c.nx, c.ny, c.nz, view<float>(h, h->cur), h->stream);
-    }
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    hipFree(d);
-    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_init_columns: %s", hipGetErrorString(e));
-    return 0;
+    if (!h) return fail(MPF_EINVAL, "mpf_init_columns: null handle");
+    if (h->batch) return fail(MPF_EINVAL, "mpf_init_columns: use mpf_batch_init_columns for a batch");
+    return init_columns_inst(h, 0, grain_map, profile, nprof, solid, liquid);
 }
 
-extern "C" int mpf_upload_temp(mpf_handle *h, const double *temp)
+extern "C" int mpf_batch_init_columns(mpf_handle *h, int ii, const int32_t *grain_map, const int32_t *profile,
+                                      int nprof, const double *solid, const double *liquid)
 {
-    if (!h || !temp) return fail(MPF_EINVAL, "mpf_upload_temp: null argument");
+    if (int rc = check_inst(h, ii, "mpf_batch_init_columns")) return rc;
+    return init_columns_inst(h, ii, grain_map, profile, nprof, solid, liquid);
+}
+
+static int upload_temp_inst(mpf_handle *h, int ii, const double *temp)
+{
     const mpf_config &c = h->cfg;
+    const Inst &g = h->inst[ii];
     if (!c.temp_dense) {
-        HIP_TRY(hipMemcpy(h->T[h->tcur], temp, h->nT * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(h->T[h->tcur] + g.t_off, temp, (size_t)g.nz * sizeof(double), hipMemcpyHostToDevice));
         return 0;
     }
     double *d = nullptr;
@@ -364,17 +524,31 @@ extern "C" int mpf_upload_temp(mpf_handle *h, const double *temp)
     if (e == hipSuccess) e = mpf_launch_temp_layout_f64(d, h->T[h->tcur], c.nx, c.ny, c.nz, 1, 1, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     hipFree(d);
-    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_upload_temp: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(MPF_EHIP, "upload_temp: %s", hipGetErrorString(e));
     return 0;
 }
 
-extern "C" int mpf_download_temp(mpf_handle *h, double *temp)
+extern "C" int mpf_upload_temp(mpf_handle *h, const double *temp)
 {
-    if (!h || !temp) return fail(MPF_EINVAL, "mpf_download_temp: null argument");
+    if (!h || !temp) return fail(MPF_EINVAL, "mpf_upload_temp: null argument");
+    if (h->batch) return fail(MPF_EINVAL, "mpf_upload_temp: use mpf_batch_upload_temp for a batch");
+    return upload_temp_inst(h, 0, temp);
+}
+
+extern "C" int mpf_batch_upload_temp(mpf_handle *h, int ii, const double *temp)
+{
+    if (!temp) return fail(MPF_EINVAL, "mpf_batch_upload_temp: null argument");
+    if (int rc = check_inst(h, ii, "mpf_batch_upload_temp")) return rc;
+    return upload_temp_inst(h, ii, temp);
+}
+
+static int download_temp_inst(mpf_handle *h, int ii, double *temp)
+{
     const mpf_config &c = h->cfg;
+    const Inst &g = h->inst[ii];
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (!c.temp_dense) {
-        HIP_TRY(hipMemcpy(temp, h->T[h->tcur], h->nT * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(temp, h->T[h->tcur] + g.t_off, (size_t)g.nz * sizeof(double), hipMemcpyDeviceToHost));
         return 0;
     }
     double *d = nullptr;
@@ -383,8 +557,22 @@ extern "C" int mpf_download_temp(mpf_handle *h, double *temp)
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess) e = hipMemcpy(temp, d, h->ncell * sizeof(double), hipMemcpyDeviceToHost);
     hipFree(d);
-    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_download_temp: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(MPF_EHIP, "download_temp: %s", hipGetErrorString(e));
     return 0;
+}
+
+extern "C" int mpf_download_temp(mpf_handle *h, double *temp)
+{
+    if (!h || !temp) return fail(MPF_EINVAL, "mpf_download_temp: null argument");
+    if (h->batch) return fail(MPF_EINVAL, "mpf_download_temp: use mpf_batch_download_temp for a batch");
+    return download_temp_inst(h, 0, temp);
+}
+
+extern "C" int mpf_batch_download_temp(mpf_handle *h, int ii, double *temp)
+{
+    if (!temp) return fail(MPF_EINVAL, "mpf_batch_download_temp: null argument");
+    if (int rc = check_inst(h, ii, "mpf_batch_download_temp")) return rc;
+    return download_temp_inst(h, ii, temp);
 }
 
 template <typename R>
@@ -415,6 +603,12 @@ static hipError_t launch_one_step(mpf_handle *h, const int *grids, hipEvent_t *t
         a.K.cls[t][1] = (R)h->cls[t][1];
     }
     a.stats = h->stats;
+    if (h->batch) {
+        a.inst = h->inst_d;
+        a.n_inst = (int)h->inst.size();
+        a.tab_stride = h->tab_stride;
+        a.n_tiles = h->n_tiles;
+    }
     if (sizeof(R) == 8)
         return mpf_launch_step_f64((const StepArgs<double> *)&a, c.dim, grids, h->stream, h->side, h->ev_merged,
                                    h->ev_general, tev);
@@ -493,47 +687,63 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
     return done;
 }
 
-extern "C" int mpf_download_phi(mpf_handle *h, void *phi, int dtype)
+static int download_phi_inst(mpf_handle *h, int ii, void *phi, int dtype)
 {
-    if (!h || !phi) return fail(MPF_EINVAL, "mpf_download_phi: null argument");
     if (dtype != MPF_FP32 && dtype != MPF_FP64) return fail(MPF_EINVAL, "bad dtype");
     const mpf_config &c = h->cfg;
+    const Inst &g = h->inst[ii];
     size_t esz = dtype == MPF_FP64 ? 8 : 4;
-    size_t bytes = h->ncell * (size_t)c.n_phase * esz;
+    size_t bytes = inst_cells(g) * (size_t)c.n_phase * esz;
     void *d = nullptr;
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipMalloc(&d, bytes));
     hipError_t e = hipMemsetAsync(d, 0, bytes, h->stream);
     if (e == hipSuccess) {
         if (h->rsz == 8)
-            e = mpf_launch_unpack_f64(view<double>(h, h->cur), c.nx, c.ny, c.nz, d, dtype == MPF_FP64, h->stream);
+            e = mpf_launch_unpack_f64(iview<double>(h, h->cur, ii), g.nx, g.ny, g.nz, d, dtype == MPF_FP64, h->stream);
         else
-            e = mpf_launch_unpack_f32(view<float>(h, h->cur), c.nx, c.ny, c.nz, d, dtype == MPF_FP64, h->stream);
+            e = mpf_launch_unpack_f32(iview<float>(h, h->cur, ii), g.nx, g.ny, g.nz, d, dtype == MPF_FP64, h->stream);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess) e = hipMemcpy(phi, d, bytes, hipMemcpyDeviceToHost);
     hipFree(d);
-    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_download_phi: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(MPF_EHIP, "download_phi: %s", hipGetErrorString(e));
     return 0;
 }
 
-extern "C" int mpf_download_apt(mpf_handle *h, int32_t *mf, int depth, int32_t *nf, int32_t *status)
+extern "C" int mpf_download_phi(mpf_handle *h, void *phi, int dtype)
 {
-    if (!h || !mf || !nf || !status || depth < 1) return fail(MPF_EINVAL, "mpf_download_apt: bad argument");
+    if (!h || !phi) return fail(MPF_EINVAL, "mpf_download_phi: null argument");
+    if (h->batch) return fail(MPF_EINVAL, "mpf_download_phi: use mpf_batch_download_phi for a batch");
+    return download_phi_inst(h, 0, phi, dtype);
+}
+
+extern "C" int mpf_batch_download_phi(mpf_handle *h, int ii, void *phi, int dtype)
+{
+    if (!phi) return fail(MPF_EINVAL, "mpf_batch_download_phi: null argument");
+    if (int rc = check_inst(h, ii, "mpf_batch_download_phi")) return rc;
+    return download_phi_inst(h, ii, phi, dtype);
+}
+
+static int download_apt_inst(mpf_handle *h, int ii, int32_t *mf, int depth, int32_t *nf, int32_t *status)
+{
+    if (!mf || !nf || !status || depth < 1) return fail(MPF_EINVAL, "download_apt: bad argument");
     const mpf_config &c = h->cfg;
-    size_t bm = (size_t)depth * h->ncell * sizeof(int32_t), bn = h->ncell * sizeof(int32_t);
+    const Inst &g = h->inst[ii];
+    const size_t nc = inst_cells(g);
+    size_t bm = (size_t)depth * nc * sizeof(int32_t), bn = nc * sizeof(int32_t);
     void *d = nullptr;
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipMalloc(&d, bm + bn + 2 * sizeof(int32_t)));
-    int32_t *dmf = (int32_t *)d, *dnf = dmf + (size_t)depth * h->ncell, *dst = dnf + h->ncell;
+    int32_t *dmf = (int32_t *)d, *dnf = dmf + (size_t)depth * nc, *dst = dnf + nc;
     hipError_t e = hipMemsetAsync(d, 0, bm + bn + 2 * sizeof(int32_t), h->stream);
     if (e == hipSuccess) {
         if (h->rsz == 8) {
-            Consts<double> K = make_consts<double>(c.s, c.nx, c.ny, c.nz, c.n_phase, c.kmax, 1, 1, h->tab);
-            e = mpf_launch_apt_slots_f64(&K, view<double>(h, h->cur), c.dim, dmf, dnf, dst, depth, h->stream);
+            Consts<double> K = make_consts<double>(c.s, g.nx, g.ny, g.nz, c.n_phase, c.kmax, 1, 1, h->tab);
+            e = mpf_launch_apt_slots_f64(&K, iview<double>(h, h->cur, ii), c.dim, dmf, dnf, dst, depth, h->stream);
         } else {
-            Consts<float> K = make_consts<float>(c.s, c.nx, c.ny, c.nz, c.n_phase, c.kmax, 1, 1, h->tab);
-            e = mpf_launch_apt_slots_f32(&K, view<float>(h, h->cur), c.dim, dmf, dnf, dst, depth, h->stream);
+            Consts<float> K = make_consts<float>(c.s, g.nx, g.ny, g.nz, c.n_phase, c.kmax, 1, 1, h->tab);
+            e = mpf_launch_apt_slots_f32(&K, iview<float>(h, h->cur, ii), c.dim, dmf, dnf, dst, depth, h->stream);
         }
     }
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -541,8 +751,21 @@ extern "C" int mpf_download_apt(mpf_handle *h, int32_t *mf, int depth, int32_t *
     if (e == hipSuccess) e = hipMemcpy(nf, dnf, bn, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(status, dst, 2 * sizeof(int32_t), hipMemcpyDeviceToHost);
     hipFree(d);
-    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_download_apt: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(MPF_EHIP, "download_apt: %s", hipGetErrorString(e));
     return 0;
+}
+
+extern "C" int mpf_download_apt(mpf_handle *h, int32_t *mf, int depth, int32_t *nf, int32_t *status)
+{
+    if (!h) return fail(MPF_EINVAL, "mpf_download_apt: null handle");
+    if (h->batch) return fail(MPF_EINVAL, "mpf_download_apt: use mpf_batch_download_apt for a batch");
+    return download_apt_inst(h, 0, mf, depth, nf, status);
+}
+
+extern "C" int mpf_batch_download_apt(mpf_handle *h, int ii, int32_t *mf, int depth, int32_t *nf, int32_t *status)
+{
+    if (int rc = check_inst(h, ii, "mpf_batch_download_apt")) return rc;
+    return download_apt_inst(h, ii, mf, depth, nf, status);
 }
 
 extern "C" int mpf_get_stats(mpf_handle *h, mpf_stats *st)
@@ -581,14 +804,15 @@ extern "C" int mpf_reset_stats(mpf_handle *h)
     return reset_stats_dev(h);
 }
 
-extern "C" int mpf_observables(mpf_handle *h, double *grain_volume, int32_t *front, double *out)
+static int observables_inst(mpf_handle *h, int ii, double *grain_volume, int32_t *front, double *out)
 {
-    if (!h || !grain_volume || !front || !out) return fail(MPF_EINVAL, "mpf_observables: null argument");
+    if (!grain_volume || !front || !out) return fail(MPF_EINVAL, "observables: null argument");
     const mpf_config &c = h->cfg;
-    int N = c.n_phase, nz = c.nz;
-    size_t ncol = (size_t)c.nx * c.ny;
+    const Inst &g = h->inst[ii];
+    int N = c.n_phase, nz = g.nz;
+    size_t ncol = (size_t)g.nx * g.ny;
     if (((size_t)N + 256 * 6) * sizeof(double) > 64 * 1024)
-        return fail(MPF_EINVAL, "mpf_observables supports at most %d phases", 64 * 1024 / 8 - 1536);
+        return fail(MPF_EINVAL, "observables support at most %d phases", 64 * 1024 / 8 - 1536);
     size_t bytes = (size_t)N * 8 + (size_t)nz * 6 * 8 + ncol * 4;
     void *d = nullptr;
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -598,9 +822,9 @@ extern "C" int mpf_observables(mpf_handle *h, double *grain_volume, int32_t *fro
     hipError_t e = hipMemsetAsync(d, 0, bytes, h->stream);
     if (e == hipSuccess) {
         if (h->rsz == 8)
-            e = mpf_launch_observe_f64(view<double>(h, h->cur), c.nx, c.ny, nz, N, gv, part, fr, h->stream);
+            e = mpf_launch_observe_f64(iview<double>(h, h->cur, ii), g.nx, g.ny, nz, N, gv, part, fr, h->stream);
         else
-            e = mpf_launch_observe_f32(view<float>(h, h->cur), c.nx, c.ny, nz, N, gv, part, fr, h->stream);
+            e = mpf_launch_observe_f32(iview<float>(h, h->cur, ii), g.nx, g.ny, nz, N, gv, part, fr, h->stream);
     }
     std::vector<double> hp((size_t)nz * 6);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -608,7 +832,7 @@ extern "C" int mpf_observables(mpf_handle *h, double *grain_volume, int32_t *fro
     if (e == hipSuccess) e = hipMemcpy(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(front, fr, ncol * 4, hipMemcpyDeviceToHost);
     hipFree(d);
-    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_observables: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(MPF_EHIP, "observables: %s", hipGetErrorString(e));
     double emax = 0.0, pmin = 1e300, pmax = -1e300, nanc = 0.0, solid = 0.0, slots = 0.0;
     for (int k = 0; k < nz; ++k) {
         const double *q = hp.data() + (size_t)k * 6;
@@ -624,9 +848,22 @@ extern "C" int mpf_observables(mpf_handle *h, double *grain_volume, int32_t *fro
     out[2] = pmax;
     out[3] = nanc;
     out[4] = solid;
-    out[5] = (double)h->ncell;
+    out[5] = (double)inst_cells(g);
     out[6] = slots;
     return 0;
+}
+
+extern "C" int mpf_observables(mpf_handle *h, double *grain_volume, int32_t *front, double *out)
+{
+    if (!h) return fail(MPF_EINVAL, "mpf_observables: null handle");
+    if (h->batch) return fail(MPF_EINVAL, "mpf_observables: use mpf_batch_observables for a batch");
+    return observables_inst(h, 0, grain_volume, front, out);
+}
+
+extern "C" int mpf_batch_observables(mpf_handle *h, int ii, double *grain_volume, int32_t *front, double *out)
+{
+    if (int rc = check_inst(h, ii, "mpf_batch_observables")) return rc;
+    return observables_inst(h, ii, grain_volume, front, out);
 }
 
 // ------------------------------------------------------ drop-in kernels

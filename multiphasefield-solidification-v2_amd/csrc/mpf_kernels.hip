@@ -91,7 +91,90 @@ __device__ __forceinline__ int *wl_counter(const StepArgs<Real> &a, int list, in
 
 __device__ __forceinline__ int *wl_list(const StepArgs<Real> &a, int list)
 {
-    return a.wl_base + (size_t)list * ((size_t)a.K.nx * a.K.ny * a.K.nz);
+    return a.wl_base + (size_t)list * a.in.stride;
+}
+
+// ------------------------------------------------------------ batches
+// In a batch (B = 1) every instance is a grid of its own inside the shared
+// buffers: its cells start at cell_off, its tables at instance * tab_stride,
+// its temperature planes at t_off, its occupancy words at occ_off.  Work
+// lists hold batch-wide cell indices.
+__device__ __forceinline__ SlotView<Real> shift_view(SlotView<Real> v, size_t off)
+{
+    v.liq += off;
+    v.cnt += off;
+    v.sid += off;
+    v.sval += off;
+    return v;
+}
+
+// The arguments as seen by one instance.
+__device__ __forceinline__ void instance_args(StepArgs<Real> &a, const Inst &d, int ii)
+{
+    a.K.nx = d.nx;
+    a.K.ny = d.ny;
+    a.K.nz = d.nz;
+    a.K.tab = a.K.tab + (size_t)ii * a.tab_stride;
+    a.in = shift_view(a.in, d.cell_off);
+    a.out = shift_view(a.out, d.cell_off);
+    a.T_in += d.t_off;
+    a.T_out += d.t_off;
+    a.occ_in += d.occ_off;
+    a.occ_out += d.occ_off;
+}
+
+// Cell offsets of the instances in LDS for the per-cell instance search.
+template <int B>
+__device__ __forceinline__ void load_cell_offsets(const StepArgs<Real> &a, uint32_t *s_coff)
+{
+    if constexpr (B != 0) {
+        for (int i = threadIdx.x; i < a.n_inst; i += blockDim.x) s_coff[i] = a.inst[i].cell_off;
+        __syncthreads();
+    }
+}
+
+// A queued cell resolved to its grid: constants, views, temperature and the
+// cell index and coordinates within the grid.
+struct Where {
+    Consts<Real> K;
+    SlotView<Real> in, out;
+    const double *T;
+    size_t c;
+    int l, m, k;
+};
+
+template <int B>
+__device__ __forceinline__ Where locate(const StepArgs<Real> &a, const uint32_t *s_coff, size_t cg)
+{
+    Where w;
+    w.K = a.K;
+    w.in = a.in;
+    w.out = a.out;
+    w.T = a.T_out;
+    w.c = cg;
+    if constexpr (B != 0) {
+        int lo = 0, hi = a.n_inst - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if ((size_t)s_coff[mid] <= cg)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const Inst d = a.inst[lo];
+        w.K.nx = d.nx;
+        w.K.ny = d.ny;
+        w.K.nz = d.nz;
+        w.K.tab = a.K.tab + (size_t)lo * a.tab_stride;
+        w.in = shift_view(a.in, d.cell_off);
+        w.out = shift_view(a.out, d.cell_off);
+        w.T = a.T_out + d.t_off;
+        w.c = cg - d.cell_off;
+    }
+    w.l = (int)(w.c % w.K.nx);
+    w.m = (int)((w.c / w.K.nx) % w.K.ny);
+    w.k = (int)(w.c / ((size_t)w.K.nx * w.K.ny));
+    return w;
 }
 
 // Pass 1 over every cell: temperature, classification and the cells whose
@@ -151,11 +234,31 @@ __device__ __forceinline__ void finish_cell(ColCell &r)
     r.sv = e ? Real(0.0) : r.sv;
 }
 
-template <int DIM, int TD, typename Off>
-__global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
+template <int DIM, int TD, typename Off, int B>
+__global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
 {
-    if (a.halt_mode && a.stats->halt_step < a.step) return;
-    if (blockIdx.x == 0 && threadIdx.x < NWL) a.wl_count[threadIdx.x * 2 + ((a.step + 1) & 1)] = 0;
+    if (a0.halt_mode && a0.stats->halt_step < a0.step) return;
+    if (blockIdx.x == 0 && threadIdx.x < NWL) a0.wl_count[threadIdx.x * 2 + ((a0.step + 1) & 1)] = 0;
+    StepArgs<Real> a = a0;
+    int lb = xcd_remap(blockIdx.x, gridDim.x);
+    size_t cell_base = 0; // first cell of this block's grid (batch-wide list entries)
+    if constexpr (B != 0) {
+        // the instance whose tile range holds this block (uniform per block)
+        int lo = 0, hi = a.n_inst - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if ((int)a.inst[mid].tile_off <= lb)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const Inst d = a.inst[lo];
+        instance_args(a, d, lo);
+        a.ntx = (d.nx + LTX - 1) / LTX;
+        a.nty = (d.ny + LTY - 1) / LTY;
+        lb -= (int)d.tile_off;
+        cell_base = d.cell_off;
+    }
     const Consts<Real> &K = a.K;
     constexpr int NP = (DIM == 3) ? 7 : 5;
     constexpr int ZC = (DIM == 3) ? LZC : 1;
@@ -168,7 +271,6 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
     __shared__ int s_qn, s_qbase, s_any;
     __shared__ uint32_t s_occ;
 
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const int tx = lb % a.ntx;
     const int rest = lb / a.ntx;
     const int ty = rest % a.nty;
@@ -378,20 +480,22 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a)
     for (int i = threadIdx.x; i < nq; i += LTX * LTY) {
         const int q = s_q[i];
         const int qx = q % LTX, qy = (q / LTX) % LTY, qz = q / (LTX * LTY);
-        dst[i] = (int)cidx(l0 + qx, m0 + qy, k0 + qz, K.nx, K.ny);
+        dst[i] = (int)(cell_base + cidx(l0 + qx, m0 + qy, k0 + qz, K.nx, K.ny));
     }
 }
 
 // Pass 2 over the queued cells: full APT merge.  Cells that need the
 // solid-liquid anisotropy terms go to k_front, cells with more active grains
 // than the register capacity to k_step_general; the rest are finished here.
-template <int DIM, int TC>
+template <int DIM, int TC, int B>
 __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
+    __shared__ uint32_t s_coff[B ? MAX_INST : 1];
+    load_cell_offsets<B>(a, s_coff);
     const Consts<Real> &K = a.K;
     const int n = *wl_counter(a, WL_MERGE, a.step);
-    const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
+    const size_t ncell = a.in.stride; // slot stride
     const bool need_terms = K.aniso || DIM == 2;
     if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->merge_cells, (unsigned long long)n);
     // Same-address device atomics serialise (about 11 ns each on MI355X), so
@@ -407,23 +511,21 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x, par ^= 1) {
         int i = base + threadIdx.x;
         bool valid = i < n;
-        size_t c = valid ? (size_t)wl_list(a, WL_MERGE)[i] : 0;
-        int l = (int)(c % K.nx);
-        int m = (int)((c / K.nx) % K.ny);
-        int k = (int)(c / ((size_t)K.nx * K.ny));
+        const size_t c = valid ? (size_t)wl_list(a, WL_MERGE)[i] : 0;
+        const Where W = locate<B>(a, s_coff, c);
         Cell<Real, DIM, FAST_A> C;
         int ntot = 0;
         bool fits = true, front = false;
         if (valid) {
-            gather<Real, DIM, FAST_A>(K, a.in, ncell, l, m, k, C, &ntot, &fits);
+            gather<Real, DIM, FAST_A>(W.K, W.in, ncell, W.l, W.m, W.k, C, &ntot, &fits);
             front = fits && need_terms && C.liq && C.nsol > 0;
             // interface cells are counted by the interface pass
             if (!front) note_active(a.stats, ntot, K.kmax, a.step, a.halt_mode);
             if (fits && !front) {
-                size_t ti = a.temp_dense ? c : (size_t)k;
-                SlotLiq<Real> L{a.in.liq, K.nx, K.ny};
-                cell_update<Real, DIM, FAST_A, false, TC>(K, L, l, m, k, a.T_out[ti], C);
-                write_cell<Real, DIM, FAST_A>(a.out, ncell, c, C);
+                size_t ti = a.temp_dense ? W.c : (size_t)W.k;
+                SlotLiq<Real> L{W.in.liq, W.K.nx, W.K.ny};
+                cell_update<Real, DIM, FAST_A, false, TC>(W.K, L, W.l, W.m, W.k, W.T[ti], C);
+                write_cell<Real, DIM, FAST_A>(W.out, ncell, W.c, C);
             }
         }
         // q = 0: the many-phase list; q = 1..NFB: interface cells with q solids
@@ -458,10 +560,12 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
 
 // Pass 3: solid-liquid interface cells, compacted so that every lane of a
 // wavefront evaluates the anisotropy terms.
-template <int DIM, int TC>
+template <int DIM, int TC, int B>
 __global__ __launch_bounds__(128) void k_front(StepArgs<Real> a)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
+    __shared__ uint32_t s_coff[B ? MAX_INST : 1];
+    load_cell_offsets<B>(a, s_coff);
     const Consts<Real> &K = a.K;
     // the buckets are processed as one sequence, fewest solids first
     int nb[NFB];
@@ -471,7 +575,7 @@ __global__ __launch_bounds__(128) void k_front(StepArgs<Real> a)
         nb[b] = *wl_counter(a, WL_FRONT + b, a.step);
         n += nb[b];
     }
-    const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
+    const size_t ncell = a.in.stride; // slot stride
     if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->front_cells, (unsigned long long)n);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         int b = 0, off = i;
@@ -482,47 +586,45 @@ __global__ __launch_bounds__(128) void k_front(StepArgs<Real> a)
                 b = q + 1;
             }
         }
-        size_t c = (size_t)wl_list(a, WL_FRONT + b)[off];
-        int l = (int)(c % K.nx);
-        int m = (int)((c / K.nx) % K.ny);
-        int k = (int)(c / ((size_t)K.nx * K.ny));
+        const size_t c = (size_t)wl_list(a, WL_FRONT + b)[off];
+        const Where W = locate<B>(a, s_coff, c);
         Cell<Real, DIM, FAST_A> C;
         int ntot;
         bool fits;
-        gather<Real, DIM, FAST_A>(K, a.in, ncell, l, m, k, C, &ntot, &fits);
+        gather<Real, DIM, FAST_A>(W.K, W.in, ncell, W.l, W.m, W.k, C, &ntot, &fits);
         note_active(a.stats, ntot, K.kmax, a.step, a.halt_mode);
-        size_t ti = a.temp_dense ? c : (size_t)k;
-        SlotLiq<Real> L{a.in.liq, K.nx, K.ny};
-        cell_update<Real, DIM, FAST_A, true, TC>(K, L, l, m, k, a.T_out[ti], C);
-        write_cell<Real, DIM, FAST_A>(a.out, ncell, c, C);
+        size_t ti = a.temp_dense ? W.c : (size_t)W.k;
+        SlotLiq<Real> L{W.in.liq, W.K.nx, W.K.ny};
+        cell_update<Real, DIM, FAST_A, true, TC>(W.K, L, W.l, W.m, W.k, W.T[ti], C);
+        write_cell<Real, DIM, FAST_A>(W.out, ncell, W.c, C);
     }
 }
 
 // Pass 4: cells with more active grains than the merge pass holds.  The
 // arrays live in scratch memory with room for KMAX phases; the loops run over
 // the occupied entries only.
-template <int DIM, int TC>
+template <int DIM, int TC, int B>
 __global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
+    __shared__ uint32_t s_coff[B ? MAX_INST : 1];
+    load_cell_offsets<B>(a, s_coff);
     const Consts<Real> &K = a.K;
     const int n = *wl_counter(a, WL_GENERAL, a.step);
-    const size_t ncell = (size_t)K.nx * K.ny * (DIM == 3 ? K.nz : 1);
+    const size_t ncell = a.in.stride; // slot stride
     if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->slow_cells, (unsigned long long)n);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        size_t c = (size_t)wl_list(a, WL_GENERAL)[i];
-        int l = (int)(c % K.nx);
-        int m = (int)((c / K.nx) % K.ny);
-        int k = (int)(c / ((size_t)K.nx * K.ny));
-        size_t ti = a.temp_dense ? c : (size_t)k;
-        double Tn = a.T_out[ti];
+        const size_t c = (size_t)wl_list(a, WL_GENERAL)[i];
+        const Where W = locate<B>(a, s_coff, c);
+        size_t ti = a.temp_dense ? W.c : (size_t)W.k;
+        double Tn = W.T[ti];
         Cell<Real, DIM, KCAP> C;
         int ntot;
         bool fits;
-        gather<Real, DIM, KCAP>(K, a.in, ncell, l, m, k, C, &ntot, &fits);
-        SlotLiq<Real> L{a.in.liq, K.nx, K.ny};
-        cell_update<Real, DIM, KCAP, true, TC>(K, L, l, m, k, Tn, C);
-        write_cell<Real, DIM, KCAP>(a.out, ncell, c, C);
+        gather<Real, DIM, KCAP>(W.K, W.in, ncell, W.l, W.m, W.k, C, &ntot, &fits);
+        SlotLiq<Real> L{W.in.liq, W.K.nx, W.K.ny};
+        cell_update<Real, DIM, KCAP, true, TC>(W.K, L, W.l, W.m, W.k, Tn, C);
+        write_cell<Real, DIM, KCAP>(W.out, ncell, W.c, C);
     }
 }
 
@@ -550,10 +652,10 @@ __global__ void k_apt_slots(Consts<Real> K, SlotView<Real> S, int32_t *mf, int32
         n = 1;
         if (n <= K.kmax && n <= depth) mf[(size_t)(n - 1) * ncell + r] = 0;
     }
-    M.init(S, ncell);
+    M.init(S, S.stride);
     Real vals[NP];
     uint32_t g;
-    while ((g = M.next(S, ncell, vals)) != NO_ID) {
+    while ((g = M.next(S, S.stride, vals)) != NO_ID) {
         if (!apt_active<NP>(vals)) continue;
         n += 1;
         if (n <= K.kmax && n <= depth) mf[(size_t)(n - 1) * ncell + r] = (int32_t)g;
@@ -585,8 +687,8 @@ __global__ void k_pack(const In *phi, int N, int nx, int ny, int nz, SlotView<Re
         Real v = (Real)phi[(size_t)g * ncell + r];
         if (v != Real(0.0)) {
             if (w < scap) {
-                S.sid[(size_t)w * ncell + c] = (uint16_t)g;
-                S.sval[(size_t)w * ncell + c] = v;
+                S.sid[(size_t)w * S.stride + c] = (uint16_t)g;
+                S.sval[(size_t)w * S.stride + c] = v;
             } else {
                 atomicMax(err, 1);
             }
@@ -610,7 +712,7 @@ __global__ void k_unpack(SlotView<Real> S, int nx, int ny, int nz, Out *phi)
     phi[r] = (Out)S.liq[c];
     int n = S.cnt[c];
     for (int s = 0; s < n; ++s) {
-        size_t o = (size_t)s * ncell + c;
+        size_t o = (size_t)s * S.stride + c;
         phi[(size_t)S.sid[o] * ncell + r] = (Out)S.sval[o];
     }
 }
@@ -829,7 +931,6 @@ __global__ __launch_bounds__(256) void k_observe(SlotView<Real> S, int nx, int n
     double *vol = sh;                  // [N]
     double *red = sh + N;              // [256*6]
     const int k = blockIdx.x;
-    const size_t ncell = (size_t)nx * ny * nz;
     for (int g = threadIdx.x; g < N; g += blockDim.x) vol[g] = 0.0;
     __syncthreads();
     double e_max = 0.0, p_min = 1e300, p_max = -1e300, nan_c = 0.0, solid = 0.0, slots = 0.0;
@@ -845,7 +946,7 @@ __global__ __launch_bounds__(256) void k_observe(SlotView<Real> S, int nx, int n
         p_max = lq > p_max ? lq : p_max;
         atomicAdd(&vol[0], lq);
         for (int s = 0; s < n; ++s) {
-            size_t o = (size_t)s * ncell + c;
+            size_t o = (size_t)s * S.stride + c;
             double v = (double)S.sval[o];
             sum += v;
             if (v != v) nan_c += 1.0;
@@ -913,7 +1014,7 @@ static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1
 // tev (optional): 6 timing events, recorded around the kernels: [0] before
 // k_light, [1] after it, [2] after k_merge, [3] after k_front, and [4]/[5]
 // around k_step_general.
-template <int DIM, int TC>
+template <int DIM, int TC, int B>
 static void launch_step_kernels(const StepArgs<Real> &a, unsigned grid, const int *grids, hipStream_t st,
                                 hipStream_t side, hipEvent_t ev_merged, hipEvent_t ev_general, hipEvent_t *tev)
 {
@@ -921,32 +1022,36 @@ static void launch_step_kernels(const StepArgs<Real> &a, unsigned grid, const in
     if (tev) hipEventRecord(tev[0], st);
     // 32-bit offsets when every byte offset of the light pass fits (8-byte
     // temperature per cell is the widest element it indexes per cell)
-    const size_t ncell = (size_t)a.K.nx * a.K.ny * (DIM == 3 ? a.K.nz : 1);
-    const bool off32 = ncell * 8 <= 0xffffffffull;
-    if (a.temp_dense && off32)
-        hipLaunchKernelGGL((k_light<DIM, 1, uint32_t>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+    const bool off32 = a.in.stride * 8 <= 0xffffffffull;
+    if (B != 0) {
+        if (off32)
+            hipLaunchKernelGGL((k_light<DIM, 0, uint32_t, B>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_light<DIM, 0, size_t, B>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+    } else if (a.temp_dense && off32)
+        hipLaunchKernelGGL((k_light<DIM, 1, uint32_t, 0>), dim3(grid), dim3(LTX * LTY), 0, st, a);
     else if (a.temp_dense)
-        hipLaunchKernelGGL((k_light<DIM, 1, size_t>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+        hipLaunchKernelGGL((k_light<DIM, 1, size_t, 0>), dim3(grid), dim3(LTX * LTY), 0, st, a);
     else if (off32)
-        hipLaunchKernelGGL((k_light<DIM, 0, uint32_t>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+        hipLaunchKernelGGL((k_light<DIM, 0, uint32_t, 0>), dim3(grid), dim3(LTX * LTY), 0, st, a);
     else
-        hipLaunchKernelGGL((k_light<DIM, 0, size_t>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+        hipLaunchKernelGGL((k_light<DIM, 0, size_t, 0>), dim3(grid), dim3(LTX * LTY), 0, st, a);
     if (tev) hipEventRecord(tev[1], st);
-    hipLaunchKernelGGL((k_merge<DIM, TC>), dim3(grids[0]), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_merge<DIM, TC, B>), dim3(grids[0]), dim3(256), 0, st, a);
     if (tev) hipEventRecord(tev[2], st);
     if (side) {
         hipEventRecord(ev_merged, st);
         hipStreamWaitEvent(side, ev_merged, 0);
     } else {
-        hipLaunchKernelGGL((k_front<DIM, TC>), dim3(grids[1]), dim3(128), 0, st, a);
+        hipLaunchKernelGGL((k_front<DIM, TC, B>), dim3(grids[1]), dim3(128), 0, st, a);
         if (tev) hipEventRecord(tev[3], st);
     }
     if (tev) hipEventRecord(tev[4], gs);
-    hipLaunchKernelGGL((k_step_general<DIM, TC>), dim3(grids[2]), dim3(64), 0, gs, a);
+    hipLaunchKernelGGL((k_step_general<DIM, TC, B>), dim3(grids[2]), dim3(64), 0, gs, a);
     if (tev) hipEventRecord(tev[5], gs);
     if (side) {
         hipEventRecord(ev_general, side);
-        hipLaunchKernelGGL((k_front<DIM, TC>), dim3(grids[1]), dim3(128), 0, st, a);
+        hipLaunchKernelGGL((k_front<DIM, TC, B>), dim3(grids[1]), dim3(128), 0, st, a);
         if (tev) hipEventRecord(tev[3], st);
         hipStreamWaitEvent(st, ev_general, 0);
     }
@@ -959,20 +1064,29 @@ hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const 
                                      hipEvent_t *tev)
 {
     StepArgs<Real> a = *args;
+    if (a.n_inst > 0) {
+        // a batch of 3D grids: one launch per kernel covers every instance
+        const unsigned grid = (unsigned)a.n_tiles;
+        if (a.tab_class)
+            launch_step_kernels<3, 1, 1>(a, grid, grids, st, side, ev_merged, ev_general, tev);
+        else
+            launch_step_kernels<3, 0, 1>(a, grid, grids, st, side, ev_merged, ev_general, tev);
+        return hipGetLastError();
+    }
     a.ntx = (a.K.nx + LTX - 1) / LTX;
     a.nty = (a.K.ny + LTY - 1) / LTY;
     const int nzc = dim == 3 ? (a.K.nz + LZC - 1) / LZC : 1;
     const unsigned grid = (unsigned)a.ntx * a.nty * nzc;
     if (dim == 3) {
         if (a.tab_class)
-            launch_step_kernels<3, 1>(a, grid, grids, st, side, ev_merged, ev_general, tev);
+            launch_step_kernels<3, 1, 0>(a, grid, grids, st, side, ev_merged, ev_general, tev);
         else
-            launch_step_kernels<3, 0>(a, grid, grids, st, side, ev_merged, ev_general, tev);
+            launch_step_kernels<3, 0, 0>(a, grid, grids, st, side, ev_merged, ev_general, tev);
     } else {
         if (a.tab_class)
-            launch_step_kernels<2, 1>(a, grid, grids, st, side, ev_merged, ev_general, tev);
+            launch_step_kernels<2, 1, 0>(a, grid, grids, st, side, ev_merged, ev_general, tev);
         else
-            launch_step_kernels<2, 0>(a, grid, grids, st, side, ev_merged, ev_general, tev);
+            launch_step_kernels<2, 0, 0>(a, grid, grids, st, side, ev_merged, ev_general, tev);
     }
     return hipGetLastError();
 }

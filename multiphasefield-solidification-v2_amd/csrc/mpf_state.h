@@ -54,6 +54,7 @@ struct SlotView {
     uint8_t *cnt;
     uint16_t *sid;
     Real *sval;
+    size_t stride; // slot s of cell c is at s*stride + c (the buffer's cell count)
 };
 
 struct Stats {
@@ -66,6 +67,20 @@ struct Stats {
     unsigned long long merge_cells;    // cells queued for the merge pass (summed over steps)
     unsigned long long front_cells;    // solid-liquid interface cells (summed over steps)
 };
+
+// One instance of a batch: its grid and where its cells, classification
+// tiles, occupancy words and temperature planes start in the batch's arrays.
+// A batch is a set of independent 3D grids stepped together: every kernel
+// launch covers all instances.
+struct Inst {
+    int nx, ny, nz;
+    int pad;
+    uint32_t cell_off;
+    uint32_t tile_off;
+    uint32_t occ_off;
+    uint32_t t_off;
+};
+constexpr int MAX_INST = 4096;
 
 template <typename Real>
 struct StepArgs {
@@ -89,6 +104,10 @@ struct StepArgs {
     int step;        // step index (for the ping-pong counters and halt)
     int halt_mode;   // stop at the first KMAX overflow
     int tab_class;   // 1: the tables have the class form (K.cls holds the values)
+    const Inst *inst; // batch instances (n_inst > 0 only for the batched kernels)
+    int n_inst;
+    size_t tab_stride; // table elements per instance (batched)
+    int n_tiles;       // classification tiles over all instances (batched)
     Stats *stats;
     int ntx, nty;    // tiles in x and y
 };

@@ -5,6 +5,7 @@ the C ABI in include/mpf.h; this package is its host-side front end.
 """
 from ._lib import MPFError, CapacityError, lib  # noqa: F401
 from .solver import Solver, KMaxOverflowError  # noqa: F401
+from .batch import Batch  # noqa: F401
 
 KMAX = 50
 LIQ = 0

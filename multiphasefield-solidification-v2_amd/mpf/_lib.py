@@ -26,6 +26,9 @@ EXPORTS = (
     "mpf_dense_apt", "mpf_dense_update", "mpf_dense_temp",
     "mpf_malloc", "mpf_free", "mpf_memcpy_htod", "mpf_memcpy_dtoh", "mpf_memcpy_dtod", "mpf_memset",
     "mpf_device_synchronize", "mpf_last_error", "mpf_abi_version",
+    "mpf_batch_create", "mpf_batch_size", "mpf_batch_set_tables", "mpf_batch_upload_phi",
+    "mpf_batch_init_columns", "mpf_batch_upload_temp", "mpf_batch_download_temp", "mpf_batch_download_phi",
+    "mpf_batch_download_apt", "mpf_batch_observables",
 )
 
 
@@ -110,6 +113,16 @@ def lib():
         "mpf_device_synchronize": (C.c_int, []),
         "mpf_last_error": (C.c_char_p, []),
         "mpf_abi_version": (C.c_int, []),
+        "mpf_batch_create": (C.c_int, [C.POINTER(Config), C.c_int, ip, C.POINTER(C.c_void_p)]),
+        "mpf_batch_size": (C.c_int, [H]),
+        "mpf_batch_set_tables": (C.c_int, [H, C.c_int, dp, dp, dp, dp]),
+        "mpf_batch_upload_phi": (C.c_int, [H, C.c_int, vp, C.c_int]),
+        "mpf_batch_init_columns": (C.c_int, [H, C.c_int, ip, ip, C.c_int, dp, dp]),
+        "mpf_batch_upload_temp": (C.c_int, [H, C.c_int, dp]),
+        "mpf_batch_download_temp": (C.c_int, [H, C.c_int, dp]),
+        "mpf_batch_download_phi": (C.c_int, [H, C.c_int, vp, C.c_int]),
+        "mpf_batch_download_apt": (C.c_int, [H, C.c_int, ip, C.c_int, ip, ip]),
+        "mpf_batch_observables": (C.c_int, [H, C.c_int, dp, ip, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -152,13 +152,7 @@ class Solver:
     def stats(self) -> dict:
         s = _lib.Stats()
         check(_lib.lib().mpf_get_stats(self._h, C.byref(s)))
-        return {"max_active": s.max_active, "kmax_overflow": s.kmax_overflow,
-                "overflow_cells": s.overflow_cells, "general_cells": s.general_cells,
-                "halt_step": s.halt_step, "steps_done": s.steps_done,
-                "last_step_ms": s.last_step_ms, "merge_cells": s.merge_cells,
-                "front_cells": s.front_cells, "class_tables": bool(s.class_tables),
-                "kernel_steps": s.kernel_steps,
-                "kernel_ms": dict(zip(("light", "merge", "front", "general"), list(s.kernel_ms)))}
+        return stats_dict(s)
 
     def set_kernel_timing(self, on: bool = True):
         """Time each kernel of every step with HIP events (summed in stats()["kernel_ms"])."""
@@ -174,19 +168,36 @@ class Solver:
         front = np.zeros((self.nx, self.ny), np.int32)
         out = np.zeros(7, np.float64)
         check(_lib.lib().mpf_observables(self._h, ptr(gv), ptr(front, C.c_int32), ptr(out)))
-        scale = float(dx) ** 3 if dx is not None else 1.0
-        f = front.astype(np.float64)
-        f[front < 0] = np.nan
-        finite = f[np.isfinite(f)]
-        return {
-            "phase_sums": gv,
-            "grain_volumes": gv[1:] * scale,
-            "phi_sum_error_max": out[0], "phi_min": out[1], "phi_max": out[2],
-            "nan_count": int(out[3]), "solid_sum": out[4], "cells": int(out[5]),
-            "stored_slots": int(out[6]),
-            "solid_fraction": out[4] / out[5],
-            "front": f,
-            "front_mean": float(np.mean(finite)) if finite.size else float("nan"),
-            "front_max": float(np.max(finite)) if finite.size else float("nan"),
-            "front_std": float(np.std(finite)) if finite.size else float("nan"),
-        }
+        return summarize_observables(gv, front, out, dx)
+
+
+def stats_dict(s) -> dict:
+    return {"max_active": s.max_active, "kmax_overflow": s.kmax_overflow,
+            "overflow_cells": s.overflow_cells, "general_cells": s.general_cells,
+            "halt_step": s.halt_step, "steps_done": s.steps_done,
+            "last_step_ms": s.last_step_ms, "merge_cells": s.merge_cells,
+            "front_cells": s.front_cells, "class_tables": bool(s.class_tables),
+            "kernel_steps": s.kernel_steps,
+            "kernel_ms": dict(zip(("light", "merge", "front", "general"), list(s.kernel_ms)))}
+
+
+def summarize_observables(gv, front, out, dx) -> dict:
+    """Observables from the device reductions: per-phase sums, the topmost
+    solid cell per column (interface_positions_3d, tests/verification_3d.py:409-416)
+    and the scalar partials."""
+    scale = float(dx) ** 3 if dx is not None else 1.0
+    f = front.astype(np.float64)
+    f[front < 0] = np.nan
+    finite = f[np.isfinite(f)]
+    return {
+        "phase_sums": gv,
+        "grain_volumes": gv[1:] * scale,
+        "phi_sum_error_max": out[0], "phi_min": out[1], "phi_max": out[2],
+        "nan_count": int(out[3]), "solid_sum": out[4], "cells": int(out[5]),
+        "stored_slots": int(out[6]),
+        "solid_fraction": out[4] / out[5],
+        "front": f,
+        "front_mean": float(np.mean(finite)) if finite.size else float("nan"),
+        "front_max": float(np.max(finite)) if finite.size else float("nan"),
+        "front_std": float(np.std(finite)) if finite.size else float("nan"),
+    }

@@ -114,20 +114,21 @@ extern "C" int emu_step(int precision, int dim, const mpf_scalars *s, int nx, in
                         uint16_t *sid_out, void *sval_out, const double *T, int mode, int *paths, int8_t *path_of,
                         int use_class)
 {
+    const size_t ncell = (size_t)nx * ny * (dim == 3 ? nz : 1);
     if (precision == MPF_FP64) {
         Consts<double> K = make_consts<double>(*s, nx, ny, nz, N, kmax, dim == 2 ? 1 : aniso, dim == 2 ? 1 : torque,
                                                tables);
         double cls[3][2];
         if (use_class && !class_tables<double>(K.tab, N, cls)) return -1;
-        SlotView<double> in{(double *)liq_in, (uint8_t *)cnt_in, (uint16_t *)sid_in, (double *)sval_in};
-        SlotView<double> out{(double *)liq_out, cnt_out, sid_out, (double *)sval_out};
+        SlotView<double> in{(double *)liq_in, (uint8_t *)cnt_in, (uint16_t *)sid_in, (double *)sval_in, ncell};
+        SlotView<double> out{(double *)liq_out, cnt_out, sid_out, (double *)sval_out, ncell};
         return step_any<double>(K, dim, use_class, in, out, T, mode, paths, path_of);
     }
     Consts<float> K = make_consts<float>(*s, nx, ny, nz, N, kmax, dim == 2 ? 1 : aniso, dim == 2 ? 1 : torque,
                                          tables);
     float cls[3][2];
     if (use_class && !class_tables<float>(K.tab, N, cls)) return -1;
-    SlotView<float> in{(float *)liq_in, (uint8_t *)cnt_in, (uint16_t *)sid_in, (float *)sval_in};
-    SlotView<float> out{(float *)liq_out, cnt_out, sid_out, (float *)sval_out};
+    SlotView<float> in{(float *)liq_in, (uint8_t *)cnt_in, (uint16_t *)sid_in, (float *)sval_in, ncell};
+    SlotView<float> out{(float *)liq_out, cnt_out, sid_out, (float *)sval_out, ncell};
     return step_any<float>(K, dim, use_class, in, out, T, mode, paths, path_of);
 }

@@ -266,9 +266,8 @@ def test_fp64_general_and_class_tables_bit_exact():
 
 @pytest.mark.parametrize("shape", [(10, 8, 14), (12, 17, 14), (68, 18, 20)])
 def test_fp64_classification_kernels_bit_exact(shape):
-    """Both classification kernels: four cells per thread (nx % 4 == 0 with
-    distinct halo threads) and one cell per thread (the other shapes), on
-    partial tiles in x and y."""
+    """Grids whose classification tiles (64 x 8 cells) are partial in x and y,
+    including the 3D path with 32-bit offsets on odd sizes."""
     nx, ny, nz = shape
     c = cases.voronoi_case(nx=nx, ny=ny, nz=nz, n_solid=5, seed=5, qseed=6, seed_height=5)
     steps = 6
