@@ -53,10 +53,12 @@ def workload(nx=NX, ny=NY, nz=NZ, n_solid=N_SOLID, seed_height=SEED_H):
             "cooling": float(np.float32(d["cooling_rate"])), "N": n_solid + 1}
 
 
-def run_gpu(W, precision, steps, warmup, kernel_steps=0):
+def run_gpu(W, precision, steps, warmup, kernel_steps=0, graphs=False, autotune=0):
     """Warm-up, then `steps` timed steps (HIP events on the solver's stream,
     inputs resident).  With kernel_steps, a further run of that many steps with
-    every kernel bracketed by HIP events gives each kernel's live duration."""
+    every kernel bracketed by HIP events gives each kernel's live duration.
+    graphs: replay steps from captured HIP graphs; autotune: time launch shapes
+    (that many steps each) after the warm-up and keep the fastest."""
     s = Solver((NX, NY, NZ), W["N"], W["scalars"], precision=precision, kmax=50, cooling_rate=W["cooling"])
     try:
         s.set_tables(*W["tables"])
@@ -64,6 +66,10 @@ def run_gpu(W, precision, steps, warmup, kernel_steps=0):
         s.upload_temp(W["temp"])
         if warmup:
             s.step(warmup)
+        tuned = s.autotune(autotune) if autotune else None
+        if graphs:
+            s.set_graphs(True)
+            s.step(2)   # capture both state parities outside the timed region
         ob0 = s.observables(dx=W["p"]["dx"])
         s.reset_stats()
         t0 = time.perf_counter()
@@ -80,7 +86,7 @@ def run_gpu(W, precision, steps, warmup, kernel_steps=0):
             kern = {k: v / ks["kernel_steps"] for k, v in ks["kernel_ms"].items()}
             kern["step"] = ks["last_step_ms"] / ks["kernel_steps"]
         return {"ms": st["last_step_ms"], "wall": wall, "done": done, "ob0": ob0, "ob1": ob1, "stats": st,
-                "bytes": s.device_bytes, "kernel_ms": kern}
+                "bytes": s.device_bytes, "kernel_ms": kern, "launch": s.launch(), "tuned": tuned}
     finally:
         s.close()
 
@@ -144,6 +150,9 @@ def main():
                     help="also run one 256 MiB device-to-device copy (reference for HBM counters)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "pmc_traffic_512x512x256_fp32.json"),
                     help="rocprofv3 PMC traffic summary to report as roofline.traffic")
+    ap.add_argument("--graphs", type=int, default=0, help="replay steps from HIP graphs (1) or launch eagerly (0)")
+    ap.add_argument("--autotune", type=int, default=4,
+                    help="time launch shapes for this many steps each after the warm-up (0: default shape)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the solver has no "
@@ -161,10 +170,12 @@ def main():
     W = workload()
     ncell = NX * NY * NZ
     log(f"workload ready: {NX}x{NY}x{NZ}, N={W['N']}")
-    r32 = run_gpu(W, "fp32", args.steps, args.warmup, kernel_steps=min(args.steps, 100))
+    r32 = run_gpu(W, "fp32", args.steps, args.warmup, kernel_steps=min(args.steps, 100), graphs=bool(args.graphs),
+                  autotune=args.autotune)
     ms32 = r32["ms"] / args.steps
     log(f"fp32: {ms32:.3f} ms/step, stats {r32['stats']}")
     log(f"fp32 per-kernel ms (live HIP events): {r32['kernel_ms']}")
+    log(f"fp32 launch {r32['launch']} graphs={bool(args.graphs)} autotune={r32['tuned']}")
     B32, slot32 = alg_bytes_per_step(r32["ob0"], r32["ob1"], 4, ncell, NZ)
     achieved = B32 / (ms32 * 1e-3) / 1e9
     traffic = measured_traffic(args.traffic_json)
@@ -176,7 +187,8 @@ def main():
         "config": {"workload": "3D random mode, plain method, 512x512x256, 200 Voronoi grains, 24-cell seed "
                                "layer, random quaternions, uniform cooling, KMAX=50",
                    "nx": NX, "ny": NY, "nz": NZ, "n_solid": N_SOLID, "seed_height": SEED_H, "kmax": 50,
-                   "precision": "fp32", "parallelism": "single GPU"},
+                   "precision": "fp32", "parallelism": "single GPU", "graphs": bool(args.graphs),
+                   "launch": r32["launch"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic["bytes_per_step"] if traffic else None,
@@ -194,7 +206,7 @@ def main():
     }
     if not args.no_fp64:
         k64 = args.fp64_steps or args.steps
-        r64 = run_gpu(W, "fp64", k64, args.warmup)
+        r64 = run_gpu(W, "fp64", k64, args.warmup, graphs=bool(args.graphs))
         ms64 = r64["ms"] / k64
         log(f"fp64: {ms64:.3f} ms/step")
         out["fp64_mlups"] = round(ncell * k64 / (r64["ms"] * 1e-3) / 1e6, 2)

@@ -150,6 +150,22 @@ int mpf_batch_download_phi(mpf_handle *h, int inst, void *phi, int dtype);
 int mpf_batch_download_apt(mpf_handle *h, int inst, int32_t *mf, int depth, int32_t *nf, int32_t *status);
 int mpf_batch_observables(mpf_handle *h, int inst, double *grain_volume, int32_t *front, double *out);
 
+/* ---- launch shape, graphs and autotuning (no counterpart in the reference,
+ * which launches each kernel once per step from Python).
+ * mpf_set_launch: blocks of the merge, interface and many-phase kernels (0
+ * keeps the default for that kernel) and whether the many-phase kernel runs
+ * on a side stream concurrent with the interface kernel.  mpf_get_launch:
+ * out[4] = the three block counts in use and the side-stream flag.
+ * mpf_set_graphs: replay each step from a HIP graph captured on first use
+ * (one per state parity); not with halt_on_kmax, and kernel timing bypasses
+ * it.  mpf_autotune: times launch shapes on the current state (trial_steps
+ * per shape), keeps the fastest and restores the state; out (optional, 5
+ * ints): the chosen shape and its time per step in microseconds. */
+int mpf_set_launch(mpf_handle *h, int merge_blocks, int front_blocks, int general_blocks, int side_stream);
+int mpf_get_launch(mpf_handle *h, int *out);
+int mpf_set_graphs(mpf_handle *h, int on);
+int mpf_autotune(mpf_handle *h, int trial_steps, int *out);
+
 /* Time every kernel of every step with HIP events (on: 1) and sum the times
  * into mpf_stats.kernel_ms.  Adds a few microseconds per step.  No
  * counterpart in the reference (it has no timing). */

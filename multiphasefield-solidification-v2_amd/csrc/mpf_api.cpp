@@ -110,7 +110,22 @@ struct mpf_handle {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     size_t bytes = 0;
+    // launch shape: blocks of the merge, interface and many-phase kernels
+    // (0: from the grid size) and the side stream for the many-phase kernel
+    int grid_req[3] = {0, 0, 0};
+    int use_side = 1;
+    // captured steps, one per (state buffer, temperature buffer, step parity)
+    int use_graphs = 0;
+    hipGraphExec_t gexec[8] = {};
 };
+
+static void drop_graphs(mpf_handle *h)
+{
+    for (int i = 0; i < 8; ++i) {
+        if (h->gexec[i]) hipGraphExecDestroy(h->gexec[i]);
+        h->gexec[i] = nullptr;
+    }
+}
 
 static size_t inst_cells(const Inst &d) { return (size_t)d.nx * d.ny * d.nz; }
 
@@ -169,6 +184,7 @@ extern "C" void mpf_destroy(mpf_handle *h)
     hipFree(h->wl_count);
     hipFree(h->stats);
     hipFree(h->inst_d);
+    drop_graphs(h);
     if (h->side) hipStreamSynchronize(h->side);
     for (hipEvent_t e : h->tev) hipEventDestroy(e);
     if (h->ev_merged) hipEventDestroy(h->ev_merged);
@@ -378,6 +394,7 @@ static int set_tables_inst(mpf_handle *h, int ii, const double *wij, const doubl
         cls_ok = class_tables<float>(f.data(), N, cls);
         for (int t = 0; t < 6; ++t) cv[t] = cls[t / 2][t % 2];
     }
+    drop_graphs(h); // the class form and values are part of a captured step
     h->tab_set[ii] = 1;
     h->tab_cls[ii] = cls_ok;
     for (int t = 0; t < 6; ++t) h->cls_vals[(size_t)ii * 6 + t] = cv[t];
@@ -609,11 +626,52 @@ static hipError_t launch_one_step(mpf_handle *h, const int *grids, hipEvent_t *t
         a.tab_stride = h->tab_stride;
         a.n_tiles = h->n_tiles;
     }
+    hipStream_t side = h->use_side ? h->side : nullptr;
     if (sizeof(R) == 8)
-        return mpf_launch_step_f64((const StepArgs<double> *)&a, c.dim, grids, h->stream, h->side, h->ev_merged,
+        return mpf_launch_step_f64((const StepArgs<double> *)&a, c.dim, grids, h->stream, side, h->ev_merged,
                                    h->ev_general, tev);
-    return mpf_launch_step_f32((const StepArgs<float> *)&a, c.dim, grids, h->stream, h->side, h->ev_merged,
+    return mpf_launch_step_f32((const StepArgs<float> *)&a, c.dim, grids, h->stream, side, h->ev_merged,
                                h->ev_general, tev);
+}
+
+// Launch shape of the work-list kernels: the requested block counts, or by
+// default enough waves to fill the GPU and fewer for small grids.
+static void step_grids(mpf_handle *h, int grids[3])
+{
+    auto blocks = [&](size_t threads_per_block, int cap) {
+        size_t b = (h->ncell + threads_per_block - 1) / threads_per_block;
+        return (int)(b < (size_t)cap ? (b ? b : 1) : cap);
+    };
+    grids[0] = h->grid_req[0] > 0 ? h->grid_req[0] : blocks(256, 2048);
+    grids[1] = h->grid_req[1] > 0 ? h->grid_req[1] : blocks(128, 2048);
+    grids[2] = h->grid_req[2] > 0 ? h->grid_req[2] : blocks(64, 256);
+}
+
+static hipError_t launch_step(mpf_handle *h, const int *grids, hipEvent_t *tev)
+{
+    return (h->rsz == 8) ? launch_one_step<double>(h, grids, tev) : launch_one_step<float>(h, grids, tev);
+}
+
+// One step replayed from a graph captured on first use for this state
+// parity.  The captured step covers the side-stream fork and join.
+static hipError_t graph_step(mpf_handle *h, const int *grids)
+{
+    const int key = (h->cur & 1) | ((h->tcur & 1) << 1) | ((h->step_counter & 1) << 2);
+    if (!h->gexec[key]) {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        hipError_t el = launch_step(h, grids, nullptr);
+        e = hipStreamEndCapture(h->stream, &g);
+        if (el != hipSuccess) e = el;
+        if (e == hipSuccess) e = hipGraphInstantiate(&h->gexec[key], g, nullptr, nullptr, 0);
+        if (g) hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            h->gexec[key] = nullptr;
+            return e;
+        }
+    }
+    return hipGraphLaunch(h->gexec[key], h->stream);
 }
 
 extern "C" int mpf_step(mpf_handle *h, int nsteps)
@@ -629,13 +687,9 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
     HIP_TRY(hipMemcpy(&h->stats->halt_step, &m1, sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(h->wl_count, 0, 2 * NWL * sizeof(int)));
     const int s0 = h->step_counter, cur0 = h->cur, tcur0 = h->tcur;
-    // grid-stride work-list kernels: enough waves to fill the GPU, fewer for
-    // small grids
-    auto blocks = [&](size_t threads_per_block, int cap) {
-        size_t b = (h->ncell + threads_per_block - 1) / threads_per_block;
-        return (int)(b < (size_t)cap ? (b ? b : 1) : cap);
-    };
-    const int grids[3] = {blocks(256, 2048), blocks(128, 2048), blocks(64, 256)};
+    int grids[3];
+    step_grids(h, grids);
+    const bool graphs = h->use_graphs && !h->kernel_timing && !c.halt_on_kmax;
     if (h->kernel_timing) {
         const size_t need = (size_t)nsteps * 6;
         while (h->tev.size() < need) {
@@ -647,8 +701,7 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
     for (int i = 0; i < nsteps; ++i) {
         hipEvent_t *tev = h->kernel_timing ? h->tev.data() + (size_t)i * 6 : nullptr;
-        hipError_t e = (h->rsz == 8) ? launch_one_step<double>(h, grids, tev)
-                                     : launch_one_step<float>(h, grids, tev);
+        hipError_t e = graphs ? graph_step(h, grids) : launch_step(h, grids, tev);
         if (e != hipSuccess) return fail(MPF_EHIP, "step launch: %s", hipGetErrorString(e));
         h->cur ^= 1;
         h->tcur ^= 1;
@@ -661,6 +714,7 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
     h->last_ms = ms;
     if (h->kernel_timing) {
         HIP_TRY(hipStreamSynchronize(h->side));
+        if (!h->use_side) HIP_TRY(hipStreamSynchronize(h->stream));
         for (int i = 0; i < nsteps; ++i) {
             hipEvent_t *t = h->tev.data() + (size_t)i * 6;
             float k[4];
@@ -793,6 +847,135 @@ extern "C" int mpf_set_kernel_timing(mpf_handle *h, int on)
 {
     if (!h) return fail(MPF_EINVAL, "mpf_set_kernel_timing: null handle");
     h->kernel_timing = on ? 1 : 0;
+    return 0;
+}
+
+extern "C" int mpf_set_launch(mpf_handle *h, int merge_blocks, int front_blocks, int general_blocks,
+                              int side_stream)
+{
+    if (!h) return fail(MPF_EINVAL, "mpf_set_launch: null handle");
+    if (merge_blocks < 0 || front_blocks < 0 || general_blocks < 0)
+        return fail(MPF_EINVAL, "mpf_set_launch: negative block count");
+    h->grid_req[0] = merge_blocks;
+    h->grid_req[1] = front_blocks;
+    h->grid_req[2] = general_blocks;
+    h->use_side = side_stream ? 1 : 0;
+    drop_graphs(h);
+    return 0;
+}
+
+extern "C" int mpf_get_launch(mpf_handle *h, int *out)
+{
+    if (!h || !out) return fail(MPF_EINVAL, "mpf_get_launch: null argument");
+    int g[3];
+    step_grids(h, g);
+    out[0] = g[0];
+    out[1] = g[1];
+    out[2] = g[2];
+    out[3] = h->use_side;
+    return 0;
+}
+
+extern "C" int mpf_set_graphs(mpf_handle *h, int on)
+{
+    if (!h) return fail(MPF_EINVAL, "mpf_set_graphs: null handle");
+    if (on && h->cfg.halt_on_kmax) return fail(MPF_EINVAL, "mpf_set_graphs: not with halt_on_kmax");
+    h->use_graphs = on ? 1 : 0;
+    if (!on) drop_graphs(h);
+    return 0;
+}
+
+// Copies of the evolving state (both slot buffers hold live data only in
+// `cur`, but the step reads the occupancy of `cur` and writes the other).
+struct Snapshot {
+    void *mem = nullptr;
+};
+
+static int snapshot(mpf_handle *h, Snapshot &sn, bool save)
+{
+    const size_t bl = h->ncell * h->rsz, bc = h->ncell, bs = h->ncell * h->scap * 2,
+                 bv = h->ncell * h->scap * h->rsz, bo = h->nocc * 2, bt = h->nT * 8;
+    const size_t tot = bl + bc + bs + bv + bo + bt;
+    if (save) HIP_TRY(hipMalloc(&sn.mem, tot));
+    char *p = (char *)sn.mem;
+    const Buffers &b = h->b[h->cur];
+    void *src[6] = {b.liq, b.cnt, b.sid, b.sval, b.occ, h->T[h->tcur]};
+    const size_t sz[6] = {bl, bc, bs, bv, bo, bt};
+    for (int i = 0; i < 6; ++i) {
+        if (save)
+            HIP_TRY(hipMemcpyAsync(p, src[i], sz[i], hipMemcpyDeviceToDevice, h->stream));
+        else
+            HIP_TRY(hipMemcpyAsync(src[i], p, sz[i], hipMemcpyDeviceToDevice, h->stream));
+        p += sz[i];
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// Try launch shapes on the current state (trial_steps steps each, after one
+// untimed step) and keep the fastest; the state, step counter and statistics
+// are restored afterwards.  out (optional): the chosen shape as
+// mpf_get_launch reports it, and out[4] = its time per step in microseconds.
+extern "C" int mpf_autotune(mpf_handle *h, int trial_steps, int *out)
+{
+    if (!h) return fail(MPF_EINVAL, "mpf_autotune: null handle");
+    if (!h->tables_set) return fail(MPF_EINVAL, "mpf_autotune: tables not set");
+    if (trial_steps < 1) trial_steps = 4;
+    Snapshot sn;
+    Stats st0;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(&st0, h->stats, sizeof(st0), hipMemcpyDeviceToHost));
+    if (int rc = snapshot(h, sn, true)) return rc;
+    const int s0 = h->step_counter, cur0 = h->cur, tcur0 = h->tcur, done0 = h->steps_done;
+    const int timing0 = h->kernel_timing, graphs0 = h->use_graphs;
+    h->kernel_timing = 0;
+    h->use_graphs = 0;
+    const int mb[] = {512, 1024, 2048, 4096}, fb[] = {512, 1024, 2048, 4096}, gb[] = {64, 256};
+    double best = 1e300;
+    int pick[4] = {0, 0, 0, 1};
+    int rc = 0;
+    for (int side = 1; side >= 0 && !rc; --side)
+        for (int g : gb)
+            for (int f : fb)
+                for (int m : mb) {
+                    if (rc) break;
+                    h->grid_req[0] = m;
+                    h->grid_req[1] = f;
+                    h->grid_req[2] = g;
+                    h->use_side = side;
+                    h->cur = cur0;
+                    h->tcur = tcur0;
+                    h->step_counter = s0;
+                    if ((rc = snapshot(h, sn, false))) break;
+                    if ((rc = mpf_step(h, 1)) < 0) break;
+                    if ((rc = mpf_step(h, trial_steps)) < 0) break;
+                    rc = 0;
+                    const double t = h->last_ms / trial_steps;
+                    if (t < best) {
+                        best = t;
+                        pick[0] = m, pick[1] = f, pick[2] = g, pick[3] = side;
+                    }
+                }
+    h->cur = cur0;
+    h->tcur = tcur0;
+    h->step_counter = s0;
+    h->steps_done = done0;
+    int rc2 = snapshot(h, sn, false);
+    hipFree(sn.mem);
+    h->kernel_timing = timing0;
+    h->use_graphs = graphs0;
+    if (rc < 0) return rc;
+    if (rc2) return rc2;
+    h->grid_req[0] = pick[0];
+    h->grid_req[1] = pick[1];
+    h->grid_req[2] = pick[2];
+    h->use_side = pick[3];
+    drop_graphs(h);
+    HIP_TRY(hipMemcpy(h->stats, &st0, sizeof(st0), hipMemcpyHostToDevice));
+    if (out) {
+        mpf_get_launch(h, out);
+        out[4] = (int)(best * 1000.0 + 0.5);
+    }
     return 0;
 }
 

@@ -29,6 +29,7 @@ EXPORTS = (
     "mpf_batch_create", "mpf_batch_size", "mpf_batch_set_tables", "mpf_batch_upload_phi",
     "mpf_batch_init_columns", "mpf_batch_upload_temp", "mpf_batch_download_temp", "mpf_batch_download_phi",
     "mpf_batch_download_apt", "mpf_batch_observables",
+    "mpf_set_launch", "mpf_get_launch", "mpf_set_graphs", "mpf_autotune",
 )
 
 
@@ -123,6 +124,10 @@ def lib():
         "mpf_batch_download_phi": (C.c_int, [H, C.c_int, vp, C.c_int]),
         "mpf_batch_download_apt": (C.c_int, [H, C.c_int, ip, C.c_int, ip, ip]),
         "mpf_batch_observables": (C.c_int, [H, C.c_int, dp, ip, dp]),
+        "mpf_set_launch": (C.c_int, [H, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "mpf_get_launch": (C.c_int, [H, ip]),
+        "mpf_set_graphs": (C.c_int, [H, C.c_int]),
+        "mpf_autotune": (C.c_int, [H, C.c_int, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -151,5 +151,21 @@ class Batch:
     def reset_stats(self):
         check(_lib.lib().mpf_reset_stats(self._h))
 
+    def set_launch(self, merge_blocks: int = 0, front_blocks: int = 0, general_blocks: int = 0,
+                   side_stream: bool = True):
+        from .solver import set_launch
+        set_launch(self._h, merge_blocks, front_blocks, general_blocks, side_stream)
+
+    def launch(self) -> dict:
+        from .solver import get_launch
+        return get_launch(self._h)
+
+    def set_graphs(self, on: bool = True):
+        check(_lib.lib().mpf_set_graphs(self._h, int(bool(on))))
+
+    def autotune(self, trial_steps: int = 4) -> dict:
+        from .solver import autotune
+        return autotune(self._h, trial_steps)
+
     def set_kernel_timing(self, on: bool = True):
         check(_lib.lib().mpf_set_kernel_timing(self._h, int(bool(on))))

@@ -154,6 +154,19 @@ class Solver:
         check(_lib.lib().mpf_get_stats(self._h, C.byref(s)))
         return stats_dict(s)
 
+    def set_launch(self, merge_blocks: int = 0, front_blocks: int = 0, general_blocks: int = 0,
+                   side_stream: bool = True):
+        set_launch(self._h, merge_blocks, front_blocks, general_blocks, side_stream)
+
+    def launch(self) -> dict:
+        return get_launch(self._h)
+
+    def set_graphs(self, on: bool = True):
+        check(_lib.lib().mpf_set_graphs(self._h, int(bool(on))))
+
+    def autotune(self, trial_steps: int = 4) -> dict:
+        return autotune(self._h, trial_steps)
+
     def set_kernel_timing(self, on: bool = True):
         """Time each kernel of every step with HIP events (summed in stats()["kernel_ms"])."""
         check(_lib.lib().mpf_set_kernel_timing(self._h, int(bool(on))))
@@ -169,6 +182,25 @@ class Solver:
         out = np.zeros(7, np.float64)
         check(_lib.lib().mpf_observables(self._h, ptr(gv), ptr(front, C.c_int32), ptr(out)))
         return summarize_observables(gv, front, out, dx)
+
+
+def set_launch(h, merge_blocks, front_blocks, general_blocks, side_stream):
+    check(_lib.lib().mpf_set_launch(h, int(merge_blocks), int(front_blocks), int(general_blocks),
+                                    int(bool(side_stream))))
+
+
+def get_launch(h) -> dict:
+    out = np.zeros(4, np.int32)
+    check(_lib.lib().mpf_get_launch(h, ptr(out, C.c_int32)))
+    return {"merge_blocks": int(out[0]), "front_blocks": int(out[1]), "general_blocks": int(out[2]),
+            "side_stream": bool(out[3])}
+
+
+def autotune(h, trial_steps) -> dict:
+    out = np.zeros(5, np.int32)
+    check(_lib.lib().mpf_autotune(h, int(trial_steps), ptr(out, C.c_int32)))
+    return {"merge_blocks": int(out[0]), "front_blocks": int(out[1]), "general_blocks": int(out[2]),
+            "side_stream": bool(out[3]), "us_per_step": int(out[4])}
 
 
 def stats_dict(s) -> dict:
