@@ -150,6 +150,24 @@ int mpf_batch_download_phi(mpf_handle *h, int inst, void *phi, int dtype);
 int mpf_batch_download_apt(mpf_handle *h, int inst, int32_t *mf, int depth, int32_t *nf, int32_t *status);
 int mpf_batch_observables(mpf_handle *h, int inst, double *grain_volume, int32_t *front, double *out);
 
+/* Observables of every grid of a batch at once (also accepts a single-grid
+ * handle): grain_volume[n*N] phase sums, front (optional) the topmost solid
+ * cell of each column of each grid in turn (sum of nx*ny), out[n*7] as in
+ * mpf_batch_observables.  Replaces the per-grid host reductions of the
+ * sweeps (run_orientation_scan_3d.py:354-394, tests/verification_3d.py:612-654)
+ * with one launch sequence. */
+int mpf_batch_observables_all(mpf_handle *h, double *grain_volume, int32_t *front, double *out);
+
+/* Ray probes of the orientation scan's directional radii
+ * (run_orientation_scan_3d.py:236-331, trilinear_sample,
+ * max_distance_to_boundary, directional_radius_one_side): ray r starts at
+ * origin[3r..] (cells) in grid inst[r] (0 for a single grid), follows the
+ * unit vector dir[3r..] in steps of step_cells, samples phase phase[r], and
+ * out[r] is the distance (cells) to the first downward crossing of threshold
+ * or NaN.  3D only; computed in double with the reference's rounding. */
+int mpf_probe_rays(mpf_handle *h, int n_ray, const int32_t *inst, const int32_t *phase, const double *origin,
+                   const double *dir, double threshold, double step_cells, double *out);
+
 /* ---- launch shape, graphs and autotuning (no counterpart in the reference,
  * which launches each kernel once per step from Python).
  * mpf_set_launch: blocks of the merge, interface and many-phase kernels (0

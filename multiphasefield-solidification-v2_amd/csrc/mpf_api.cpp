@@ -37,7 +37,10 @@ using namespace mpf;
                                                        const int32_t *, int, hipStream_t);               \
     extern "C" hipError_t mpf_launch_dense_temp##SFX(void *, int, double, size_t, hipStream_t);          \
     extern "C" hipError_t mpf_launch_observe##SFX(SlotView<R>, int, int, int, int, double *, double *,   \
-                                                  int32_t *, hipStream_t);
+                                                  int32_t *, hipStream_t);                                \
+    extern "C" hipError_t mpf_launch_probe##SFX(SlotView<R>, const Inst *, int, const int32_t *,         \
+                                                const int32_t *, const double *, const double *, double,  \
+                                                double, double *, hipStream_t);
 
 MPF_DECLARE_LAUNCHERS(double, _f64)
 MPF_DECLARE_LAUNCHERS(float, _f32)
@@ -81,7 +84,7 @@ struct mpf_handle {
     size_t rsz = 8; // sizeof(Real)
     int batch = 0;                  // created by mpf_batch_create
     std::vector<Inst> inst;         // one entry per grid; a single grid is instance 0
-    Inst *inst_d = nullptr;         // device copy (batches)
+    Inst *inst_d = nullptr;         // device copy
     int n_tiles = 0;                // classification tiles over all instances
     size_t tab_stride = 0;          // table elements per instance (batches)
     std::vector<int> tab_set;       // per instance: tables uploaded
@@ -297,7 +300,7 @@ static int create_handle(const mpf_config &c, const std::vector<Inst> &grids, in
     TRY_RC(dev_alloc(h, (void **)&h->wl, (size_t)NWL * ncell * sizeof(int)));
     TRY_RC(dev_alloc(h, (void **)&h->wl_count, 2 * NWL * sizeof(int)));
     TRY_RC(dev_alloc(h, (void **)&h->stats, sizeof(Stats)));
-    if (batch) {
+    {
         TRY_RC(dev_alloc(h, (void **)&h->inst_d, inst.size() * sizeof(Inst)));
         e = hipMemcpy(h->inst_d, inst.data(), inst.size() * sizeof(Inst), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
@@ -987,53 +990,79 @@ extern "C" int mpf_reset_stats(mpf_handle *h)
     return reset_stats_dev(h);
 }
 
-static int observables_inst(mpf_handle *h, int ii, double *grain_volume, int32_t *front, double *out)
+// Observables of instances [i0, i0 + n): per instance N phase sums, the
+// topmost solid cell of every column (fronts one instance after the other)
+// and 7 scalars.  One device buffer, the reductions of all instances
+// launched back to back, one synchronisation.
+static int observe_many(mpf_handle *h, int i0, int n, double *grain_volume, int32_t *front, double *out)
 {
-    if (!grain_volume || !front || !out) return fail(MPF_EINVAL, "observables: null argument");
+    if (!grain_volume || !out) return fail(MPF_EINVAL, "observables: null argument");
     const mpf_config &c = h->cfg;
-    const Inst &g = h->inst[ii];
-    int N = c.n_phase, nz = g.nz;
-    size_t ncol = (size_t)g.nx * g.ny;
+    const int N = c.n_phase;
     if (((size_t)N + 256 * 6) * sizeof(double) > 64 * 1024)
         return fail(MPF_EINVAL, "observables support at most %d phases", 64 * 1024 / 8 - 1536);
-    size_t bytes = (size_t)N * 8 + (size_t)nz * 6 * 8 + ncol * 4;
+    size_t planes = 0, cols = 0;
+    for (int q = 0; q < n; ++q) {
+        planes += h->inst[i0 + q].nz;
+        cols += (size_t)h->inst[i0 + q].nx * h->inst[i0 + q].ny;
+    }
+    const size_t bytes = (size_t)n * N * 8 + planes * 6 * 8 + cols * 4;
     void *d = nullptr;
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipMalloc(&d, bytes));
-    double *gv = (double *)d, *part = gv + N;
-    int32_t *fr = (int32_t *)(part + (size_t)nz * 6);
+    double *gv = (double *)d, *part = gv + (size_t)n * N;
+    int32_t *fr = (int32_t *)(part + planes * 6);
     hipError_t e = hipMemsetAsync(d, 0, bytes, h->stream);
-    if (e == hipSuccess) {
+    size_t po = 0, fo = 0;
+    for (int q = 0; q < n && e == hipSuccess; ++q) {
+        const int ii = i0 + q;
+        const Inst &g = h->inst[ii];
+        double *gq = gv + (size_t)q * N, *pq = part + po * 6;
+        int32_t *fq = fr + fo;
         if (h->rsz == 8)
-            e = mpf_launch_observe_f64(iview<double>(h, h->cur, ii), g.nx, g.ny, nz, N, gv, part, fr, h->stream);
+            e = mpf_launch_observe_f64(iview<double>(h, h->cur, ii), g.nx, g.ny, g.nz, N, gq, pq, fq, h->stream);
         else
-            e = mpf_launch_observe_f32(iview<float>(h, h->cur, ii), g.nx, g.ny, nz, N, gv, part, fr, h->stream);
+            e = mpf_launch_observe_f32(iview<float>(h, h->cur, ii), g.nx, g.ny, g.nz, N, gq, pq, fq, h->stream);
+        po += g.nz;
+        fo += (size_t)g.nx * g.ny;
     }
-    std::vector<double> hp((size_t)nz * 6);
+    std::vector<double> hp(planes * 6);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    if (e == hipSuccess) e = hipMemcpy(grain_volume, gv, (size_t)N * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(grain_volume, gv, (size_t)n * N * 8, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(front, fr, ncol * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && front) e = hipMemcpy(front, fr, cols * 4, hipMemcpyDeviceToHost);
     hipFree(d);
     if (e != hipSuccess) return fail(MPF_EHIP, "observables: %s", hipGetErrorString(e));
-    double emax = 0.0, pmin = 1e300, pmax = -1e300, nanc = 0.0, solid = 0.0, slots = 0.0;
-    for (int k = 0; k < nz; ++k) {
-        const double *q = hp.data() + (size_t)k * 6;
-        emax = q[0] > emax ? q[0] : emax;
-        pmin = q[1] < pmin ? q[1] : pmin;
-        pmax = q[2] > pmax ? q[2] : pmax;
-        nanc += q[3];
-        solid += q[4];
-        slots += q[5];
+    po = 0;
+    for (int q = 0; q < n; ++q) {
+        const Inst &g = h->inst[i0 + q];
+        double emax = 0.0, pmin = 1e300, pmax = -1e300, nanc = 0.0, solid = 0.0, slots = 0.0;
+        for (int k = 0; k < g.nz; ++k) {
+            const double *p = hp.data() + (po + k) * 6;
+            emax = p[0] > emax ? p[0] : emax;
+            pmin = p[1] < pmin ? p[1] : pmin;
+            pmax = p[2] > pmax ? p[2] : pmax;
+            nanc += p[3];
+            solid += p[4];
+            slots += p[5];
+        }
+        po += g.nz;
+        double *o = out + (size_t)q * 7;
+        o[0] = emax;
+        o[1] = pmin;
+        o[2] = pmax;
+        o[3] = nanc;
+        o[4] = solid;
+        o[5] = (double)inst_cells(g);
+        o[6] = slots;
     }
-    out[0] = emax;
-    out[1] = pmin;
-    out[2] = pmax;
-    out[3] = nanc;
-    out[4] = solid;
-    out[5] = (double)inst_cells(g);
-    out[6] = slots;
     return 0;
+}
+
+static int observables_inst(mpf_handle *h, int ii, double *grain_volume, int32_t *front, double *out)
+{
+    if (!front) return fail(MPF_EINVAL, "observables: null argument");
+    return observe_many(h, ii, 1, grain_volume, front, out);
 }
 
 extern "C" int mpf_observables(mpf_handle *h, double *grain_volume, int32_t *front, double *out)
@@ -1047,6 +1076,54 @@ extern "C" int mpf_batch_observables(mpf_handle *h, int ii, double *grain_volume
 {
     if (int rc = check_inst(h, ii, "mpf_batch_observables")) return rc;
     return observables_inst(h, ii, grain_volume, front, out);
+}
+
+extern "C" int mpf_batch_observables_all(mpf_handle *h, double *grain_volume, int32_t *front, double *out)
+{
+    if (!h) return fail(MPF_EINVAL, "mpf_batch_observables_all: null handle");
+    return observe_many(h, 0, (int)h->inst.size(), grain_volume, front, out);
+}
+
+extern "C" int mpf_probe_rays(mpf_handle *h, int n_ray, const int32_t *inst, const int32_t *phase,
+                              const double *origin, const double *dir, double threshold, double step_cells,
+                              double *out)
+{
+    if (!h) return fail(MPF_EINVAL, "mpf_probe_rays: null handle");
+    if (n_ray < 0) return fail(MPF_EINVAL, "mpf_probe_rays: negative ray count");
+    if (n_ray == 0) return 0;
+    if (!inst || !phase || !origin || !dir || !out) return fail(MPF_EINVAL, "mpf_probe_rays: null argument");
+    if (h->cfg.dim != 3) return fail(MPF_EINVAL, "mpf_probe_rays: 3D grids only");
+    if (!(step_cells > 0.0)) return fail(MPF_EINVAL, "mpf_probe_rays: step_cells must be positive");
+    for (int r = 0; r < n_ray; ++r) {
+        if (inst[r] < 0 || inst[r] >= (int)h->inst.size())
+            return fail(MPF_EINVAL, "mpf_probe_rays: ray %d names grid %d of %d", r, inst[r], (int)h->inst.size());
+        if (phase[r] < 0 || phase[r] >= h->cfg.n_phase)
+            return fail(MPF_EINVAL, "mpf_probe_rays: ray %d names phase %d of %d", r, phase[r], h->cfg.n_phase);
+    }
+    const size_t n = (size_t)n_ray;
+    const size_t bytes = n * 8 * (3 + 3 + 1) + n * 4 * 2;
+    void *d = nullptr;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMalloc(&d, bytes));
+    double *d_or = (double *)d, *d_dir = d_or + 3 * n, *d_out = d_dir + 3 * n;
+    int32_t *d_in = (int32_t *)(d_out + n), *d_ph = d_in + n;
+    hipError_t e = hipMemcpy(d_or, origin, 3 * n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_dir, dir, 3 * n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_in, inst, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_ph, phase, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        if (h->rsz == 8)
+            e = mpf_launch_probe_f64(view<double>(h, h->cur), h->inst_d, n_ray, d_in, d_ph, d_or, d_dir, threshold,
+                                     step_cells, d_out, h->stream);
+        else
+            e = mpf_launch_probe_f32(view<float>(h, h->cur), h->inst_d, n_ray, d_in, d_ph, d_or, d_dir, threshold,
+                                     step_cells, d_out, h->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, d_out, n * 8, hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_probe_rays: %s", hipGetErrorString(e));
+    return 0;
 }
 
 // ------------------------------------------------------ drop-in kernels

@@ -999,6 +999,114 @@ __global__ void k_front_height(const Real *liq, int nx, int ny, int nz, int32_t 
     front[(size_t)l * ny + m] = f;
 }
 
+// ------------------------------------------------------------ ray probes
+// Directional radius probes of the orientation scan
+// (run_orientation_scan_3d.py:236-331): march from an origin along a unit
+// direction in steps of step_cells, sampling phase g by trilinear
+// interpolation, and return the distance (cells) of the first downward
+// crossing of the threshold, linearly interpolated; NaN when the origin is
+// below the threshold or outside, or no crossing is met before the grid
+// boundary.  Contraction is off so that every sum and product rounds as in
+// the reference's double-precision Python.
+
+// Value of phase g at cell c: the liquid array for g = 0, else the slot that
+// holds g (0 when the cell stores no slot for g).
+__device__ __forceinline__ double phase_value(const SlotView<Real> &S, size_t c, int g)
+{
+    if (g == 0) return (double)S.liq[c];
+    const int n = S.cnt[c];
+    for (int s = 0; s < n; ++s) {
+        const size_t o = (size_t)s * S.stride + c;
+        if ((int)S.sid[o] == g) return (double)S.sval[o];
+    }
+    return 0.0;
+}
+
+// trilinear_sample (run_orientation_scan_3d.py:236-267)
+__device__ double probe_sample(const SlotView<Real> &S, int nx, int ny, int nz, int g, double x, double y,
+                               double z)
+{
+#pragma clang fp contract(off)
+    if (x < 0.0 || y < 0.0 || z < 0.0 || x > (double)(nx - 1) || y > (double)(ny - 1) || z > (double)(nz - 1))
+        return __builtin_nan("");
+    const int x0 = (int)floor(x), y0 = (int)floor(y), z0 = (int)floor(z);
+    const int x1 = min(x0 + 1, nx - 1), y1 = min(y0 + 1, ny - 1), z1 = min(z0 + 1, nz - 1);
+    const double tx = x - (double)x0, ty = y - (double)y0, tz = z - (double)z0;
+    auto at = [&](int l, int m, int k) {
+        return phase_value(S, (size_t)l + (size_t)nx * ((size_t)m + (size_t)ny * (size_t)k), g);
+    };
+    const double c000 = at(x0, y0, z0), c100 = at(x1, y0, z0), c010 = at(x0, y1, z0), c110 = at(x1, y1, z0);
+    const double c001 = at(x0, y0, z1), c101 = at(x1, y0, z1), c011 = at(x0, y1, z1), c111 = at(x1, y1, z1);
+    const double c00 = c000 * (1.0 - tx) + c100 * tx;
+    const double c01 = c001 * (1.0 - tx) + c101 * tx;
+    const double c10 = c010 * (1.0 - tx) + c110 * tx;
+    const double c11 = c011 * (1.0 - tx) + c111 * tx;
+    const double c0 = c00 * (1.0 - ty) + c10 * ty;
+    const double c1 = c01 * (1.0 - ty) + c11 * ty;
+    return c0 * (1.0 - tz) + c1 * tz;
+}
+
+// One ray per thread: max_distance_to_boundary and directional_radius_one_side
+// (run_orientation_scan_3d.py:270-331).
+__global__ void k_probe(SlotView<Real> S, const Inst *inst, int n_ray, const int32_t *rinst,
+                        const int32_t *rphase, const double *origin, const double *dir, double thr,
+                        double step, double *out)
+{
+#pragma clang fp contract(off)
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_ray) return;
+    const Inst d = inst[rinst[r]];
+    SlotView<Real> V = S;
+    V.liq += d.cell_off;
+    V.cnt += d.cell_off;
+    V.sid += d.cell_off;
+    V.sval += d.cell_off;
+    const int g = rphase[r];
+    const double c[3] = {origin[3 * r], origin[3 * r + 1], origin[3 * r + 2]};
+    const double u[3] = {dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]};
+    const double up[3] = {(double)d.nx - 1.0, (double)d.ny - 1.0, (double)d.nz - 1.0};
+    double max_s = 0.0;
+    bool any = false;
+    for (int a = 0; a < 3; ++a) {
+        if (fabs(u[a]) < 1.0e-12) continue;
+        const double lim = u[a] > 0.0 ? (up[a] - c[a]) / u[a] : (0.0 - c[a]) / u[a];
+        if (lim > 0.0) {
+            max_s = (!any || lim < max_s) ? lim : max_s;
+            any = true;
+        }
+    }
+    double res = __builtin_nan("");
+    if (max_s > 0.0) {
+        const double v0 = probe_sample(V, d.nx, d.ny, d.nz, g, c[0], c[1], c[2]);
+        if (isfinite(v0) && !(v0 < thr)) {
+            double prev_s = 0.0, prev_v = v0;
+            long n_steps = (long)ceil(max_s / step) + 1;
+            n_steps = n_steps > 2 ? n_steps : 2;
+            for (long i = 1; i <= n_steps; ++i) {
+                const double si = (double)i * step;
+                const double s = si < max_s ? si : max_s;
+                const double v = probe_sample(V, d.nx, d.ny, d.nz, g, c[0] + u[0] * s, c[1] + u[1] * s,
+                                              c[2] + u[2] * s);
+                if (!isfinite(v)) break;
+                if (prev_v >= thr && v < thr) {
+                    const double dv = prev_v - v;
+                    if (fabs(dv) < 1.0e-12) {
+                        res = s;
+                        break;
+                    }
+                    double frac = (prev_v - thr) / dv;
+                    frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
+                    res = prev_s + frac * (s - prev_s);
+                    break;
+                }
+                prev_s = s;
+                prev_v = v;
+            }
+        }
+    }
+    out[r] = res;
+}
+
 } // namespace impl_*
 } // namespace mpf
 
@@ -1183,6 +1291,16 @@ hipError_t MPF_NAME(mpf_launch_dense_update)(const Consts<Real> *K, int dim, con
 hipError_t MPF_NAME(mpf_launch_dense_temp)(void *temp, int temp_f64, double cooling, size_t n, hipStream_t st)
 {
     hipLaunchKernelGGL(k_dense_temp, dim3(nblk(n, 256)), dim3(256), 0, st, temp, temp_f64, cooling, n);
+    return hipGetLastError();
+}
+
+hipError_t MPF_NAME(mpf_launch_probe)(SlotView<Real> S, const Inst *inst, int n_ray, const int32_t *rinst,
+                                      const int32_t *rphase, const double *origin, const double *dir, double thr,
+                                      double step, double *out, hipStream_t st)
+{
+    if (n_ray <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_probe, dim3(nblk((size_t)n_ray, 64)), dim3(64), 0, st, S, inst, n_ray, rinst, rphase,
+                       origin, dir, thr, step, out);
     return hipGetLastError();
 }
 

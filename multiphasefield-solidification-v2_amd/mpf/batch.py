@@ -142,6 +142,20 @@ class Batch:
         check(_lib.lib().mpf_batch_observables(self._h, int(i), ptr(gv), ptr(front, C.c_int32), ptr(out)))
         return summarize_observables(gv, front, out, dx)
 
+    def observables_all(self, with_front: bool = True):
+        """(phase sums [n, N], scalars [n, 7], fronts) of every grid at once
+        (mpf.observe.observables_all)."""
+        from .observe import observables_all
+        return observables_all(self._h, self.shapes, self.n_phase, with_front)
+
+    def directional_radii(self, centers, phase: int = 1, threshold: float = 0.5, step_cells: float = 0.5,
+                          grids=None) -> list[dict]:
+        """Directional radii (cells) of phase `phase` about each grid's centre
+        (mpf.observe.directional_radii)."""
+        from .observe import directional_radii
+        grids = range(len(self.shapes)) if grids is None else grids
+        return directional_radii(self._h, grids, centers, phase, threshold, step_cells)
+
     def stats(self) -> dict:
         from .solver import stats_dict
         s = _lib.Stats()

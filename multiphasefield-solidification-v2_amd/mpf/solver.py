@@ -167,6 +167,12 @@ class Solver:
     def autotune(self, trial_steps: int = 4) -> dict:
         return autotune(self._h, trial_steps)
 
+    def directional_radii(self, center, phase: int = 1, threshold: float = 0.5, step_cells: float = 0.5) -> dict:
+        """Directional radii (cells) of phase `phase` about `center` (cells),
+        from device ray probes (mpf.observe.directional_radii); 3D only."""
+        from .observe import directional_radii
+        return directional_radii(self._h, [0], [center], phase, threshold, step_cells)[0]
+
     def set_kernel_timing(self, on: bool = True):
         """Time each kernel of every step with HIP events (summed in stats()["kernel_ms"])."""
         check(_lib.lib().mpf_set_kernel_timing(self._h, int(bool(on))))
