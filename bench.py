@@ -110,6 +110,31 @@ def measured_traffic(path):
     return {"bytes_per_step": t["bytes_per_step"], "source": os.path.relpath(path, ROOT)}
 
 
+def batched_sweep(steps, graphs):
+    """The batched bicrystal sweep: 256 grids (nx = ny in {64, 96, 128},
+    nz in {96, 128, 192}, two grains split at nx/2, random quaternion pairs,
+    undercooling U[0.05, 0.3), nondimensional units), FP32, all grids in one
+    batch (every kernel launch covers all of them), launch shape autotuned,
+    observables every 500 steps on the GPU."""
+    from mpf import sweep
+    cases = sweep.bicrystal_cases(256, seed=0)
+    res = sweep.run_sweep(cases, steps, precision="fp32", max_batch_cells=1 << 30, sample_every=500,
+                          graphs=graphs, autotune=4, log=log)
+    t = res["timing"]
+    b = t["batches"][0]
+    wins = sum(1 for r in res["cases"] if r["winner_gid"] == 1)
+    return {"workload": "256 bicrystal grids, nx=ny in {64,96,128}, nz in {96,128,192}, N=3, FP32, "
+                        f"{steps} steps, one batch", "instances": t["instances"], "steps": steps,
+            "cell_updates": t["cell_updates"], "instances_per_s": round(t["instances_per_s"], 3),
+            "wall_s": round(t["wall_s"], 3), "gpu_s": round(t["gpu_s"], 3),
+            "mlups_gpu": round(t["mlups_gpu"], 1), "mlups_wall": round(t["mlups_wall"], 1),
+            "per_instance_time_s": {k: round(v, 3) for k, v in t["latency_s"].items()},
+            "step_ms": {k: round(v, 4) for k, v in t["step_ms"].items()},
+            "setup_s": round(b["setup_s"], 3), "autotune_s": round(b["autotune_s"], 3),
+            "launch": b["launch"], "graphs": graphs, "grain1_wins": wins,
+            "kmax_overflow": b["kmax_overflow"]}
+
+
 def cpu_baseline(W, steps=6):
     """FP64 CPU oracle (reference dense layout, OpenMP over cells) on a
     256 x 256 x 48 slab holding the seed layer and the front."""
@@ -153,6 +178,9 @@ def main():
     ap.add_argument("--graphs", type=int, default=0, help="replay steps from HIP graphs (1) or launch eagerly (0)")
     ap.add_argument("--autotune", type=int, default=4,
                     help="time launch shapes for this many steps each after the warm-up (0: default shape)")
+    ap.add_argument("--sweep-steps", type=int, default=3000,
+                    help="steps of the batched bicrystal sweep (256 grids); 0 skips it")
+    ap.add_argument("--sweep-graphs", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the solver has no "
@@ -222,6 +250,8 @@ def main():
                 "grain_fraction_max_rel_diff": float(np.max(np.abs(f32 - f64) / np.maximum(f64, 1e-30))),
                 "solid_fraction_diff": abs(r32["ob1"]["solid_fraction"] - r64["ob1"]["solid_fraction"]),
             }
+    if args.sweep_steps > 0:
+        out["batched_sweep"] = batched_sweep(args.sweep_steps, bool(args.sweep_graphs))
     if not args.no_cpu:
         cb = cpu_baseline(W)
         log(f"cpu baseline: {cb}")

@@ -177,7 +177,8 @@ int mpf_probe_rays(mpf_handle *h, int n_ray, const int32_t *inst, const int32_t 
  * mpf_set_graphs: replay each step from a HIP graph captured on first use
  * (one per state parity); not with halt_on_kmax, and kernel timing bypasses
  * it.  mpf_autotune: times launch shapes on the current state (trial_steps
- * per shape), keeps the fastest and restores the state; out (optional, 5
+ * per shape, coordinate descent over the side stream and the three block
+ * counts), keeps the fastest and restores the state; out (optional, 5
  * ints): the chosen shape and its time per step in microseconds. */
 int mpf_set_launch(mpf_handle *h, int merge_blocks, int front_blocks, int general_blocks, int side_stream);
 int mpf_get_launch(mpf_handle *h, int *out);

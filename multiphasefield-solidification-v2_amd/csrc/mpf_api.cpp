@@ -916,8 +916,8 @@ static int snapshot(mpf_handle *h, Snapshot &sn, bool save)
 }
 
 // Try launch shapes on the current state (trial_steps steps each, after one
-// untimed step) and keep the fastest; the state, step counter and statistics
-// are restored afterwards.  out (optional): the chosen shape as
+// untimed step; about 15 shapes by coordinate descent) and keep the fastest;
+// the state, step counter and statistics are restored afterwards.  out (optional): the chosen shape as
 // mpf_get_launch reports it, and out[4] = its time per step in microseconds.
 extern "C" int mpf_autotune(mpf_handle *h, int trial_steps, int *out)
 {
@@ -933,32 +933,54 @@ extern "C" int mpf_autotune(mpf_handle *h, int trial_steps, int *out)
     const int timing0 = h->kernel_timing, graphs0 = h->use_graphs;
     h->kernel_timing = 0;
     h->use_graphs = 0;
-    const int mb[] = {512, 1024, 2048, 4096}, fb[] = {512, 1024, 2048, 4096}, gb[] = {64, 256};
+    // Coordinate descent from the default shape: the side stream, then the
+    // block counts of the merge, interface and many-phase kernels, each over
+    // its candidates with the others held at their best so far.
+    int cur[4];
+    {
+        int g[3];
+        const int r0[3] = {h->grid_req[0], h->grid_req[1], h->grid_req[2]};
+        h->grid_req[0] = h->grid_req[1] = h->grid_req[2] = 0;
+        step_grids(h, g);
+        h->grid_req[0] = r0[0], h->grid_req[1] = r0[1], h->grid_req[2] = r0[2];
+        cur[0] = g[0], cur[1] = g[1], cur[2] = g[2], cur[3] = 1;
+    }
+    static const int cand_m[] = {512, 1024, 2048, 4096, 8192}, cand_g[] = {64, 128, 256, 512};
+    const int *cands[4] = {cand_m, cand_m, cand_g, nullptr};
+    const int ncand[4] = {5, 5, 4, 2};
     double best = 1e300;
-    int pick[4] = {0, 0, 0, 1};
+    int pick[4] = {cur[0], cur[1], cur[2], cur[3]};
     int rc = 0;
-    for (int side = 1; side >= 0 && !rc; --side)
-        for (int g : gb)
-            for (int f : fb)
-                for (int m : mb) {
-                    if (rc) break;
-                    h->grid_req[0] = m;
-                    h->grid_req[1] = f;
-                    h->grid_req[2] = g;
-                    h->use_side = side;
-                    h->cur = cur0;
-                    h->tcur = tcur0;
-                    h->step_counter = s0;
-                    if ((rc = snapshot(h, sn, false))) break;
-                    if ((rc = mpf_step(h, 1)) < 0) break;
-                    if ((rc = mpf_step(h, trial_steps)) < 0) break;
-                    rc = 0;
-                    const double t = h->last_ms / trial_steps;
-                    if (t < best) {
-                        best = t;
-                        pick[0] = m, pick[1] = f, pick[2] = g, pick[3] = side;
-                    }
-                }
+    auto trial = [&](const int *shape) -> int {
+        h->grid_req[0] = shape[0];
+        h->grid_req[1] = shape[1];
+        h->grid_req[2] = shape[2];
+        h->use_side = shape[3];
+        h->cur = cur0;
+        h->tcur = tcur0;
+        h->step_counter = s0;
+        if (int r = snapshot(h, sn, false)) return r;
+        if (int r = mpf_step(h, 1); r < 0) return r;
+        if (int r = mpf_step(h, trial_steps); r < 0) return r;
+        const double t = h->last_ms / trial_steps;
+        if (t < best) {
+            best = t;
+            for (int j = 0; j < 4; ++j) pick[j] = shape[j];
+        }
+        return 0;
+    };
+    rc = trial(cur);
+    const int order[4] = {3, 0, 1, 2};
+    for (int oi = 0; oi < 4 && !rc; ++oi) {
+        const int dmn = order[oi];
+        for (int v = 0; v < ncand[dmn] && !rc; ++v) {
+            int shape[4] = {pick[0], pick[1], pick[2], pick[3]};
+            const int val = dmn == 3 ? 1 - v : cands[dmn][v];
+            if (val == pick[dmn]) continue;
+            shape[dmn] = val;
+            rc = trial(shape);
+        }
+    }
     h->cur = cur0;
     h->tcur = tcur0;
     h->step_counter = s0;

@@ -228,7 +228,7 @@ def case_temperature(c: Case, p: dict) -> np.ndarray:
 
 
 def run_sweep(cases: list[Case], steps: int, params: dict | None = None, precision: str = "fp32",
-              max_batch_cells: int = 1 << 28, sample_every: int = 0, graphs: bool = False, autotune: int = 0,
+              max_batch_cells: int = 1 << 30, sample_every: int = 0, graphs: bool = False, autotune: int = 0,
               aniso_mode: str = "full", fit_fraction: float = 0.5, fit_end_time: float | None = None,
               probe_step_cells: float = 0.5, log=None) -> dict:
     """Run every case for `steps` steps in batches; returns per-case results
@@ -262,7 +262,9 @@ def run_sweep(cases: list[Case], steps: int, params: dict | None = None, precisi
                     b.init_columns(q, *inputs.columns_twomode_3d(nx, ny, nz, dx, d["delta"], c.seed_height,
                                                                 c.split_index))
                 b.upload_temp(q, case_temperature(c, p))
+            t_tune = time.perf_counter()
             tuned = b.autotune(autotune) if autotune else None
+            t_run = time.perf_counter()
             if graphs:
                 b.set_graphs(True)
             series = [[] for _ in bc]
@@ -303,11 +305,13 @@ def run_sweep(cases: list[Case], steps: int, params: dict | None = None, precisi
             results[members[q]] = _case_result(c, series[q], dx, fit_fraction, fit_end_time, latency=tb1 - t0)
         cells = sum(c.cells for c in bc)
         batch_rows.append({"instances": len(bc), "cells": cells, "gpu_ms_per_step": gpu_ms / steps,
-                           "wall_s": tb1 - tb, "mlups_gpu": cells * steps / (gpu_ms * 1e-3) / 1e6,
+                           "wall_s": tb1 - tb, "setup_s": t_tune - tb, "autotune_s": t_run - t_tune,
+                           "run_s": tb1 - t_run, "mlups_gpu": cells * steps / (gpu_ms * 1e-3) / 1e6,
                            "launch": launch, "autotune": tuned, "max_active": st["max_active"],
                            "kmax_overflow": st["kmax_overflow"]})
-        say(f"batch {bi + 1}/{len(batches)}: {len(bc)} grids, {cells} cells, "
-            f"{gpu_ms / steps:.3f} ms/step, wall {tb1 - tb:.2f} s")
+        say(f"batch {bi + 1}/{len(batches)}: {len(bc)} grids, {cells} cells, {gpu_ms / steps:.3f} ms/step, "
+            f"wall {tb1 - tb:.2f} s (setup {t_tune - tb:.2f}, autotune {t_run - t_tune:.2f}, "
+            f"steps+samples {tb1 - t_run:.2f}), launch {launch}")
     wall = time.perf_counter() - t0
     cell_updates = sum(c.cells for c in cases) * steps
     lat = [r["latency_s"] for r in results]
@@ -419,7 +423,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--sample-every", type=int, default=500)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
-    ap.add_argument("--batch-cells", type=int, default=1 << 28)
+    ap.add_argument("--batch-cells", type=int, default=1 << 30,
+                    help="cells per batch (default: the whole bicrystal sweep in one batch)")
     ap.add_argument("--graphs", type=int, default=0)
     ap.add_argument("--autotune", type=int, default=4)
     ap.add_argument("--nx", type=int, default=96)
