@@ -168,6 +168,14 @@ int mpf_batch_observables_all(mpf_handle *h, double *grain_volume, int32_t *fron
 int mpf_probe_rays(mpf_handle *h, int n_ray, const int32_t *inst, const int32_t *phase, const double *origin,
                    const double *dir, double threshold, double step_cells, double *out);
 
+/* Dominant phase id of every cell of one plane, for the phase-map slices of
+ * save_phase_map_slice_3d (src/plot_utils_3d.py:36-45: np.argmax over
+ * phases, lowest id on ties).  axis 0: xy plane k = index, out[nx][ny];
+ * axis 1: xz plane m = index, out[nx][nz]; axis 2: yz plane l = index,
+ * out[ny][nz].  3D only. */
+int mpf_phase_map(mpf_handle *h, int axis, int index, int32_t *out);
+int mpf_batch_phase_map(mpf_handle *h, int grid, int axis, int index, int32_t *out);
+
 /* ---- launch shape, graphs and autotuning (no counterpart in the reference,
  * which launches each kernel once per step from Python).
  * mpf_set_launch: blocks of the merge, interface and many-phase kernels (0

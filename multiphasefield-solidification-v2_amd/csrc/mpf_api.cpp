@@ -40,7 +40,9 @@ using namespace mpf;
                                                   int32_t *, hipStream_t);                                \
     extern "C" hipError_t mpf_launch_probe##SFX(SlotView<R>, const Inst *, int, const int32_t *,         \
                                                 const int32_t *, const double *, const double *, double,  \
-                                                double, double *, hipStream_t);
+                                                double, double *, hipStream_t);                           \
+    extern "C" hipError_t mpf_launch_phase_map##SFX(SlotView<R>, int, int, int, int, int, int32_t *,      \
+                                                    hipStream_t);
 
 MPF_DECLARE_LAUNCHERS(double, _f64)
 MPF_DECLARE_LAUNCHERS(float, _f32)
@@ -1104,6 +1106,43 @@ extern "C" int mpf_batch_observables_all(mpf_handle *h, double *grain_volume, in
 {
     if (!h) return fail(MPF_EINVAL, "mpf_batch_observables_all: null handle");
     return observe_many(h, 0, (int)h->inst.size(), grain_volume, front, out);
+}
+
+static int phase_map_inst(mpf_handle *h, int ii, int axis, int index, int32_t *out)
+{
+    if (!out) return fail(MPF_EINVAL, "phase_map: null argument");
+    if (h->cfg.dim != 3) return fail(MPF_EINVAL, "phase_map: 3D grids only");
+    const Inst &g = h->inst[ii];
+    const int lim = axis == 0 ? g.nz : (axis == 1 ? g.ny : g.nx);
+    if (axis < 0 || axis > 2) return fail(MPF_EINVAL, "phase_map: axis must be 0 (xy), 1 (xz) or 2 (yz)");
+    if (index < 0 || index >= lim) return fail(MPF_EINVAL, "phase_map: index %d outside [0, %d)", index, lim);
+    const size_t n = (size_t)(axis == 2 ? g.ny : g.nx) * (size_t)(axis == 0 ? g.ny : g.nz);
+    void *d = nullptr;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMalloc(&d, n * 4));
+    hipError_t e = h->rsz == 8
+                       ? mpf_launch_phase_map_f64(iview<double>(h, h->cur, ii), g.nx, g.ny, g.nz, axis, index,
+                                                  (int32_t *)d, h->stream)
+                       : mpf_launch_phase_map_f32(iview<float>(h, h->cur, ii), g.nx, g.ny, g.nz, axis, index,
+                                                  (int32_t *)d, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, d, n * 4, hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(MPF_EHIP, "phase_map: %s", hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int mpf_phase_map(mpf_handle *h, int axis, int index, int32_t *out)
+{
+    if (!h) return fail(MPF_EINVAL, "mpf_phase_map: null handle");
+    if (h->batch) return fail(MPF_EINVAL, "mpf_phase_map: use mpf_batch_phase_map for a batch");
+    return phase_map_inst(h, 0, axis, index, out);
+}
+
+extern "C" int mpf_batch_phase_map(mpf_handle *h, int ii, int axis, int index, int32_t *out)
+{
+    if (int rc = check_inst(h, ii, "mpf_batch_phase_map")) return rc;
+    return phase_map_inst(h, ii, axis, index, out);
 }
 
 extern "C" int mpf_probe_rays(mpf_handle *h, int n_ray, const int32_t *inst, const int32_t *phase,

@@ -1107,6 +1107,37 @@ __global__ void k_probe(SlotView<Real> S, const Inst *inst, int n_ray, const int
     out[r] = res;
 }
 
+// ------------------------------------------------------------ phase maps
+// Dominant phase of every cell of one plane: np.argmax over the phases with
+// the lowest id on ties (plot_utils_3d.save_phase_map_slice_3d,
+// src/plot_utils_3d.py:36-45).  axis 0: the xy plane k = index, out[l][m];
+// axis 1: xz, m = index, out[l][k]; axis 2: yz, l = index, out[m][k].
+__global__ void k_phase_map(SlotView<Real> S, int nx, int ny, int nz, int axis, int index, int32_t *out)
+{
+    const int n0 = axis == 2 ? ny : nx;
+    const int n1 = axis == 0 ? ny : nz;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n0 * n1) return;
+    const int a = p / n1, b = p % n1;
+    const int l = axis == 2 ? index : a;
+    const int m = axis == 0 ? b : (axis == 1 ? index : a);
+    const int k = axis == 0 ? index : b;
+    const size_t c = (size_t)l + (size_t)nx * ((size_t)m + (size_t)ny * (size_t)k);
+    Real best = S.liq[c];
+    int g = 0;
+    const int n = S.cnt[c];
+    for (int s = 0; s < n; ++s) {
+        const size_t o = (size_t)s * S.stride + c;
+        const Real v = S.sval[o];
+        const int id = S.sid[o];
+        if (v > best || (v == best && id < g)) {
+            best = v;
+            g = id;
+        }
+    }
+    out[p] = g;
+}
+
 } // namespace impl_*
 } // namespace mpf
 
@@ -1301,6 +1332,15 @@ hipError_t MPF_NAME(mpf_launch_probe)(SlotView<Real> S, const Inst *inst, int n_
     if (n_ray <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_probe, dim3(nblk((size_t)n_ray, 64)), dim3(64), 0, st, S, inst, n_ray, rinst, rphase,
                        origin, dir, thr, step, out);
+    return hipGetLastError();
+}
+
+hipError_t MPF_NAME(mpf_launch_phase_map)(SlotView<Real> S, int nx, int ny, int nz, int axis, int index,
+                                          int32_t *out, hipStream_t st)
+{
+    const int n0 = axis == 2 ? ny : nx, n1 = axis == 0 ? ny : nz;
+    hipLaunchKernelGGL(k_phase_map, dim3(nblk((size_t)n0 * n1, 256)), dim3(256), 0, st, S, nx, ny, nz, axis, index,
+                       out);
     return hipGetLastError();
 }
 

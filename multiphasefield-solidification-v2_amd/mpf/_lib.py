@@ -30,7 +30,7 @@ EXPORTS = (
     "mpf_batch_init_columns", "mpf_batch_upload_temp", "mpf_batch_download_temp", "mpf_batch_download_phi",
     "mpf_batch_download_apt", "mpf_batch_observables",
     "mpf_set_launch", "mpf_get_launch", "mpf_set_graphs", "mpf_autotune",
-    "mpf_batch_observables_all", "mpf_probe_rays",
+    "mpf_batch_observables_all", "mpf_probe_rays", "mpf_phase_map", "mpf_batch_phase_map",
 )
 
 
@@ -131,6 +131,8 @@ def lib():
         "mpf_autotune": (C.c_int, [H, C.c_int, ip]),
         "mpf_batch_observables_all": (C.c_int, [H, dp, ip, dp]),
         "mpf_probe_rays": (C.c_int, [H, C.c_int, ip, ip, dp, dp, C.c_double, C.c_double, dp]),
+        "mpf_phase_map": (C.c_int, [H, C.c_int, C.c_int, ip]),
+        "mpf_batch_phase_map": (C.c_int, [H, C.c_int, C.c_int, C.c_int, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

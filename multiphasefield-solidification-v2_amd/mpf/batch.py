@@ -148,6 +148,16 @@ class Batch:
         from .observe import observables_all
         return observables_all(self._h, self.shapes, self.n_phase, with_front)
 
+    def phase_map(self, i: int, axis: str = "xy", index: int | None = None) -> np.ndarray:
+        """Dominant phase id over one plane of grid i (Solver.phase_map)."""
+        from .plot import AXES, resolve_slice_index
+        nx, ny, nz = self.shapes[i]
+        idx = resolve_slice_index(axis, self.shapes[i], index)
+        shp = {"xy": (nx, ny), "xz": (nx, nz), "yz": (ny, nz)}[axis]
+        out = np.empty(shp, np.int32)
+        check(_lib.lib().mpf_batch_phase_map(self._h, int(i), AXES[axis], idx, ptr(out, C.c_int32)))
+        return out
+
     def directional_radii(self, centers, phase: int = 1, threshold: float = 0.5, step_cells: float = 0.5,
                           grids=None) -> list[dict]:
         """Directional radii (cells) of phase `phase` about each grid's centre

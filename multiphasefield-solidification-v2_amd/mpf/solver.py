@@ -167,6 +167,18 @@ class Solver:
     def autotune(self, trial_steps: int = 4) -> dict:
         return autotune(self._h, trial_steps)
 
+    def phase_map(self, axis: str = "xy", index: int | None = None) -> np.ndarray:
+        """Dominant phase id over one plane (np.argmax over phases, lowest id
+        on ties): axis 'xy' at z = index -> [nx, ny]; 'xz' at y -> [nx, nz];
+        'yz' at x -> [ny, nz].  index None: the middle plane."""
+        from .plot import AXES, resolve_slice_index
+        nx, ny, nz = self.shape
+        idx = resolve_slice_index(axis, self.shape, index)
+        shp = {"xy": (nx, ny), "xz": (nx, nz), "yz": (ny, nz)}[axis]
+        out = np.empty(shp, np.int32)
+        check(_lib.lib().mpf_phase_map(self._h, AXES[axis], idx, ptr(out, C.c_int32)))
+        return out
+
     def directional_radii(self, center, phase: int = 1, threshold: float = 0.5, step_cells: float = 0.5) -> dict:
         """Directional radii (cells) of phase `phase` about `center` (cells),
         from device ray probes (mpf.observe.directional_radii); 3D only."""
