@@ -176,6 +176,14 @@ int mpf_probe_rays(mpf_handle *h, int n_ray, const int32_t *inst, const int32_t 
 int mpf_phase_map(mpf_handle *h, int axis, int index, int32_t *out);
 int mpf_batch_phase_map(mpf_handle *h, int grid, int axis, int index, int32_t *out);
 
+/* Dense phi exchanged through device memory (no counterpart in the
+ * reference; for device-side consumers such as the surrogate): the same
+ * layouts and checks as mpf_upload_phi / mpf_download_phi and their batch
+ * forms, with phi a device pointer. */
+int mpf_upload_phi_device(mpf_handle *h, const void *phi, int dtype);
+int mpf_download_phi_device(mpf_handle *h, void *phi, int dtype);
+int mpf_batch_download_phi_device(mpf_handle *h, int grid, void *phi, int dtype);
+
 /* ---- launch shape, graphs and autotuning (no counterpart in the reference,
  * which launches each kernel once per step from Python).
  * mpf_set_launch: blocks of the merge, interface and many-phase kernels (0

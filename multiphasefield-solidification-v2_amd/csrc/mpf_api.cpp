@@ -434,17 +434,21 @@ extern "C" int mpf_batch_set_tables(mpf_handle *h, int ii, const double *wij, co
     return set_tables_inst(h, ii, wij, aij, mij, n111);
 }
 
-static int upload_phi_inst(mpf_handle *h, int ii, const void *phi, int dtype)
+// phi: host memory, or device memory with on_device (no staging copy).
+static int upload_phi_inst(mpf_handle *h, int ii, const void *phi, int dtype, bool on_device = false)
 {
     if (dtype != MPF_FP32 && dtype != MPF_FP64) return fail(MPF_EINVAL, "bad dtype");
     const Inst &g = h->inst[ii];
     size_t esz = dtype == MPF_FP64 ? 8 : 4;
     size_t bytes = inst_cells(g) * (size_t)h->cfg.n_phase * esz;
     void *d = nullptr;
-    HIP_TRY(hipMalloc(&d, bytes));
+    if (!on_device) HIP_TRY(hipMalloc(&d, bytes));
     int *err = nullptr;
     hipError_t e = hipMalloc((void **)&err, sizeof(int));
-    if (e == hipSuccess) e = hipMemcpy(d, phi, bytes, hipMemcpyHostToDevice);
+    if (on_device)
+        d = const_cast<void *>(phi);
+    else if (e == hipSuccess)
+        e = hipMemcpy(d, phi, bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(err, 0, sizeof(int));
     // the new state may hold grains anywhere
     if (e == hipSuccess) e = hipMemsetAsync(h->b[h->cur].occ, 0xff, h->nocc * sizeof(uint16_t), h->stream);
@@ -460,7 +464,7 @@ static int upload_phi_inst(mpf_handle *h, int ii, const void *phi, int dtype)
     int herr = 0;
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess) e = hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost);
-    hipFree(d);
+    if (!on_device) hipFree(d);
     hipFree(err);
     if (e != hipSuccess) return fail(MPF_EHIP, "upload_phi: %s", hipGetErrorString(e));
     if (herr) return fail(MPF_ECAPACITY, "a cell holds more than %d non-zero solid phases", h->scap);
@@ -479,6 +483,13 @@ extern "C" int mpf_batch_upload_phi(mpf_handle *h, int ii, const void *phi, int 
     if (!phi) return fail(MPF_EINVAL, "mpf_batch_upload_phi: null argument");
     if (int rc = check_inst(h, ii, "mpf_batch_upload_phi")) return rc;
     return upload_phi_inst(h, ii, phi, dtype);
+}
+
+extern "C" int mpf_upload_phi_device(mpf_handle *h, const void *phi, int dtype)
+{
+    if (!h || !phi) return fail(MPF_EINVAL, "mpf_upload_phi_device: null argument");
+    if (h->batch) return fail(MPF_EINVAL, "mpf_upload_phi_device: single grids only");
+    return upload_phi_inst(h, 0, phi, dtype, true);
 }
 
 static int init_columns_inst(mpf_handle *h, int ii, const int32_t *grain_map, const int32_t *profile, int nprof,
@@ -746,16 +757,17 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
     return done;
 }
 
-static int download_phi_inst(mpf_handle *h, int ii, void *phi, int dtype)
+// phi: host memory, or device memory with on_device (unpacked in place).
+static int download_phi_inst(mpf_handle *h, int ii, void *phi, int dtype, bool on_device = false)
 {
     if (dtype != MPF_FP32 && dtype != MPF_FP64) return fail(MPF_EINVAL, "bad dtype");
     const mpf_config &c = h->cfg;
     const Inst &g = h->inst[ii];
     size_t esz = dtype == MPF_FP64 ? 8 : 4;
     size_t bytes = inst_cells(g) * (size_t)c.n_phase * esz;
-    void *d = nullptr;
+    void *d = phi;
     HIP_TRY(hipStreamSynchronize(h->stream));
-    HIP_TRY(hipMalloc(&d, bytes));
+    if (!on_device) HIP_TRY(hipMalloc(&d, bytes));
     hipError_t e = hipMemsetAsync(d, 0, bytes, h->stream);
     if (e == hipSuccess) {
         if (h->rsz == 8)
@@ -764,8 +776,8 @@ static int download_phi_inst(mpf_handle *h, int ii, void *phi, int dtype)
             e = mpf_launch_unpack_f32(iview<float>(h, h->cur, ii), g.nx, g.ny, g.nz, d, dtype == MPF_FP64, h->stream);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    if (e == hipSuccess) e = hipMemcpy(phi, d, bytes, hipMemcpyDeviceToHost);
-    hipFree(d);
+    if (e == hipSuccess && !on_device) e = hipMemcpy(phi, d, bytes, hipMemcpyDeviceToHost);
+    if (!on_device) hipFree(d);
     if (e != hipSuccess) return fail(MPF_EHIP, "download_phi: %s", hipGetErrorString(e));
     return 0;
 }
@@ -782,6 +794,20 @@ extern "C" int mpf_batch_download_phi(mpf_handle *h, int ii, void *phi, int dtyp
     if (!phi) return fail(MPF_EINVAL, "mpf_batch_download_phi: null argument");
     if (int rc = check_inst(h, ii, "mpf_batch_download_phi")) return rc;
     return download_phi_inst(h, ii, phi, dtype);
+}
+
+extern "C" int mpf_download_phi_device(mpf_handle *h, void *phi, int dtype)
+{
+    if (!h || !phi) return fail(MPF_EINVAL, "mpf_download_phi_device: null argument");
+    if (h->batch) return fail(MPF_EINVAL, "mpf_download_phi_device: use mpf_batch_download_phi_device");
+    return download_phi_inst(h, 0, phi, dtype, true);
+}
+
+extern "C" int mpf_batch_download_phi_device(mpf_handle *h, int ii, void *phi, int dtype)
+{
+    if (!phi) return fail(MPF_EINVAL, "mpf_batch_download_phi_device: null argument");
+    if (int rc = check_inst(h, ii, "mpf_batch_download_phi_device")) return rc;
+    return download_phi_inst(h, ii, phi, dtype, true);
 }
 
 static int download_apt_inst(mpf_handle *h, int ii, int32_t *mf, int depth, int32_t *nf, int32_t *status)

@@ -1,0 +1,816 @@
+// mpf_nn.hip -- kernels of the learned surrogate (no counterpart in the
+// reference): a slot-equivariant 3D U-Net trained on the solver's own
+// trajectories and used to propose 16-step advances.
+//
+// Tensors are channels-last, [B][X][Y][Z][C] with z fastest among the
+// spatial axes; activations and MFMA operands are bf16, accumulation and
+// master weights fp32.  Convolutions are implicit GEMMs on the gfx950 matrix
+// cores (v_mfma_f32_16x16x32_bf16): M = voxels, N = output channels,
+// K = taps x input channels.  Input and output channel counts are multiples
+// of 32.  Each axis is padded periodically or with zeros.
+//
+//   k_conv_fwd     y = act(conv(x, w) + b), 3x3x3 (27 taps) or 1x1x1 (1 tap)
+//   k_conv_bwd_w   dw[tap][co][ci] += sum_v dy[v][co] x[v + tap][ci]
+//   (backward data is k_conv_fwd on the flipped, transposed weights)
+//   k_pool/k_up    2x average pooling, 2x nearest upsampling and their adjoints
+//   slot kernels   per-cell active grains of a dense phi, their features,
+//                  sum pooling, the head's input, targets, applying increments
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mpf_nn.h"
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+struct Geom {
+    int B, X, Y, Z;    // voxel grid
+    int cin, cout;     // channels
+    int ntap;          // 27 or 1
+    int px, py, pz;    // 1: periodic axis, 0: zero padding
+};
+
+__device__ __forceinline__ int wrap_axis(int c, int n, int periodic, bool &ok)
+{
+    if (c >= 0 && c < n) return c;
+    if (!periodic) {
+        ok = false;
+        return 0;
+    }
+    return c < 0 ? c + n : c - n;
+}
+
+// Flat index of the neighbour (b, x+dx, y+dy, z+dz), or -1 outside a
+// zero-padded axis.
+__device__ __forceinline__ int neighbour(const Geom &g, int b, int x, int y, int z, int dx, int dy, int dz)
+{
+    bool ok = true;
+    const int xx = wrap_axis(x + dx, g.X, g.px, ok);
+    const int yy = wrap_axis(y + dy, g.Y, g.py, ok);
+    const int zz = wrap_axis(z + dz, g.Z, g.pz, ok);
+    return ok ? ((b * g.X + xx) * g.Y + yy) * g.Z + zz : -1;
+}
+
+__device__ __forceinline__ void tap_offsets(int ntap, int tap, int &dx, int &dy, int &dz)
+{
+    if (ntap == 27) {
+        dx = tap / 9 - 1;
+        dy = (tap / 3) % 3 - 1;
+        dz = tap % 3 - 1;
+    } else {
+        dx = dy = dz = 0;
+    }
+}
+
+__device__ __forceinline__ void decompose(const Geom &g, int v, int &b, int &x, int &y, int &z)
+{
+    z = v % g.Z;
+    int r = v / g.Z;
+    y = r % g.Y;
+    r /= g.Y;
+    x = r % g.X;
+    b = r / g.X;
+}
+
+// ------------------------------------------------------------ convolution
+// Block: 4 waves; block tile 128 voxels x 32 output channels; each wave
+// 32 voxels x 32 channels as 2 x 2 MFMA tiles of 16 x 16.  MFMA operand
+// layout (16x16x32): lane l supplies A row l%16 and B column l%16, eight K
+// values each from K group l/16; it receives D rows 4*(l/16)+i, column l%16.
+__global__ __launch_bounds__(256) void k_conv_fwd(const bf16 *__restrict__ X, const bf16 *__restrict__ W,
+                                                  const float *__restrict__ bias, bf16 *__restrict__ Y,
+                                                  float *__restrict__ Yf, Geom g, int relu, int nvox)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m0 = blockIdx.x * 128 + wave * 32;
+    const int n0 = blockIdx.y * 32;
+    const int row = lane & 15, kq = lane >> 4;
+    int vb[2], vx[2], vy[2], vz[2];
+    bool vok[2];
+    for (int t = 0; t < 2; ++t) {
+        const int v = m0 + 16 * t + row;
+        vok[t] = v < nvox;
+        decompose(g, vok[t] ? v : 0, vb[t], vx[t], vy[t], vz[t]);
+    }
+    f32x4 acc[2][2];
+    for (int t = 0; t < 2; ++t)
+        for (int u = 0; u < 2; ++u) acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const bf16x8 zero8 = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+    const int kchunks = g.cin >> 5;
+    for (int tap = 0; tap < g.ntap; ++tap) {
+        int dx, dy, dz;
+        tap_offsets(g.ntap, tap, dx, dy, dz);
+        int nb[2];
+        for (int t = 0; t < 2; ++t) nb[t] = vok[t] ? neighbour(g, vb[t], vx[t], vy[t], vz[t], dx, dy, dz) : -1;
+        const bf16 *wt = W + ((size_t)tap * g.cout + n0 + row) * g.cin + kq * 8;
+        for (int kc = 0; kc < kchunks; ++kc) {
+            bf16x8 a[2], b[2];
+            for (int t = 0; t < 2; ++t)
+                a[t] = nb[t] >= 0 ? *(const bf16x8 *)(X + (size_t)nb[t] * g.cin + kc * 32 + kq * 8) : zero8;
+            for (int u = 0; u < 2; ++u) b[u] = *(const bf16x8 *)(wt + (size_t)16 * u * g.cin + kc * 32);
+            for (int t = 0; t < 2; ++t)
+                for (int u = 0; u < 2; ++u)
+                    acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b[u], acc[t][u], 0, 0, 0);
+        }
+    }
+    for (int t = 0; t < 2; ++t)
+        for (int u = 0; u < 2; ++u) {
+            const int c = n0 + 16 * u + row;
+            const float bc = bias ? bias[c] : 0.f;
+            for (int i = 0; i < 4; ++i) {
+                const int v = m0 + 16 * t + 4 * kq + i;
+                if (v >= nvox) continue;
+                float r = acc[t][u][i] + bc;
+                if (relu) r = r > 0.f ? r : 0.f;
+                if (Y) Y[(size_t)v * g.cout + c] = (bf16)r;
+                if (Yf) Yf[(size_t)v * g.cout + c] = r;
+            }
+        }
+}
+
+// dW[tap][co][ci] += sum over a chunk of voxels of dY[v][co] * X[v + tap][ci].
+// Block: 4 waves over one tap and a 32 x 32 (co, ci) tile, each wave a
+// 16 x 16 sub-tile; M = co, N = ci, K = voxels.
+__global__ __launch_bounds__(256) void k_conv_bwd_w(const bf16 *__restrict__ X, const bf16 *__restrict__ dY,
+                                                    float *__restrict__ dW, Geom g, int nvox, int vchunk)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, kq = lane >> 4;
+    const int tap = blockIdx.y;
+    const int nct = g.cout >> 5;
+    const int c0 = (blockIdx.z % nct) * 32 + (wave & 1) * 16;
+    const int i0 = (blockIdx.z / nct) * 32 + (wave >> 1) * 16;
+    int dx, dy, dz;
+    tap_offsets(g.ntap, tap, dx, dy, dz);
+    f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const int vbeg = blockIdx.x * vchunk;
+    const int vend = min(vbeg + vchunk, nvox);
+    for (int v0 = vbeg; v0 < vend; v0 += 32) {
+        bf16x8 a, b;
+        int v = v0 + 8 * kq;
+        int vb_, vx_, vy_, vz_;
+        decompose(g, v < nvox ? v : 0, vb_, vx_, vy_, vz_);
+        for (int e = 0; e < 8; ++e, ++v) {
+            float av = 0.f, bv = 0.f;
+            if (v < vend) {
+                av = (float)dY[(size_t)v * g.cout + c0 + row];
+                const int n = neighbour(g, vb_, vx_, vy_, vz_, dx, dy, dz);
+                bv = n >= 0 ? (float)X[(size_t)n * g.cin + i0 + row] : 0.f;
+            }
+            a[e] = (bf16)av;
+            b[e] = (bf16)bv;
+            if (++vz_ == g.Z) {
+                vz_ = 0;
+                if (++vy_ == g.Y) {
+                    vy_ = 0;
+                    if (++vx_ == g.X) {
+                        vx_ = 0;
+                        ++vb_;
+                    }
+                }
+            }
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    for (int i = 0; i < 4; ++i)
+        atomicAdd(&dW[((size_t)tap * g.cout + c0 + 4 * kq + i) * g.cin + i0 + row], acc[i]);
+}
+
+// fp32 master weights [ntap][co][ci] -> bf16, as they are (flip 0) or flipped
+// in space and transposed to [ntap][ci][co] (flip 1: the weights of the
+// backward-data convolution).
+__global__ void k_weights_bf16(const float *W, bf16 *out, int ntap, int co, int ci, int flip)
+{
+    const size_t n = (size_t)ntap * co * ci;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!flip) {
+        out[i] = (bf16)W[i];
+        return;
+    }
+    // out[t][c_in][c_out] = W[ntap-1-t][c_out][c_in]
+    const int c_out = (int)(i % co);
+    const int c_in = (int)((i / co) % ci);
+    const int t = (int)(i / ((size_t)co * ci));
+    out[i] = (bf16)W[((size_t)(ntap - 1 - t) * co + c_out) * ci + c_in];
+}
+
+__global__ void k_relu_bwd(bf16 *dY, const bf16 *Y, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !((float)Y[i] > 0.f)) dY[i] = (bf16)0.f;
+}
+
+// db[c] += sum_v dY[v][c]; one block per 256-voxel chunk per 64 channels.
+__global__ void k_bias_grad(const bf16 *dY, float *db, int nvox, int C)
+{
+    const int c = blockIdx.y * 64 + (threadIdx.x & 63);
+    const int part = threadIdx.x >> 6;
+    __shared__ float red[4][64];
+    float s = 0.f;
+    if (c < C) {
+        const int vbeg = blockIdx.x * 256;
+        for (int v = vbeg + part; v < min(vbeg + 256, nvox); v += 4) s += (float)dY[(size_t)v * C + c];
+    }
+    red[part][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (part == 0 && c < C) atomicAdd(&db[c], red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] +
+                                                  red[3][threadIdx.x]);
+}
+
+// 2x average pooling [B][X][Y][Z][C] -> [B][X/2][Y/2][Z/2][C] and its adjoint.
+__global__ void k_pool_fwd(const bf16 *X, bf16 *Y, int B, int nx, int ny, int nz, int C)
+{
+    const int hx = nx / 2, hy = ny / 2, hz = nz / 2;
+    const size_t n = (size_t)B * hx * hy * hz * C;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    size_t r = i / C;
+    const int z = (int)(r % hz); r /= hz;
+    const int y = (int)(r % hy); r /= hy;
+    const int x = (int)(r % hx);
+    const int b = (int)(r / hx);
+    float s = 0.f;
+    for (int a = 0; a < 8; ++a) {
+        const int xx = 2 * x + (a >> 2), yy = 2 * y + ((a >> 1) & 1), zz = 2 * z + (a & 1);
+        s += (float)X[((((size_t)b * nx + xx) * ny + yy) * nz + zz) * C + c];
+    }
+    Y[i] = (bf16)(s * 0.125f);
+}
+
+__global__ void k_pool_bwd(const bf16 *dY, bf16 *dX, int B, int nx, int ny, int nz, int C)
+{
+    const size_t n = (size_t)B * nx * ny * nz * C;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    size_t r = i / C;
+    const int z = (int)(r % nz); r /= nz;
+    const int y = (int)(r % ny); r /= ny;
+    const int x = (int)(r % nx);
+    const int b = (int)(r / nx);
+    const int hx = nx / 2, hy = ny / 2, hz = nz / 2;
+    dX[i] = (bf16)(0.125f * (float)dY[((((size_t)b * hx + x / 2) * hy + y / 2) * hz + z / 2) * C + c]);
+}
+
+// 2x nearest upsampling [B][x][y][z][C] -> [B][2x][2y][2z][C] and its adjoint.
+__global__ void k_up_fwd(const bf16 *X, bf16 *Y, int B, int nx, int ny, int nz, int C)
+{
+    const int fx = 2 * nx, fy = 2 * ny, fz = 2 * nz;
+    const size_t n = (size_t)B * fx * fy * fz * C;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    size_t r = i / C;
+    const int z = (int)(r % fz); r /= fz;
+    const int y = (int)(r % fy); r /= fy;
+    const int x = (int)(r % fx);
+    const int b = (int)(r / fx);
+    Y[i] = X[((((size_t)b * nx + x / 2) * ny + y / 2) * nz + z / 2) * C + c];
+}
+
+__global__ void k_up_bwd(const bf16 *dY, bf16 *dX, int B, int nx, int ny, int nz, int C)
+{
+    const size_t n = (size_t)B * nx * ny * nz * C;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    size_t r = i / C;
+    const int z = (int)(r % nz); r /= nz;
+    const int y = (int)(r % ny); r /= ny;
+    const int x = (int)(r % nx);
+    const int b = (int)(r / nx);
+    const int fx = 2 * nx, fy = 2 * ny, fz = 2 * nz;
+    float s = 0.f;
+    for (int a = 0; a < 8; ++a) {
+        const int xx = 2 * x + (a >> 2), yy = 2 * y + ((a >> 1) & 1), zz = 2 * z + (a & 1);
+        s += (float)dY[((((size_t)b * fx + xx) * fy + yy) * fz + zz) * C + c];
+    }
+    dX[i] = (bf16)s;
+}
+
+// dst[v][doff + j] (=, or += with acc) src[v][soff + j] for j < nch.
+__global__ void k_copy_ch(const bf16 *src, int sC, int soff, bf16 *dst, int dC, int doff, int nch, int nvox, int acc)
+{
+    const size_t n = (size_t)nvox * nch;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int j = (int)(i % nch);
+    const size_t v = i / nch;
+    const float s = (float)src[v * sC + soff + j];
+    bf16 *d = dst + v * dC + doff + j;
+    *d = acc ? (bf16)((float)*d + s) : (bf16)s;
+}
+
+// Adam on fp32 parameters (bias-corrected step sizes passed in).
+__global__ void k_adam(float *p, const float *gr, float *m, float *v, size_t n, float lr_c, float b1, float b2,
+                       float eps, float c2)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = gr[i];
+    const float mi = b1 * m[i] + (1.f - b1) * g;
+    const float vi = b2 * v[i] + (1.f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr_c * mi / (sqrtf(vi * c2) + eps);
+}
+
+// ------------------------------------------------------------ slot kernels
+// The band [z0, z0 + Zb) of a dense phi[N][nx][ny][nz] (float32, the
+// reference's layout) as network input.  A cell's slots are the grains
+// (id >= 1) with phi > 0 at the cell or at one of its six neighbours (x, y
+// periodic, z clamped), ascending id, at most K.  Per slot s:
+//   sid[s][v]          grain id or -1
+//   feat[s][v][0..31]  phi_g, three orientation features of g, zeros
+// and per cell aux[v][0..31] = phi_liquid, temperature feature, 1, zeros.
+// *overflow counts cells with more than K candidate grains.
+__global__ void k_slot_features(const float *phi, int N, int nx, int ny, int nz, int z0, int Zb, int K,
+                                const float *orient, const float *tfeat, int32_t *sid, bf16 *feat, bf16 *aux,
+                                int *overflow)
+{
+    const int nv = nx * ny * Zb;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const int kz = v % Zb, m = (v / Zb) % ny, l = v / (Zb * ny);
+    const int k = z0 + kz;
+    const size_t ncell = (size_t)nx * ny * nz;
+    auto at = [&](int g, int ll, int mm, int kk) { return phi[(size_t)g * ncell + ((size_t)ll * ny + mm) * nz + kk]; };
+    const int lp = l + 1 < nx ? l + 1 : 0, lm = l > 0 ? l - 1 : nx - 1;
+    const int mp = m + 1 < ny ? m + 1 : 0, mmn = m > 0 ? m - 1 : ny - 1;
+    const int kp = k + 1 < nz ? k + 1 : nz - 1, km = k > 0 ? k - 1 : 0;
+    int cnt = 0;
+    bool over = false;
+    for (int g = 1; g < N; ++g) {
+        const bool act = at(g, l, m, k) > 0.f || at(g, lp, m, k) > 0.f || at(g, lm, m, k) > 0.f ||
+                         at(g, l, mp, k) > 0.f || at(g, l, mmn, k) > 0.f || at(g, l, m, kp) > 0.f ||
+                         at(g, l, m, km) > 0.f;
+        if (!act) continue;
+        if (cnt == K) {
+            over = true;
+            continue;
+        }
+        sid[(size_t)cnt * nv + v] = g;
+        bf16 *f = feat + ((size_t)cnt * nv + v) * 32;
+        f[0] = (bf16)at(g, l, m, k);
+        f[1] = (bf16)orient[g * 3 + 0];
+        f[2] = (bf16)orient[g * 3 + 1];
+        f[3] = (bf16)orient[g * 3 + 2];
+        for (int j = 4; j < 32; ++j) f[j] = (bf16)0.f;
+        ++cnt;
+    }
+    for (int s = cnt; s < K; ++s) {
+        sid[(size_t)s * nv + v] = -1;
+        bf16 *f = feat + ((size_t)s * nv + v) * 32;
+        for (int j = 0; j < 32; ++j) f[j] = (bf16)0.f;
+    }
+    if (over) atomicAdd(overflow, 1);
+    bf16 *a = aux + (size_t)v * 32;
+    a[0] = (bf16)at(0, l, m, k);
+    a[1] = (bf16)tfeat[k];
+    a[2] = (bf16)1.f;
+    for (int j = 3; j < 32; ++j) a[j] = (bf16)0.f;
+}
+
+// h[v][c] = sum over valid slots of e[s][v][c]  (e: [K][nv][C]).
+__global__ void k_slot_pool(const bf16 *e, const int32_t *sid, bf16 *h, int nv, int K, int C)
+{
+    const size_t n = (size_t)nv * C;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = (int)(i / C);
+    float s = 0.f;
+    for (int k = 0; k < K; ++k)
+        if (sid[(size_t)k * nv + v] >= 0) s += (float)e[(size_t)k * n + i];
+    h[i] = (bf16)s;
+}
+
+// de[s][v][c] (+)= dh[v][c] for valid slots, 0 for empty ones.
+__global__ void k_slot_pool_bwd(const bf16 *dh, const int32_t *sid, bf16 *de, int nv, int K, int C, int acc)
+{
+    const size_t n = (size_t)nv * C;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * K) return;
+    const int k = (int)(i / n);
+    const size_t r = i % n;
+    const int v = (int)(r / C);
+    const float d = sid[(size_t)k * nv + v] >= 0 ? (float)dh[r] : 0.f;
+    de[i] = acc ? (bf16)((float)de[i] + d) : (bf16)d;
+}
+
+// Head input hin[s][v] = [F[v][0..CF) , e[s][v][0..CE)] (channels CF + CE).
+__global__ void k_head_in(const bf16 *F, int CF, const bf16 *e, int CE, bf16 *hin, int nv, int K)
+{
+    const int C = CF + CE;
+    const size_t n = (size_t)K * nv * C;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    const size_t sv = i / C;
+    const int v = (int)(sv % nv);
+    hin[i] = c < CF ? F[(size_t)v * CF + c] : e[sv * CE + (c - CF)];
+}
+
+// Adjoint of k_head_in: dF[v] = sum_s dhin[s][v][0..CF), de[s][v] = dhin[s][v][CF..).
+__global__ void k_head_in_bwd(const bf16 *dhin, bf16 *dF, int CF, bf16 *de, int CE, int nv, int K)
+{
+    const int C = CF + CE;
+    const size_t n = (size_t)nv * C;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    const int v = (int)(i / C);
+    if (c < CF) {
+        float s = 0.f;
+        for (int k = 0; k < K; ++k) s += (float)dhin[((size_t)k * nv + v) * C + c];
+        dF[(size_t)v * CF + c] = (bf16)s;
+    } else {
+        for (int k = 0; k < K; ++k)
+            de[((size_t)k * nv + v) * CE + (c - CF)] = dhin[((size_t)k * nv + v) * C + c];
+    }
+}
+
+// Targets: tgt[s][v] = scale * (phi1[g][cell] - phi0[g][cell]) for valid
+// slots (mask 1), 0 with mask 0 otherwise.
+__global__ void k_slot_targets(const float *phi0, const float *phi1, int nx, int ny, int nz, int z0, int Zb, int K,
+                               const int32_t *sid, float scale, float *tgt, float *mask)
+{
+    const int nv = nx * ny * Zb;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)K * nv) return;
+    const int v = (int)(i % nv);
+    const int g = sid[i];
+    if (g < 0) {
+        tgt[i] = 0.f;
+        mask[i] = 0.f;
+        return;
+    }
+    const int kz = v % Zb, m = (v / Zb) % ny, l = v / (Zb * ny);
+    const size_t c = (size_t)g * nx * ny * nz + ((size_t)l * ny + m) * nz + z0 + kz;
+    tgt[i] = scale * (phi1[c] - phi0[c]);
+    mask[i] = 1.f;
+}
+
+// Masked squared error of pred[s][v] = out[(s*nv+v)*stride] against tgt:
+// loss partial sums (atomic) and the gradient d/dpred of sum/count written
+// into dout[(s*nv+v)*stride] (other channels zero).
+__global__ void k_mse(const float *out, int stride, const float *tgt, const float *mask, size_t n, float inv_count,
+                      bf16 *dout, float *loss)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float red[256];
+    float l = 0.f;
+    if (i < n) {
+        const float d = (out[i * stride] - tgt[i]) * mask[i];
+        l = d * d;
+        bf16 *o = dout + i * stride;
+        o[0] = (bf16)(2.f * d * inv_count);
+        for (int j = 1; j < stride; ++j) o[j] = (bf16)0.f;
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+}
+
+// Apply predicted increments to the band of phi (in place): each slot's
+// grain gets phi + out/scale clipped to [0, 1]; then the liquid is
+// 1 - sum of solids, and solids summing above 1 are scaled down to 1.
+__global__ void k_slot_apply(float *phi, int N, int nx, int ny, int nz, int z0, int Zb, int K, const int32_t *sid,
+                             const float *out, int stride, float inv_scale)
+{
+    const int nv = nx * ny * Zb;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const int kz = v % Zb, m = (v / Zb) % ny, l = v / (Zb * ny);
+    const size_t ncell = (size_t)nx * ny * nz;
+    const size_t c = ((size_t)l * ny + m) * nz + z0 + kz;
+    for (int s = 0; s < K; ++s) {
+        const int g = sid[(size_t)s * nv + v];
+        if (g < 0) continue;
+        float p = phi[(size_t)g * ncell + c] + out[((size_t)s * nv + v) * stride] * inv_scale;
+        phi[(size_t)g * ncell + c] = p < 0.f ? 0.f : (p > 1.f ? 1.f : p);
+    }
+    float sum = 0.f;
+    for (int g = 1; g < N; ++g) sum += phi[(size_t)g * ncell + c];
+    if (sum > 1.f) {
+        const float r = 1.f / sum;
+        for (int g = 1; g < N; ++g) phi[(size_t)g * ncell + c] *= r;
+        phi[c] = 0.f;
+    } else {
+        phi[c] = 1.f - sum;
+    }
+}
+
+// max |a - b| over the band of two dense phi, as float bits (non-negative).
+__global__ void k_band_maxdiff(const float *a, const float *b, int N, int nx, int ny, int nz, int z0, int Zb,
+                               unsigned *out)
+{
+    const size_t nv = (size_t)nx * ny * Zb;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float red[256];
+    float d = 0.f;
+    if (i < nv * N) {
+        const int g = (int)(i / nv);
+        const int v = (int)(i % nv);
+        const int kz = v % Zb, m = (v / Zb) % ny, l = v / (Zb * ny);
+        const size_t c = (size_t)g * nx * ny * nz + ((size_t)l * ny + m) * nz + z0 + kz;
+        d = fabsf(a[c] - b[c]);
+    }
+    red[threadIdx.x] = d;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, __float_as_uint(red[0]));
+}
+
+// max over the band of |(c - b) - (b - a) * inv_h|: the exact one-step change
+// c - b at a proposal b against the proposal's mean rate from a over h steps.
+__global__ void k_band_residual(const float *a, const float *b, const float *c, int N, int nx, int ny, int nz,
+                                int z0, int Zb, float inv_h, unsigned *out)
+{
+    const size_t nv = (size_t)nx * ny * Zb;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float red[256];
+    float d = 0.f;
+    if (i < nv * N) {
+        const int g = (int)(i / nv);
+        const int v = (int)(i % nv);
+        const int kz = v % Zb, m = (v / Zb) % ny, l = v / (Zb * ny);
+        const size_t q = (size_t)g * nx * ny * nz + ((size_t)l * ny + m) * nz + z0 + kz;
+        d = fabsf((c[q] - b[q]) - (b[q] - a[q]) * inv_h);
+    }
+    red[threadIdx.x] = d;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, __float_as_uint(red[0]));
+}
+
+// float <-> bf16 conversions of plain arrays.
+__global__ void k_to_bf16(const float *x, bf16 *y, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (bf16)x[i];
+}
+
+__global__ void k_fill_f32(float *x, float v, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+inline Geom geom(const int *dims, int cin, int cout, int ntap, const int *periodic)
+{
+    Geom g;
+    g.B = dims[0];
+    g.X = dims[1];
+    g.Y = dims[2];
+    g.Z = dims[3];
+    g.cin = cin;
+    g.cout = cout;
+    g.ntap = ntap;
+    g.px = periodic[0];
+    g.py = periodic[1];
+    g.pz = periodic[2];
+    return g;
+}
+
+inline int conv_args_ok(const int *dims, int cin, int cout, int ntap)
+{
+    if (!dims || dims[0] < 1 || dims[1] < 1 || dims[2] < 1 || dims[3] < 1) return 0;
+    if (cin < 32 || cout < 32 || (cin & 31) || (cout & 31)) return 0;
+    if (ntap != 27 && ntap != 1) return 0;
+    const double nv = (double)dims[0] * dims[1] * dims[2] * dims[3];
+    return nv * (cin > cout ? cin : cout) < 2147483647.0;
+}
+
+} // namespace
+
+extern "C" {
+
+int mpf_nn_conv_fwd(const void *x, const void *w, const float *bias, void *y, float *yf, const int *dims, int cin,
+                    int cout, int ntap, const int *periodic, int relu, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!x || !w || (!y && !yf) || !periodic || !conv_args_ok(dims, cin, cout, ntap)) return -1;
+    const Geom g = geom(dims, cin, cout, ntap, periodic);
+    const int nvox = g.B * g.X * g.Y * g.Z;
+    hipLaunchKernelGGL(k_conv_fwd, dim3(nblk(nvox, 128), cout / 32), dim3(256), 0, st, (const bf16 *)x,
+                       (const bf16 *)w, bias, (bf16 *)y, yf, g, relu, nvox);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_conv_bwd_w(const void *x, const void *dy, float *dw, const int *dims, int cin, int cout, int ntap,
+                      const int *periodic, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!x || !dy || !dw || !periodic || !conv_args_ok(dims, cin, cout, ntap)) return -1;
+    const Geom g = geom(dims, cin, cout, ntap, periodic);
+    const int nvox = g.B * g.X * g.Y * g.Z;
+    const int vchunk = 2048;
+    hipLaunchKernelGGL(k_conv_bwd_w, dim3(nblk(nvox, vchunk), ntap, (cout / 32) * (cin / 32)), dim3(256), 0, st,
+                       (const bf16 *)x, (const bf16 *)dy, dw, g, nvox, vchunk);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_weights_bf16(const float *w, void *out, int ntap, int cout, int cin, int flip, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!w || !out || ntap < 1 || cout < 1 || cin < 1) return -1;
+    const size_t n = (size_t)ntap * cout * cin;
+    hipLaunchKernelGGL(k_weights_bf16, dim3(nblk(n, 256)), dim3(256), 0, st, w, (bf16 *)out, ntap, cout, cin, flip);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_relu_bwd(void *dy, const void *y, size_t n, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!dy || !y) return -1;
+    if (n) hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(n, 256)), dim3(256), 0, st, (bf16 *)dy, (const bf16 *)y, n);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_bias_grad(const void *dy, float *db, int nvox, int c, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!dy || !db || nvox < 1 || c < 1) return -1;
+    hipLaunchKernelGGL(k_bias_grad, dim3(nblk(nvox, 256), (c + 63) / 64), dim3(256), 0, st, (const bf16 *)dy, db,
+                       nvox, c);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_pool(const void *x, void *y, const int *dims, int c, int backward, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!x || !y || !dims || c < 1 || (dims[1] & 1) || (dims[2] & 1) || (dims[3] & 1)) return -1;
+    const size_t n = (size_t)dims[0] * dims[1] * dims[2] * dims[3] * c / (backward ? 1 : 8);
+    if (backward)
+        hipLaunchKernelGGL(k_pool_bwd, dim3(nblk(n, 256)), dim3(256), 0, st, (const bf16 *)x, (bf16 *)y, dims[0],
+                           dims[1], dims[2], dims[3], c);
+    else
+        hipLaunchKernelGGL(k_pool_fwd, dim3(nblk(n, 256)), dim3(256), 0, st, (const bf16 *)x, (bf16 *)y, dims[0],
+                           dims[1], dims[2], dims[3], c);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_upsample(const void *x, void *y, const int *dims, int c, int backward, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!x || !y || !dims || c < 1) return -1;
+    const size_t n = (size_t)dims[0] * dims[1] * dims[2] * dims[3] * c * (backward ? 1 : 8);
+    if (backward)
+        hipLaunchKernelGGL(k_up_bwd, dim3(nblk(n, 256)), dim3(256), 0, st, (const bf16 *)x, (bf16 *)y, dims[0],
+                           dims[1], dims[2], dims[3], c);
+    else
+        hipLaunchKernelGGL(k_up_fwd, dim3(nblk(n, 256)), dim3(256), 0, st, (const bf16 *)x, (bf16 *)y, dims[0],
+                           dims[1], dims[2], dims[3], c);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_copy_channels(const void *src, int src_c, int src_off, void *dst, int dst_c, int dst_off, int nch,
+                         int nvox, int accumulate, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!src || !dst || nch < 1 || nvox < 1 || src_off + nch > src_c || dst_off + nch > dst_c) return -1;
+    const size_t n = (size_t)nvox * nch;
+    hipLaunchKernelGGL(k_copy_ch, dim3(nblk(n, 256)), dim3(256), 0, st, (const bf16 *)src, src_c, src_off,
+                       (bf16 *)dst, dst_c, dst_off, nch, nvox, accumulate);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_adam(float *p, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
+                int t, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!p || !g || !m || !v || t < 1) return -1;
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f / (1.f - powf(b2, (float)t));
+    if (n) hipLaunchKernelGGL(k_adam, dim3(nblk(n, 256)), dim3(256), 0, st, p, g, m, v, n, lr / c1, b1, b2, eps, c2);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_slot_features(const float *phi, int n_phase, int nx, int ny, int nz, int z0, int zb, int k,
+                         const float *orient, const float *tfeat, int32_t *sid, void *feat, void *aux, int *overflow,
+                         void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!phi || !orient || !tfeat || !sid || !feat || !aux || !overflow || n_phase < 2 || k < 1 || z0 < 0 ||
+        zb < 1 || z0 + zb > nz)
+        return -1;
+    const int nv = nx * ny * zb;
+    hipLaunchKernelGGL(k_slot_features, dim3(nblk(nv, 128)), dim3(128), 0, st, phi, n_phase, nx, ny, nz, z0, zb, k,
+                       orient, tfeat, sid, (bf16 *)feat, (bf16 *)aux, overflow);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_slot_pool(const void *e, const int32_t *sid, void *h, int nv, int k, int c, int backward, int accumulate,
+                     void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!e || !sid || !h || nv < 1 || k < 1 || c < 1) return -1;
+    if (backward)   // e is dh, h is de
+        hipLaunchKernelGGL(k_slot_pool_bwd, dim3(nblk((size_t)nv * c * k, 256)), dim3(256), 0, st,
+                           (const bf16 *)e, sid, (bf16 *)h, nv, k, c, accumulate);
+    else
+        hipLaunchKernelGGL(k_slot_pool, dim3(nblk((size_t)nv * c, 256)), dim3(256), 0, st, (const bf16 *)e, sid,
+                           (bf16 *)h, nv, k, c);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_head_in(const void *f, int cf, const void *e, int ce, void *hin, int nv, int k, int backward,
+                   void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!f || !e || !hin || nv < 1 || k < 1) return -1;
+    if (backward)   // hin is dhin, f is dF, e is de
+        hipLaunchKernelGGL(k_head_in_bwd, dim3(nblk((size_t)nv * (cf + ce), 256)), dim3(256), 0, st,
+                           (const bf16 *)hin, (bf16 *)f, cf, (bf16 *)e, ce, nv, k);
+    else
+        hipLaunchKernelGGL(k_head_in, dim3(nblk((size_t)k * nv * (cf + ce), 256)), dim3(256), 0, st,
+                           (const bf16 *)f, cf, (const bf16 *)e, ce, (bf16 *)hin, nv, k);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_slot_targets(const float *phi0, const float *phi1, int nx, int ny, int nz, int z0, int zb, int k,
+                        const int32_t *sid, float scale, float *tgt, float *mask, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!phi0 || !phi1 || !sid || !tgt || !mask || z0 < 0 || zb < 1 || z0 + zb > nz) return -1;
+    const size_t n = (size_t)k * nx * ny * zb;
+    hipLaunchKernelGGL(k_slot_targets, dim3(nblk(n, 256)), dim3(256), 0, st, phi0, phi1, nx, ny, nz, z0, zb, k, sid,
+                       scale, tgt, mask);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_mse(const float *out, int stride, const float *tgt, const float *mask, size_t n, float inv_count,
+               void *dout, float *loss, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!out || !tgt || !mask || !dout || !loss || stride < 1) return -1;
+    if (n) hipLaunchKernelGGL(k_mse, dim3(nblk(n, 256)), dim3(256), 0, st, out, stride, tgt, mask, n, inv_count,
+                              (bf16 *)dout, loss);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_slot_apply(float *phi, int n_phase, int nx, int ny, int nz, int z0, int zb, int k, const int32_t *sid,
+                      const float *out, int stride, float inv_scale, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!phi || !sid || !out || z0 < 0 || zb < 1 || z0 + zb > nz || stride < 1) return -1;
+    const int nv = nx * ny * zb;
+    hipLaunchKernelGGL(k_slot_apply, dim3(nblk(nv, 256)), dim3(256), 0, st, phi, n_phase, nx, ny, nz, z0, zb, k,
+                       sid, out, stride, inv_scale);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_band_maxdiff(const float *a, const float *b, int n_phase, int nx, int ny, int nz, int z0, int zb,
+                        unsigned *out, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!a || !b || !out || z0 < 0 || zb < 1 || z0 + zb > nz) return -1;
+    const size_t n = (size_t)n_phase * nx * ny * zb;
+    hipLaunchKernelGGL(k_band_maxdiff, dim3(nblk(n, 256)), dim3(256), 0, st, a, b, n_phase, nx, ny, nz, z0, zb, out);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_band_residual(const float *a, const float *b, const float *c, int n_phase, int nx, int ny, int nz,
+                         int z0, int zb, float inv_h, unsigned *out, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!a || !b || !c || !out || z0 < 0 || zb < 1 || z0 + zb > nz) return -1;
+    const size_t n = (size_t)n_phase * nx * ny * zb;
+    hipLaunchKernelGGL(k_band_residual, dim3(nblk(n, 256)), dim3(256), 0, st, a, b, c, n_phase, nx, ny, nz, z0, zb,
+                       inv_h, out);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_to_bf16(const float *x, void *y, size_t n, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!x || !y) return -1;
+    if (n) hipLaunchKernelGGL(k_to_bf16, dim3(nblk(n, 256)), dim3(256), 0, st, x, (bf16 *)y, n);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_fill(float *x, float v, size_t n, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!x) return -1;
+    if (n) hipLaunchKernelGGL(k_fill_f32, dim3(nblk(n, 256)), dim3(256), 0, st, x, v, n);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+} // extern "C"

@@ -61,6 +61,9 @@ def build(verbose: bool = False) -> str:
         _run([cc, *_common(), f"-DMPF_REAL={real}", f"-DMPF_SFX={sfx}", *flags, "-c", kern, "-o", o],
              verbose)
         objs.append(o)
+    nn = os.path.join(BUILD, "mpf_nn.o")
+    _run([cc, *_common(), "-ffp-contract=fast", "-c", os.path.join(CSRC, "mpf_nn.hip"), "-o", nn], verbose)
+    objs.append(nn)
     api = os.path.join(BUILD, "mpf_api.o")
     _run([cc, *_common(), "-x", "hip", "-c", os.path.join(CSRC, "mpf_api.cpp"), "-o", api], verbose)
     objs.append(api)

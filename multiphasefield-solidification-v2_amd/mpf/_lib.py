@@ -31,6 +31,7 @@ EXPORTS = (
     "mpf_batch_download_apt", "mpf_batch_observables",
     "mpf_set_launch", "mpf_get_launch", "mpf_set_graphs", "mpf_autotune",
     "mpf_batch_observables_all", "mpf_probe_rays", "mpf_phase_map", "mpf_batch_phase_map",
+    "mpf_upload_phi_device", "mpf_download_phi_device", "mpf_batch_download_phi_device",
 )
 
 
@@ -132,6 +133,9 @@ def lib():
         "mpf_batch_observables_all": (C.c_int, [H, dp, ip, dp]),
         "mpf_probe_rays": (C.c_int, [H, C.c_int, ip, ip, dp, dp, C.c_double, C.c_double, dp]),
         "mpf_phase_map": (C.c_int, [H, C.c_int, C.c_int, ip]),
+        "mpf_upload_phi_device": (C.c_int, [H, vp, C.c_int]),
+        "mpf_download_phi_device": (C.c_int, [H, vp, C.c_int]),
+        "mpf_batch_download_phi_device": (C.c_int, [H, C.c_int, vp, C.c_int]),
         "mpf_batch_phase_map": (C.c_int, [H, C.c_int, C.c_int, C.c_int, ip]),
     }
     for name, (res, args) in sig.items():

@@ -10,20 +10,24 @@ from mpf import _lib
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared():
-    src = open(os.path.join(ROOT, "include", "mpf.h")).read()
+def _declared(header="mpf.h"):
+    src = open(os.path.join(ROOT, "include", header)).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(mpf_[a-z0-9_]+)\s*\(", src)))
 
 
 def test_header_matches_binding_table():
+    from mpf import _nnlib
     assert sorted(_lib.EXPORTS) == _declared()
+    assert sorted(_nnlib.NN_EXPORTS) == _declared("mpf_nn.h")
 
 
 def test_library_exports_every_declared_symbol():
     L = _lib.lib()
-    for name in _declared():
+    for name in _declared() + _declared("mpf_nn.h"):
         assert hasattr(L, name), name
+    from mpf import _nnlib
+    _nnlib.lib()        # argtypes bind without a device
 
 
 def test_abi_version_and_errors_without_gpu():
