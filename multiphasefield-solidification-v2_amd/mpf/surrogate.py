@@ -44,7 +44,7 @@ from . import _lib
 
 SCALE = 20.0        # targets are 20 x the 16-step increment
 HORIZON = 16
-K_SLOTS = 4
+K_SLOTS = 6
 BASE = 32
 
 
@@ -486,6 +486,7 @@ class SurrogateStepper:
         self.accepted = self.rejected = self.overflow = 0
         self.steps = 0
         self.last_residual = self.last_change = float("nan")
+        self.ratios = []                     # residual / exact change of every cycle
 
     def _download(self, dst):
         check(_lib.lib().mpf_download_phi_device(self.s._h, _p(dst), FP32))
@@ -528,26 +529,36 @@ class SurrogateStepper:
         nx, ny, nz = self.shape
         z0 = self._band()
         T0 = self.s.temp()
+        # cells with more candidate grains than slots keep the extra grains'
+        # values; the residual check judges the proposal as a whole
         over = self.propose(z0, model)
         self.overflow += int(over > 0)
-        if over == 0:
-            T = T0.copy()
-            for _ in range(HORIZON):          # the proposal's temperature: HORIZON cooling steps
-                T -= self.cooling
+        T = T0.copy()
+        for _ in range(HORIZON):          # the proposal's temperature: HORIZON cooling steps
+            T -= self.cooling
+        try:
             self._upload(self.prop)
-            self.s.upload_temp(T)
-            self.s.step(1)
-            self._download(self.chk)
-            r = self._reduce("mpf_nn_band_residual", _p(self.phi), _p(self.prop), _p(self.chk), self.N, nx, ny, nz,
-                             z0, self.zb, 1.0 / HORIZON)
-            c = self._reduce("mpf_nn_band_maxdiff", _p(self.prop), _p(self.chk), self.N, nx, ny, nz, z0, self.zb)
-            self.last_residual, self.last_change = r, c
-            if r <= self.rate_tol * c:
-                self.accepted += 1
-                self.steps += HORIZON + 1
-                return HORIZON + 1, True
-            self._upload(self.phi)             # restore the state and its temperature
-            self.s.upload_temp(T0)
+        except _lib.CapacityError:          # more solids in a cell than the state's slots hold
+            self._upload(self.phi)
+            self.rejected += 1
+            self.ratios.append(float("inf"))
+            self.s.step(HORIZON)
+            self.steps += HORIZON
+            return HORIZON, False
+        self.s.upload_temp(T)
+        self.s.step(1)
+        self._download(self.chk)
+        r = self._reduce("mpf_nn_band_residual", _p(self.phi), _p(self.prop), _p(self.chk), self.N, nx, ny, nz,
+                         z0, self.zb, 1.0 / HORIZON)
+        c = self._reduce("mpf_nn_band_maxdiff", _p(self.prop), _p(self.chk), self.N, nx, ny, nz, z0, self.zb)
+        self.last_residual, self.last_change = r, c
+        self.ratios.append(r / c if c > 0 else float("inf"))
+        if r <= self.rate_tol * c:
+            self.accepted += 1
+            self.steps += HORIZON + 1
+            return HORIZON + 1, True
+        self._upload(self.phi)             # restore the state and its temperature
+        self.s.upload_temp(T0)
         self.rejected += 1
         self.s.step(HORIZON)
         self.steps += HORIZON
