@@ -135,6 +135,41 @@ def batched_sweep(steps, graphs):
             "kmax_overflow": b["kmax_overflow"]}
 
 
+BF16_DENSE_PEAK_TFLOPS = 2500.0
+
+
+def surrogate_block():
+    """The learned surrogate: trained here on 1920 trajectory pairs from the
+    batched solver (48 grids of 64^3, 2000 steps), then 2000 steps of the
+    128^3 16-grain random-mode case plain and surrogate-assisted.  The
+    speedup counted against the target is the best over the tolerances
+    whose run stays within 1 cell of front error and 1% grain fraction."""
+    from mpf import surrogate_run as SR
+    summ, _ = SR.run(grids=48, traj_steps=2000, every=50, train_steps=2500, eval_steps=2000, log=log)
+    c2 = summ["c2"]
+    ok = [v for k, v in c2.items() if k.startswith("surrogate") and v["front_max_err_cells"] <= 1.0
+          and v["grain_fraction_max_err"] <= 0.01]
+    best = max(ok, key=lambda v: v["speedup_vs_plain"]) if ok else None
+    pr = summ["proposal"]
+    return {
+        "workload": "128^3, 16 grains, 2000 steps, FP32; proposals of 16 steps over a 128x128x32 band",
+        "speedup_at_1cell_front_err": round(best["speedup_vs_plain"], 3) if best else None,
+        "accepted_at_that_tolerance": best["accepted"] if best else None,
+        "plain_wall_s": round(c2["plain"]["wall_s"], 4),
+        "runs": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()
+                     if kk in ("wall_s", "speedup_vs_plain", "accepted", "rejected", "acceptance",
+                               "front_mean_err_cells", "front_max_err_cells", "grain_fraction_max_err",
+                               "rate_tol", "residual_ratio_p50")}
+                 for k, v in c2.items() if k != "plain"},
+        "held_out_skill": round(summ["skill_trained"], 4), "train_s": round(summ["train_s"], 2),
+        "train_ms_per_step": round(summ["train_ms_per_step"], 2), "pairs": summ["pairs"], "params": summ["params"],
+        "roofline": {"bound": "mfma", "kernel": "network forward of one proposal (k_conv_fwd dominant)",
+                     "achieved": round(pr["tflops"], 1), "peak": BF16_DENSE_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(pr["tflops"] / BF16_DENSE_PEAK_TFLOPS, 4), "ms": round(pr["ms"], 3),
+                     "gflop": round(pr["gflop"], 1), "dtype": "bf16 x bf16 -> fp32"},
+    }
+
+
 def cpu_baseline(W, steps=6):
     """FP64 CPU oracle (reference dense layout, OpenMP over cells) on a
     256 x 256 x 48 slab holding the seed layer and the front."""
@@ -181,6 +216,8 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=3000,
                     help="steps of the batched bicrystal sweep (256 grids); 0 skips it")
     ap.add_argument("--sweep-graphs", type=int, default=0)
+    ap.add_argument("--surrogate", type=int, default=1,
+                    help="train the surrogate and measure assisted stepping at 128^3 (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the solver has no "
@@ -252,6 +289,8 @@ def main():
             }
     if args.sweep_steps > 0:
         out["batched_sweep"] = batched_sweep(args.sweep_steps, bool(args.sweep_graphs))
+    if args.surrogate:
+        out["surrogate"] = surrogate_block()
     if not args.no_cpu:
         cb = cpu_baseline(W)
         log(f"cpu baseline: {cb}")

@@ -1,7 +1,7 @@
 """Train the surrogate on the solver's own trajectories and measure
 surrogate-assisted stepping against plain FP32 stepping.
 
-    python -m mpf.surrogate_run --grids 48 --train-steps 1500 --out result/surrogate
+    python -m mpf.surrogate_run --grids 48 --train-steps 2500 --out result/surrogate
 
 Steps:
 1. Trajectories: random-mode seed layers on 64^3 grids in the
@@ -53,6 +53,46 @@ def skill(model, samples):
         if base > 0:
             out.append(1.0 - err / base)
     return float(np.mean(out)) if out else float("nan")
+
+
+def unet_flops(model, nx, ny, zb):
+    """Multiply-add FLOPs (2 per MAC) of one proposal: every convolution
+    at its level's voxel count, for the U-Net and the per-slot layers."""
+    f = 0
+    for lvl, (ca, cb) in enumerate(model.unet.enc):
+        nv = (nx >> lvl) * (ny >> lvl) * (zb >> lvl)
+        f += sum(2 * c.ntap * c.cin * c.cout * nv for c in (ca, cb))
+    for j, (ca, cb) in enumerate(model.unet.dec):
+        lvl = 2 - j
+        nv = (nx >> lvl) * (ny >> lvl) * (zb >> lvl)
+        f += sum(2 * c.ntap * c.cin * c.cout * nv for c in (ca, cb))
+    nv = nx * ny * zb * model.k
+    f += sum(2 * c.ntap * c.cin * c.cout * nv for c in (model.enc, model.h1, model.h2))
+    return f
+
+
+def proposal_rate(model, nx=128, ny=128, zb=32, n_phase=17, reps=10):
+    """Time the network's forward pass of one proposal over an nx x ny x zb
+    band with HIP events on the stream its kernels run on; returns ms and
+    TFLOP/s (bf16 MFMA, fp32 accumulate)."""
+    torch = S._torch()
+    rng = np.random.default_rng(0)
+    k = model.k
+    nv = nx * ny * zb
+    sid = torch.tensor(np.where(rng.random((k, nv)) < 0.5, 1, -1).astype(np.int32), device="cuda")
+    feat = torch.tensor(rng.random((k, nv, 32)).astype(np.float32), device="cuda").to(torch.bfloat16)
+    aux = torch.tensor(rng.random((nv, 32)).astype(np.float32), device="cuda").to(torch.bfloat16)
+    model.forward(sid, feat, aux, (nx, ny, zb), zb)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        model.forward(sid, feat, aux, (nx, ny, zb), zb)
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    fl = unet_flops(model, nx, ny, zb)
+    return {"band": [nx, ny, zb], "ms": ms, "gflop": fl / 1e9, "tflops": fl / (ms * 1e-3) / 1e12}
 
 
 def c2_case(n=128, n_solid=16, seed_height=16, seed=42):
@@ -130,43 +170,55 @@ def compare_c2(model, steps=2000, zb=32, rate_tol=(0.5, 1.0, 2.0), t_scale=0.5, 
     return res
 
 
+def run(grids=48, traj_steps=2000, every=50, train_steps=2500, lr=1e-3, eval_steps=2000, rate_tol=(0.5, 1.0, 2.0),
+        seed=0, log=None):
+    """Trajectories, training, held-out skill, the network's proposal rate
+    and the 128^3 comparison; returns the summary dict and the model."""
+    torch = S._torch()
+    log = log or (lambda m: None)
+    t0 = time.perf_counter()
+    samples, meta = S.generate_samples(n_grids=grids, steps=traj_steps, every=every, seed=seed)
+    t_data = time.perf_counter() - t0
+    n_hold = max(1, len(samples) // 8)
+    order = np.random.default_rng(seed).permutation(len(samples))
+    held = [samples[i] for i in order[:n_hold]]
+    train_set = [samples[i] for i in order[n_hold:]]
+    log(f"{len(samples)} pairs from {grids} grids in {t_data:.1f} s; {len(held)} held out")
+    model = S.SlotModel(seed=seed)
+    sk0 = skill(model, held)
+    t1 = time.perf_counter()
+    hist = S.train(model, train_set, train_steps, lr=lr, log=log, seed=seed)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    sk1 = skill(model, held)
+    log(f"trained {train_steps} steps in {t_train:.1f} s; held-out skill {sk0:.3f} -> {sk1:.3f}")
+    del samples, held, train_set
+    rate = proposal_rate(model)
+    log(f"proposal forward: {rate}")
+    res = compare_c2(model, steps=eval_steps, rate_tol=tuple(rate_tol), t_scale=meta["t_scale"], log=log)
+    summary = {"pairs": int(len(order)), "grids": grids, "data_s": t_data, "train_steps": train_steps,
+               "train_s": t_train, "train_ms_per_step": 1e3 * t_train / max(train_steps, 1),
+               "loss_first": float(np.mean(hist[:50])), "loss_last": float(np.mean(hist[-50:])),
+               "skill_untrained": sk0, "skill_trained": sk1, "params": model.n_params(), "proposal": rate,
+               "c2": res, "meta": dict(meta)}
+    return summary, model
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description="Train the surrogate and measure assisted stepping.")
     ap.add_argument("--grids", type=int, default=48)
     ap.add_argument("--traj-steps", type=int, default=2000)
     ap.add_argument("--every", type=int, default=50)
-    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--train-steps", type=int, default=2500)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--eval-steps", type=int, default=2000)
     ap.add_argument("--rate-tol", type=float, nargs="+", default=[0.5, 1.0, 2.0])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=None, help="write weights.npz and summary.json here")
     args = ap.parse_args(argv)
-    torch = S._torch()
     log = lambda m: print(m, file=sys.stderr, flush=True)   # noqa: E731
-    t0 = time.perf_counter()
-    samples, meta = S.generate_samples(n_grids=args.grids, steps=args.traj_steps, every=args.every, seed=args.seed)
-    t_data = time.perf_counter() - t0
-    n_hold = max(1, len(samples) // 8)
-    rng = np.random.default_rng(args.seed)
-    order = rng.permutation(len(samples))
-    held = [samples[i] for i in order[:n_hold]]
-    train_set = [samples[i] for i in order[n_hold:]]
-    log(f"{len(samples)} pairs from {args.grids} grids in {t_data:.1f} s; {len(held)} held out")
-    model = S.SlotModel(seed=args.seed)
-    sk0 = skill(model, held)
-    t1 = time.perf_counter()
-    hist = S.train(model, train_set, args.train_steps, lr=args.lr, log=log, seed=args.seed)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t1
-    sk1 = skill(model, held)
-    log(f"trained {args.train_steps} steps in {t_train:.1f} s; held-out skill {sk0:.3f} -> {sk1:.3f}")
-    res = compare_c2(model, steps=args.eval_steps, rate_tol=tuple(args.rate_tol), t_scale=meta["t_scale"], log=log)
-    summary = {"pairs": len(samples), "grids": args.grids, "data_s": t_data, "train_steps": args.train_steps,
-               "train_s": t_train, "train_ms_per_step": 1e3 * t_train / max(args.train_steps, 1),
-               "loss_first": float(np.mean(hist[:50])), "loss_last": float(np.mean(hist[-50:])),
-               "skill_untrained": sk0, "skill_trained": sk1, "params": model.n_params(), "c2": res,
-               "meta": {k: v for k, v in meta.items()}}
+    summary, model = run(args.grids, args.traj_steps, args.every, args.train_steps, args.lr, args.eval_steps,
+                         args.rate_tol, args.seed, log)
     if args.out:
         os.makedirs(args.out, exist_ok=True)
         np.savez(os.path.join(args.out, "weights.npz"), **model.state_dict())
