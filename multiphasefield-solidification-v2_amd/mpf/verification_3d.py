@@ -11,7 +11,14 @@ kmax_overflow_detection (:1459), static_flat_interface (:1483),
 two_d_limit_consistency (:1509), boundary_conditions (:1538),
 isotropic_orientation_independence (:1568), anisotropic_preferred_growth
 (:1597), torque_term_contribution (:1637), grain_competition (:1663),
-grain_boundary_groove_trijunction (:1696), convergence_grid_dt_delta (:1726).
+grain_boundary_groove_trijunction (:1696), convergence_grid_dt_delta (:1726),
+single_grain_preferred_growth_benchmark (:2016), directional_preference_map
+(:2070), anisotropy_threshold_test (:2148),
+bicrystal_competition_low_high_driving (:2201),
+bicrystal_interfacial_energy_dominated_regime_test (:2260),
+bicrystal_kinetic_dominated_regime_test (:2326),
+multigrain_competition_extension (:2388), groove_depth_only (:2453),
+robust_groove_angle_estimation (:2491).
 
 The reference measures on host copies of small fields; so does this module
 (these grids are at most 24x12x32), with the device doing the steps and the
@@ -645,6 +652,598 @@ def case_convergence_sweep(case_dir, base, ov):
                                             "fine_runtime_s": rf["metrics"]["runtime_s"]}, case_dir)
 
 
+
+# ------------------------------------------------------------ groove and bicrystal measurements
+def interface_curve_from_field_xz(field_xz, threshold=0.5):
+    """Linear threshold crossing per column (:445-467)."""
+    nx, nz = field_xz.shape
+    xs = np.arange(nx, dtype=np.float64)
+    zc = np.full(nx, np.nan)
+    for x in range(nx):
+        col = field_xz[x]
+        above = np.where(col >= threshold)[0]
+        if above.size == 0:
+            continue
+        k0 = int(above[-1])
+        if k0 >= nz - 1:
+            zc[x] = float(nz - 1)
+            continue
+        v0, v1 = float(col[k0]), float(col[k0 + 1])
+        zc[x] = float(k0) if abs(v1 - v0) < 1e-12 else float(k0) + (threshold - v0) / (v1 - v0)
+    return xs, zc
+
+
+def solid_interface_curve_xz(phi, y_index=None, threshold=0.5):
+    """:470-474"""
+    solid = 1.0 - phi[0]
+    ny = solid.shape[1]
+    y = ny // 2 if y_index is None else int(np.clip(y_index, 0, ny - 1))
+    return interface_curve_from_field_xz(solid[:, y, :], threshold)
+
+
+def sample_column_at_fractional_z(col, z):
+    """:477-486"""
+    if not np.isfinite(z):
+        return float("nan")
+    nz = col.shape[0]
+    z0 = int(np.clip(int(np.floor(z)), 0, nz - 1))
+    z1 = min(z0 + 1, nz - 1)
+    f = float(z - z0)
+    return float((1.0 - f) * col[z0] + f * col[z1])
+
+
+def boundary_delta_profile_x(phi, gl, gr, y_index=None):
+    """:489-499"""
+    xs, zc = solid_interface_curve_xz(phi, y_index)
+    ny = phi.shape[2]
+    y = ny // 2 if y_index is None else int(np.clip(y_index, 0, ny - 1))
+    delta = np.full_like(xs, np.nan)
+    for x in range(phi.shape[1]):
+        delta[x] = sample_column_at_fractional_z(phi[gl, x, y, :] - phi[gr, x, y, :], zc[x])
+    return xs, zc, delta
+
+
+def interpolate_root_from_sign_change(xs, zc, delta, x_guess=None):
+    """:502-528"""
+    pairs = []
+    for i in range(len(xs) - 1):
+        d0, d1 = delta[i], delta[i + 1]
+        if not (np.isfinite(d0) and np.isfinite(d1)):
+            continue
+        if d0 == 0.0:
+            pairs.append((float(xs[i]), float(zc[i])))
+            continue
+        if d0 * d1 > 0.0:
+            continue
+        f = abs(d0) / max(abs(d0) + abs(d1), 1e-12)
+        xr = float(xs[i] + f * (xs[i + 1] - xs[i]))
+        z0, z1 = zc[i], zc[i + 1]
+        zr = float((1.0 - f) * z0 + f * z1) if (np.isfinite(z0) and np.isfinite(z1)) else float("nan")
+        pairs.append((xr, zr))
+    if not pairs:
+        return float("nan"), float("nan")
+    if x_guess is None:
+        return pairs[len(pairs) // 2]
+    return min(pairs, key=lambda q: abs(q[0] - x_guess))
+
+
+def grain_boundary_root_and_trijunction(phi, gl=1, gr=2, y_index=None, x_guess=None):
+    """:531-545"""
+    xs, zc, delta = boundary_delta_profile_x(phi, gl, gr, y_index)
+    rx, rz = interpolate_root_from_sign_change(xs, zc, delta, x_guess)
+    return {"root_x_cells": float(rx), "root_z_cells": float(rz), "trijunction_x_cells": float(rx),
+            "trijunction_z_cells": float(rz)}
+
+
+def fit_interface_line(xs, zc, x_min, x_max):
+    """:548-568"""
+    mask = np.isfinite(zc) & (xs >= x_min) & (xs <= x_max)
+    x, z = xs[mask], zc[mask]
+    if x.size < 2:
+        return {"slope": float("nan"), "intercept": float("nan"), "angle_deg": float("nan"), "rmse": float("nan")}
+    slope, intercept = np.polyfit(x, z, deg=1)
+    fit = slope * x + intercept
+    return {"slope": float(slope), "intercept": float(intercept), "angle_deg": float(math.degrees(math.atan(slope))),
+            "rmse": float(np.sqrt(np.mean((fit - z) ** 2)))}
+
+
+def robust_groove_metrics(phi, root_x_guess, y_index=None, fit_half_width=4.0, exclude_half_width=1.0):
+    """:571-609"""
+    xs, zc = solid_interface_curve_xz(phi, y_index)
+    root = grain_boundary_root_and_trijunction(phi, 1, 2, y_index, root_x_guess)
+    rx, rz = root["root_x_cells"], root["root_z_cells"]
+    if not np.isfinite(rx):
+        rx = float(root_x_guess)
+        i = int(np.clip(round(rx), 0, len(zc) - 1))
+        rz = float(zc[i])
+    left = fit_interface_line(xs, zc, rx - fit_half_width, rx - exclude_half_width)
+    right = fit_interface_line(xs, zc, rx + exclude_half_width, rx + fit_half_width)
+    far_mask = np.isfinite(zc) & ((xs <= rx - fit_half_width) | (xs >= rx + fit_half_width))
+    far = float(np.nanmean(zc[far_mask])) if np.any(far_mask) else float("nan")
+    depth = float(max(far - rz, 0.0)) if (np.isfinite(far) and np.isfinite(rz)) else float("nan")
+    dih = (float(abs(right["angle_deg"] - left["angle_deg"]))
+           if np.isfinite(left["angle_deg"]) and np.isfinite(right["angle_deg"]) else float("nan"))
+    return {"root_x_cells": float(rx), "root_z_cells": float(rz), "trijunction_x_cells": float(rx),
+            "trijunction_z_cells": float(rz), "left_angle_deg": left["angle_deg"],
+            "right_angle_deg": right["angle_deg"], "left_fit_rmse": left["rmse"], "right_fit_rmse": right["rmse"],
+            "groove_depth_cells": depth, "dihedral_angle_like_deg": dih}
+
+
+def finalize_metrics(metrics, phi_initial, phi_final, dx, total_time, root_initial_cells=None,
+                     trijunction_final=None, groove_metrics=None, angle_history_deg=None):
+    """:657-705"""
+    ini = interface_positions_3d(phi_initial)
+    fin = ini[np.isfinite(ini)]
+    ini_mean = float(np.mean(fin) * dx) if fin.size else float("nan")
+    metrics["solid_volume_growth"] = float(np.sum((1.0 - phi_final[0]) - (1.0 - phi_initial[0])) * (dx ** 3))
+    if np.isfinite(metrics.get("interface_position_mean", float("nan"))) and np.isfinite(ini_mean):
+        metrics["front_velocity"] = float((metrics["interface_position_mean"] - ini_mean) / max(total_time, 1e-30))
+    pos = [float(v) for v in metrics.get("grain_volumes", []) if float(v) > 0.5 * (dx ** 3)]
+    if len(pos) >= 2:
+        metrics["winner_loser_volume_ratio"] = float(max(pos) / max(min(pos), 1e-30))
+    if trijunction_final is not None:
+        metrics["trijunction_x"] = float(trijunction_final.get("trijunction_x_cells", float("nan")) * dx)
+        metrics["trijunction_z"] = float(trijunction_final.get("trijunction_z_cells", float("nan")) * dx)
+        if root_initial_cells is not None and np.isfinite(root_initial_cells):
+            metrics["grain_boundary_root_shift"] = float((trijunction_final.get("root_x_cells", float("nan"))
+                                                          - root_initial_cells) * dx)
+    if groove_metrics is not None:
+        a = float(groove_metrics.get("dihedral_angle_like_deg", float("nan")))
+        metrics["interface_angle_deg"] = a
+        metrics["dihedral_angle_like_metric"] = a
+        metrics["trijunction_x"] = float(groove_metrics.get("trijunction_x_cells", float("nan")) * dx)
+        metrics["trijunction_z"] = float(groove_metrics.get("trijunction_z_cells", float("nan")) * dx)
+        if root_initial_cells is not None and np.isfinite(root_initial_cells):
+            metrics["grain_boundary_root_shift"] = float((groove_metrics.get("root_x_cells", float("nan"))
+                                                          - root_initial_cells) * dx)
+    if angle_history_deg:
+        fa = [float(a) for a in angle_history_deg if np.isfinite(a)]
+        if fa and np.isfinite(metrics.get("interface_angle_deg", float("nan"))):
+            metrics["angle_deviation_from_final_state"] = float(np.mean(np.abs(np.array(fa)
+                                                                               - float(metrics["interface_angle_deg"]))))
+    return metrics
+
+
+def blank_metrics():
+    """:708-733"""
+    nan = float("nan")
+    return {"phi_sum_error_max": nan, "phi_min": nan, "phi_max": nan, "nf_max": -1, "nan_count": -1,
+            "solid_fraction": nan, "solid_volume": nan, "grain_volumes": [], "surviving_grains": -1,
+            "interface_position_mean": nan, "interface_position_max": nan, "interface_position_std": nan,
+            "interface_roughness": nan, "interface_angle_deg": nan, "solid_volume_growth": nan,
+            "winner_loser_volume_ratio": nan, "grain_boundary_root_shift": nan, "trijunction_x": nan,
+            "trijunction_z": nan, "dihedral_angle_like_metric": nan, "angle_deviation_from_final_state": nan,
+            "front_velocity": nan, "runtime_s": nan}
+
+
+def classify_signal_level(signal, l1, l2, passed):
+    """:1030-1037"""
+    if not passed or not np.isfinite(signal):
+        return "L0"
+    return "L2" if signal >= l2 else ("L1" if signal >= l1 else "L0")
+
+
+def bicrystal_phi_3d(nx, ny, nz, dx, delta, seed_height, split_index, left_gid=1, right_gid=2, left_offset=0.0,
+                     right_offset=0.0, waviness=0.0):
+    """:1085-1111"""
+    phi = np.zeros((3, nx, ny, nz), np.float32)
+    factor = np.float32(2.2 / delta)
+    x = np.arange(nx, dtype=np.float64).reshape(nx, 1, 1)
+    z = np.arange(nz, dtype=np.float64).reshape(1, 1, nz)
+    base = seed_height + waviness * np.cos(2.0 * math.pi * x / max(nx, 1))
+    ls = (0.5 * (1.0 - np.tanh(factor * ((z - (base + left_offset)) * dx)))).astype(np.float32)
+    rs = (0.5 * (1.0 - np.tanh(factor * ((z - (base + right_offset)) * dx)))).astype(np.float32)
+    phi[left_gid, :split_index] = np.broadcast_to(ls[:split_index], (split_index, ny, nz))
+    phi[right_gid, split_index:] = np.broadcast_to(rs[split_index:], (nx - split_index, ny, nz))
+    phi[0] = np.clip(1.0 - phi[1] - phi[2], 0.0, 1.0)
+    return phi
+
+
+def alternating_multigrain_phi_3d(nx, ny, nz, dx, delta, seed_height, ngrains, waviness=0.0):
+    """:1114-1135"""
+    gm = stripe_grain_map(nx, ny, ngrains)
+    phi = inputs.init_phi_from_grain_map_3d(gm, ngrains, nx, ny, nz, dx, delta, int(seed_height))
+    if waviness == 0.0:
+        return phi
+    factor = np.float32(2.2 / delta)
+    x = np.arange(nx, dtype=np.float64).reshape(nx, 1, 1)
+    z = np.arange(nz, dtype=np.float64).reshape(1, 1, nz)
+    base = seed_height + waviness * np.cos(2.0 * math.pi * x / max(nx, 1))
+    prof = (0.5 * (1.0 - np.tanh(factor * ((z - base) * dx)))).astype(np.float32)
+    for g in range(1, ngrains + 1):
+        phi[g] = np.where((gm == g)[:, :, None], prof, 0.0)
+    phi[0] = np.clip(1.0 - phi[1:].sum(axis=0), 0.0, 1.0)
+    return phi
+
+
+def orientation_library():
+    """:1138-1149"""
+    from scipy.spatial.transform import Rotation
+    return {"identity": Rotation.identity(), "y_15": Rotation.from_euler("y", 15.0, degrees=True),
+            "y_30": Rotation.from_euler("y", 30.0, degrees=True), "y_45": Rotation.from_euler("y", 45.0, degrees=True),
+            "x_30": Rotation.from_euler("x", 30.0, degrees=True),
+            "tilt_xyz": Rotation.from_euler("xyz", [20.0, 35.0, 10.0], degrees=True),
+            "aligned_111_to_z": _rotation("aligned_111")}
+
+
+def bicrystal_group_metrics(phi, fa, fb, dx):
+    """:1366-1381"""
+    vols = np.array([float(phi[g].sum() * (dx ** 3)) for g in range(1, phi.shape[0])])
+    va = float(np.sum([vols[g - 1] for g in fa]))
+    vb = float(np.sum([vols[g - 1] for g in fb]))
+    if va >= vb:
+        return {"volume_a": va, "volume_b": vb, "winner_loser_volume_ratio": float(va / max(vb, 1e-30)),
+                "winner_family": "A"}
+    return {"volume_a": va, "volume_b": vb, "winner_loser_volume_ratio": float(vb / max(va, 1e-30)),
+            "winner_family": "B"}
+
+
+# ------------------------------------------------------------ directional scans
+_SCAN_CACHE: dict = {}
+
+
+def build_directional_scan_case(base, delta_a, ksi, nsteps):
+    """:1157-1163"""
+    return replace(base, nx=16, ny=6, nz=20, nsteps=nsteps, delta_a=delta_a, ksi=ksi, p_round=0.02, G=0.0,
+                   V_pulling=0.0)
+
+
+def main_directional_params(base):
+    """main_directional_low/high_params (:1787-1794)"""
+    return replace(build_directional_scan_case(base, 0.5, 0.05, 8), nx=16, ny=6, nz=24)
+
+
+def single_grain_benchmark_phi(p):
+    """:1166-1168"""
+    d = inputs.derive_constants(p)
+    return wavy_single_phi_3d(p["nx"], p["ny"], p["nz"], p["dx"], d["delta"], 6.5, 1.8, phase_shift=0.3)
+
+
+def run_directional_scan(case_dir, p, undercoolings, anisotropy, torque, include_iso=False, prefix=None):
+    """:1171-1288: every library orientation at every undercooling (plus the
+    isotropic baseline), front velocity per run; rows cached per signature."""
+    key = (tuple(sorted(p.items())), tuple(sorted(undercoolings.items())), anisotropy, torque, include_iso)
+    if key not in _SCAN_CACHE:
+        rows = []
+        phi0 = single_grain_benchmark_phi(p)
+        lib = orientation_library()
+        for lab, dT in undercoolings.items():
+            temp0 = constant_temp((p["nx"], p["ny"], p["nz"]), p["T_melt"] - dT)
+            runs = [(lab, p, anisotropy, torque)]
+            if include_iso:
+                runs.append((f"{lab}_iso", replace(p, delta_a=0.0, ksi=1.0), False, False))
+            for rlab, rp, a, t in runs:
+                for name, rot in lib.items():
+                    r = run_3d_simulation(f"scan_{rlab}_{name}", rp, phi0, temp0, quaternions_from_rotations([rot]), a, t)
+                    m = finalize_metrics(dict(r["metrics"]), phi0, r["phi"], p["dx"], p["dt"] * p["nsteps"])
+                    rows.append({"orientation": name, "driving_label": rlab, "undercooling": dT,
+                                 "interface_mean": m["interface_position_mean"],
+                                 "interface_max": m["interface_position_max"],
+                                 "solid_volume_growth": m["solid_volume_growth"], "front_velocity": m["front_velocity"],
+                                 "runtime_s": m["runtime_s"]})
+        _SCAN_CACHE[key] = {"rows": rows}
+    rows = _SCAN_CACHE[key]["rows"]
+    pre = f"{prefix}_" if prefix else ""
+    with open(os.path.join(case_dir, f"{pre}directional_scan.csv"), "w", encoding="utf-8", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=list(rows[0].keys()))
+        w.writeheader()
+        w.writerows(rows)
+    return _SCAN_CACHE[key]
+
+
+def directional_stats(rows, label):
+    """:1291-1303"""
+    sub = [r for r in rows if r["driving_label"] == label]
+    if not sub:
+        raise RuntimeError(f"No directional-scan rows were found for driving label '{label}'.")
+    best = max(sub, key=lambda r: r["front_velocity"])
+    worst = min(sub, key=lambda r: r["front_velocity"])
+    return {"rows": sub, "best": best, "worst": worst, "spread": float(best["front_velocity"] - worst["front_velocity"])}
+
+
+def directional_row(rows, label, name):
+    """:1306-1316"""
+    for r in rows:
+        if r["driving_label"] == label and r["orientation"] == name:
+            return r
+    raise RuntimeError(f"No directional-scan row for '{label}' and '{name}'.")
+
+
+def groove_observer(split_index):
+    """:1877-1887"""
+    def obs(phi, step):
+        g = robust_groove_metrics(phi, float(split_index))
+        return {"step": int(step), "dihedral_angle_like_deg": g["dihedral_angle_like_deg"],
+                "trijunction_x_cells": g["trijunction_x_cells"], "trijunction_z_cells": g["trijunction_z_cells"]}
+    return obs
+
+
+def run_bicrystal_competition(label, p, left, right, undercooling, anisotropy, torque):
+    """:1951-2013"""
+    d = inputs.derive_constants(p)
+    split = p["nx"] // 2
+    phi0 = bicrystal_phi_3d(p["nx"], p["ny"], p["nz"], p["dx"], d["delta"], 7.0, split, waviness=0.8)
+    temp0 = constant_temp((p["nx"], p["ny"], p["nz"]), p["T_melt"] - undercooling)
+    lib = orientation_library()
+    quats = quaternions_from_rotations([lib[left], lib[right]])
+    root0 = grain_boundary_root_and_trijunction(phi0, 1, 2, x_guess=split)
+    r = run_3d_simulation(label, p, phi0, temp0, quats, anisotropy, torque, sample_every=max(2, p["nsteps"] // 4),
+                          observer=groove_observer(split))
+    groove = robust_groove_metrics(r["phi"], float(split))
+    hist = [smp.get("dihedral_angle_like_deg", float("nan")) for smp in r["samples"]]
+    m = finalize_metrics(dict(r["metrics"]), phi0, r["phi"], p["dx"], p["dt"] * p["nsteps"],
+                         root_initial_cells=root0["root_x_cells"],
+                         trijunction_final=grain_boundary_root_and_trijunction(r["phi"], 1, 2, x_guess=split),
+                         groove_metrics=groove, angle_history_deg=hist)
+    fam = bicrystal_group_metrics(r["phi"], [1], [2], p["dx"])
+    m["winner_loser_volume_ratio"] = fam["winner_loser_volume_ratio"]
+    win_a = fam["winner_family"] == "A"
+    return {"phi0": phi0, "run": r, "metrics": m, "winner_gid": 1 if win_a else 2,
+            "winner_orientation": left if win_a else right, "loser_orientation": right if win_a else left,
+            "family_metrics": fam, "root_initial_cells": root0["root_x_cells"], "split_index": split}
+
+
+def run_multigrain_family_competition(label, p, fa_name, fb_name, undercooling, ngrains, waviness, anisotropy,
+                                      torque):
+    """:1890-1948"""
+    d = inputs.derive_constants(p)
+    phi0 = alternating_multigrain_phi_3d(p["nx"], p["ny"], p["nz"], p["dx"], d["delta"], 7.0, ngrains, waviness)
+    fa = list(range(1, ngrains + 1, 2))
+    fb = list(range(2, ngrains + 1, 2))
+    lib = orientation_library()
+    quats = np.zeros((ngrains + 1, 4))
+    quats[0] = (0.0, 0.0, 0.0, 1.0)
+    for g in fa:
+        quats[g] = lib[fa_name].as_quat()
+    for g in fb:
+        quats[g] = lib[fb_name].as_quat()
+    r = run_3d_simulation(label, p, phi0, constant_temp((p["nx"], p["ny"], p["nz"]), p["T_melt"] - undercooling),
+                          quats, anisotropy, torque)
+    m = finalize_metrics(dict(r["metrics"]), phi0, r["phi"], p["dx"], p["dt"] * p["nsteps"])
+    g = bicrystal_group_metrics(r["phi"], fa, fb, p["dx"])
+    m["winner_loser_volume_ratio"] = g["winner_loser_volume_ratio"]
+    return {"phi0": phi0, "run": r, "metrics": m, "group": g,
+            "winner_orientation": fa_name if g["winner_family"] == "A" else fb_name, "ngrains": ngrains}
+
+
+SCHEDULE = {"low": 0.25, "high": 3.0}          # main_directional_schedule (:1869-1870)
+PAIR = ("aligned_111_to_z", "identity")       # main_bicrystal_pair (:1873-1874)
+
+
+def _bicrystal_params(base, nsteps, delta_a):
+    """main_bicrystal_switch/energy/kinetic_params (:1797-1835)"""
+    return replace(base, nx=18, ny=6, nz=28, nsteps=nsteps, delta_a=delta_a, ksi=0.05 if delta_a == 0.5 else 0.01,
+                   p_round=0.02, G=0.0, V_pulling=0.0)
+
+
+def case_single_grain_preferred_growth_benchmark(case_dir, base, ov):
+    """:2016-2067"""
+    a, t = resolve_toggles(True, False, ov)
+    rows = run_directional_scan(case_dir, main_directional_params(base), {"high": SCHEDULE["high"]}, a, t, True,
+                                "benchmark")["rows"]
+    on, off = directional_stats(rows, "high"), directional_stats(rows, "high_iso")
+    al = directional_row(rows, "high", "aligned_111_to_z")
+    pen = float(on["best"]["front_velocity"] - al["front_velocity"])
+    passed = on["spread"] >= 5.0 * max(off["spread"], 1e-12) and pen >= 5e-2
+    m = blank_metrics()
+    m.update(interface_position_mean=float(on["best"]["interface_mean"]),
+             interface_position_max=float(on["best"]["interface_max"]),
+             solid_volume_growth=float(on["best"]["solid_volume_growth"]),
+             front_velocity=float(on["best"]["front_velocity"]))
+    return make_case_result(
+        "single_grain_preferred_growth_benchmark", passed,
+        f"Single-grain preferred growth spread_on={on['spread']:.2e} m/s vs isotropic spread_off={off['spread']:.2e} m/s; "
+        f"best={on['best']['orientation']}, worst={on['worst']['orientation']}, aligned penalty={pen:.2e} m/s.",
+        "" if passed else "Preferred growth did not separate enough from the isotropic baseline.", m,
+        {"spread_on": on["spread"], "spread_off": off["spread"], "best_orientation": on["best"]["orientation"],
+         "worst_orientation": on["worst"]["orientation"], "best_front_velocity": on["best"]["front_velocity"],
+         "worst_front_velocity": on["worst"]["front_velocity"], "aligned_front_velocity": al["front_velocity"],
+         "aligned_penalty": pen}, case_dir, "main", classify_signal_level(on["spread"], 3e-2, 8e-2, passed))
+
+
+def case_directional_preference_map(case_dir, base, ov):
+    """:2070-2145"""
+    a, t = resolve_toggles(True, False, ov)
+    lo = run_directional_scan(case_dir, main_directional_params(base), {"low": SCHEDULE["low"]}, a, t, True, "low")["rows"]
+    hi = run_directional_scan(case_dir, main_directional_params(base), {"high": SCHEDULE["high"]}, a, t, True,
+                              "high")["rows"]
+    ls, hs = directional_stats(lo, "low"), directional_stats(hi, "high")
+    lis, his = directional_stats(lo, "low_iso"), directional_stats(hi, "high_iso")
+    pl = float(ls["best"]["front_velocity"] - directional_row(lo, "low", "aligned_111_to_z")["front_velocity"])
+    ph = float(hs["best"]["front_velocity"] - directional_row(hi, "high", "aligned_111_to_z")["front_velocity"])
+    passed = (ls["spread"] >= 1.5e-2 and hs["spread"] >= 5e-2 and hs["spread"] > ls["spread"]
+              and lis["spread"] <= 1e-10 and his["spread"] <= 1e-10 and ph > pl + 2e-2)
+    m = blank_metrics()
+    m.update(interface_position_mean=float(hs["best"]["interface_mean"]),
+             interface_position_max=float(hs["best"]["interface_max"]),
+             solid_volume_growth=float(hs["best"]["solid_volume_growth"]),
+             front_velocity=float(hs["best"]["front_velocity"]))
+
+    def part(st, iso, pen):
+        return {"best_orientation": st["best"]["orientation"], "worst_orientation": st["worst"]["orientation"],
+                "spread": st["spread"], "iso_spread": iso["spread"], "aligned_penalty": pen}
+
+    return make_case_result(
+        "directional_preference_map", passed,
+        f"Directional map spreads: low={ls['spread']:.2e}, high={hs['spread']:.2e}; aligned penalty grew "
+        f"{pl:.2e} -> {ph:.2e} m/s.",
+        "" if passed else "Directional preference map did not show a clear anisotropic spread.", m,
+        {"low": part(ls, lis, pl), "high": part(hs, his, ph)}, case_dir, "main",
+        classify_signal_level(hs["spread"], 4e-2, 8e-2, passed))
+
+
+def case_anisotropy_threshold_test(case_dir, base, ov):
+    """:2148-2198"""
+    a, _ = resolve_toggles(True, False, ov)
+    rows = []
+    for da in (0.0, 0.1, 0.2, 0.35, 0.5):
+        p = replace(build_directional_scan_case(base, da, 0.05, 8), nx=16, ny=6, nz=24)
+        phi0 = single_grain_benchmark_phi(p)
+        temp0 = constant_temp((p["nx"], p["ny"], p["nz"]), p["T_melt"] - 3.0)
+        ra = run_3d_simulation(f"thr_id_{da:.2f}", p, phi0, temp0,
+                               quaternions_from_rotations([_rotation("identity")]), a and da > 0.0, False)
+        rb = run_3d_simulation(f"thr_al_{da:.2f}", p, phi0, temp0,
+                               quaternions_from_rotations([_rotation("aligned_111")]), a and da > 0.0, False)
+        ma = finalize_metrics(dict(ra["metrics"]), phi0, ra["phi"], p["dx"], p["dt"] * p["nsteps"])
+        mb = finalize_metrics(dict(rb["metrics"]), phi0, rb["phi"], p["dx"], p["dt"] * p["nsteps"])
+        rows.append({"delta_a": da, "front_velocity_diff": abs(ma["front_velocity"] - mb["front_velocity"])})
+    with open(os.path.join(case_dir, "anisotropy_threshold.csv"), "w", encoding="utf-8", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=list(rows[0].keys()))
+        w.writeheader()
+        w.writerows(rows)
+    mono = all(rows[i + 1]["front_velocity_diff"] >= rows[i]["front_velocity_diff"] - 2e-2 for i in range(len(rows) - 1))
+    passed = rows[0]["front_velocity_diff"] <= 5e-4 and rows[-1]["front_velocity_diff"] >= 7e-2 and mono
+    m = blank_metrics()
+    m["front_velocity"] = float(rows[-1]["front_velocity_diff"])
+    return make_case_result(
+        "anisotropy_threshold_test", passed,
+        f"Directional contrast grew from {rows[0]['front_velocity_diff']:.2e} to {rows[-1]['front_velocity_diff']:.2e} "
+        "as delta_a increased.",
+        "" if passed else "Directional contrast did not grow cleanly with anisotropy strength.", m,
+        {"rows": rows, "monotonic_non_decreasing": mono}, case_dir, "main",
+        classify_signal_level(rows[-1]["front_velocity_diff"], 7e-2, 9e-2, passed))
+
+
+def case_bicrystal_competition_low_high_driving(case_dir, base, ov):
+    """:2201-2257"""
+    p = _bicrystal_params(base, 8, 0.5)
+    a, t = resolve_toggles(True, False, ov)
+    lo_o, hi_o = PAIR
+    lo = run_bicrystal_competition("bicrystal_low", p, lo_o, hi_o, SCHEDULE["low"], a, t)
+    hi = run_bicrystal_competition("bicrystal_high", p, lo_o, hi_o, SCHEDULE["high"], a, t)
+    switched = lo["winner_orientation"] == lo_o and hi["winner_orientation"] == hi_o
+    sig = min(lo["metrics"]["winner_loser_volume_ratio"], hi["metrics"]["winner_loser_volume_ratio"])
+    passed = switched and sig >= 1.01
+    return make_case_result(
+        "bicrystal_competition_low_high_driving", passed,
+        f"Bicrystal winner switched {lo['winner_orientation']} -> {hi['winner_orientation']} between low/high driving.",
+        "" if passed else "Low/high driving did not produce the expected bicrystal winner switch.", dict(hi["metrics"]),
+        {"pair": {"low_orientation": lo_o, "high_orientation": hi_o}, "low_winner": lo["winner_orientation"],
+         "high_winner": hi["winner_orientation"], "low_ratio": lo["metrics"]["winner_loser_volume_ratio"],
+         "high_ratio": hi["metrics"]["winner_loser_volume_ratio"],
+         "low_root_shift": lo["metrics"]["grain_boundary_root_shift"],
+         "high_root_shift": hi["metrics"]["grain_boundary_root_shift"]}, case_dir, "main",
+        classify_signal_level(sig, 1.01, 1.015, passed))
+
+
+def case_bicrystal_interfacial_energy_dominated_regime(case_dir, base, ov):
+    """:2260-2323"""
+    p = _bicrystal_params(base, 14, 0.2)
+    a, t = resolve_toggles(True, False, ov)
+    al, ident = PAIR
+    on = run_bicrystal_competition("bicrystal_low_on", p, al, ident, 0.01, a, t)
+    off = run_bicrystal_competition("bicrystal_low_off", replace(p, delta_a=0.0, ksi=1.0), al, ident, 0.01, False, False)
+    rg = on["metrics"]["winner_loser_volume_ratio"] - off["metrics"]["winner_loser_volume_ratio"]
+    vg = on["metrics"]["front_velocity"] - off["metrics"]["front_velocity"]
+    sg = abs(on["metrics"]["grain_boundary_root_shift"]) - abs(off["metrics"]["grain_boundary_root_shift"])
+    passed = on["winner_orientation"] == al and vg > 5e-3 and rg > 3e-3 and sg > 5e-7
+    return make_case_result(
+        "bicrystal_interfacial_energy_dominated_regime_test", passed,
+        f"Low-driving bicrystal favored {on['winner_orientation']} with front-velocity gain {vg:.2e} m/s and ratio gain "
+        f"{rg:.2e} over anisotropy-off.",
+        "" if passed else "Low-driving bicrystal did not show a clean aligned-facet preference over the anisotropy-off baseline.",
+        dict(on["metrics"]),
+        {"expected_low_winner": al, "winner_on": on["winner_orientation"], "winner_off": off["winner_orientation"],
+         "ratio_on": on["metrics"]["winner_loser_volume_ratio"], "ratio_off": off["metrics"]["winner_loser_volume_ratio"],
+         "front_velocity_on": on["metrics"]["front_velocity"], "front_velocity_off": off["metrics"]["front_velocity"],
+         "front_velocity_gain": vg, "root_shift_on": on["metrics"]["grain_boundary_root_shift"],
+         "root_shift_off": off["metrics"]["grain_boundary_root_shift"], "root_shift_gain": sg}, case_dir, "main",
+        classify_signal_level(vg, 5e-3, 1.2e-2, passed))
+
+
+def case_bicrystal_kinetic_dominated_regime(case_dir, base, ov):
+    """:2326-2385"""
+    p = _bicrystal_params(base, 10, 0.35)
+    a, t = resolve_toggles(True, False, ov)
+    al, ident = PAIR
+    on = run_bicrystal_competition("bicrystal_high_on", p, al, ident, 6.0, a, t)
+    off = run_bicrystal_competition("bicrystal_high_off", replace(p, delta_a=0.0, ksi=1.0), al, ident, 6.0, False, False)
+    rg = on["metrics"]["winner_loser_volume_ratio"] - off["metrics"]["winner_loser_volume_ratio"]
+    sw = on["metrics"]["grain_boundary_root_shift"] * off["metrics"]["grain_boundary_root_shift"] < 0.0
+    passed = on["winner_orientation"] == ident and off["winner_orientation"] == al and rg > 0.02 and sw
+    return make_case_result(
+        "bicrystal_kinetic_dominated_regime_test", passed,
+        f"High-driving bicrystal favored {on['winner_orientation']} with ratio gain {rg:.2e} over the anisotropy-off "
+        "baseline.",
+        "" if passed else "High-driving bicrystal did not show the expected kinetic-dominated winner.",
+        dict(on["metrics"]),
+        {"expected_high_winner": ident, "winner_on": on["winner_orientation"], "winner_off": off["winner_orientation"],
+         "ratio_on": on["metrics"]["winner_loser_volume_ratio"], "ratio_off": off["metrics"]["winner_loser_volume_ratio"],
+         "root_shift_on": on["metrics"]["grain_boundary_root_shift"],
+         "root_shift_off": off["metrics"]["grain_boundary_root_shift"], "root_switch": sw}, case_dir, "main",
+        classify_signal_level(rg, 0.02, 0.05, passed))
+
+
+def case_multigrain_competition_extension(case_dir, base, ov):
+    """:2388-2450"""
+    a, t = resolve_toggles(True, False, ov)
+    al, ident = PAIR
+    lo = run_multigrain_family_competition(
+        "multigrain_low", replace(base, nx=24, ny=10, nz=30, nsteps=22, delta_a=0.2, ksi=0.01, p_round=0.02, G=0.0,
+                                  V_pulling=0.0), al, ident, 0.01, 6, 1.5, a, t)
+    hi = run_multigrain_family_competition(
+        "multigrain_high", replace(base, nx=24, ny=10, nz=28, nsteps=10, delta_a=0.35, ksi=0.01, p_round=0.02, G=0.0,
+                                   V_pulling=0.0), al, ident, 6.0, 6, 1.0, a, t)
+    lr, hr = lo["group"]["winner_loser_volume_ratio"], hi["group"]["winner_loser_volume_ratio"]
+    passed = (lo["winner_orientation"] == al and hi["winner_orientation"] == ident and hr > 1.05
+              and (hr - 1.0) > 10.0 * max(lr - 1.0, 1e-12))
+    return make_case_result(
+        "multigrain_competition_extension", passed,
+        f"Multigrain extension switched dominant family {lo['winner_orientation']} -> {hi['winner_orientation']}; "
+        f"winner ratios were {lr:.4f} and {hr:.4f}.",
+        "" if passed else "Multigrain extension did not preserve the bicrystal family switch trend strongly enough.",
+        dict(hi["metrics"]),
+        {"low_family_name": lo["winner_orientation"], "high_family_name": hi["winner_orientation"],
+         "low_group": lo["group"], "high_group": hi["group"],
+         "low_surviving_grains": lo["metrics"]["surviving_grains"],
+         "high_surviving_grains": hi["metrics"]["surviving_grains"]}, case_dir, "main",
+        classify_signal_level(min(lr, hr), 1.005, 1.02, passed))
+
+
+def _groove_setup(base):
+    p = replace(base, nx=20, ny=6, nz=24, nsteps=20, delta_a=0.0, ksi=1.0, G=0.0, V_pulling=0.0)
+    d = inputs.derive_constants(p)
+    split = p["nx"] // 2
+    phi0 = bicrystal_phi_3d(p["nx"], p["ny"], p["nz"], p["dx"], d["delta"], 7.0, split, waviness=0.6)
+    temp0 = constant_temp((p["nx"], p["ny"], p["nz"]), p["T_melt"])
+    q = quaternions_from_rotations([_rotation("identity"), _rotation("identity")])
+    return p, split, phi0, temp0, q, grain_boundary_root_and_trijunction(phi0, 1, 2, x_guess=split)
+
+
+def case_groove_depth_only(case_dir, base, ov):
+    """:2453-2488"""
+    p, split, phi0, temp0, q, root0 = _groove_setup(base)
+    r = run_3d_simulation("groove_depth_only", p, phi0, temp0, q, False, False)
+    g = robust_groove_metrics(r["phi"], float(split))
+    m = finalize_metrics(dict(r["metrics"]), phi0, r["phi"], p["dx"], p["dt"] * p["nsteps"],
+                         root_initial_cells=root0["root_x_cells"], groove_metrics=g)
+    depth = g["groove_depth_cells"] * p["dx"]
+    passed = bool(np.isfinite(depth) and depth >= 0.10 * p["dx"])
+    return make_case_result(
+        "groove_depth_only", passed, f"Groove depth benchmark gave depth={depth:.2e} m.",
+        "" if passed else "Groove depth remained too small for a reliable capillarity benchmark.", m,
+        {"groove_depth": depth, "groove_metrics": g}, case_dir, "main",
+        classify_signal_level(depth / p["dx"] if np.isfinite(depth) else float("nan"), 0.10, 0.25, passed))
+
+
+def case_robust_groove_angle_estimation(case_dir, base, ov):
+    """:2491-2542"""
+    p, split, phi0, temp0, q, root0 = _groove_setup(base)
+    r = run_3d_simulation("robust_groove_angle", p, phi0, temp0, q, False, False, sample_every=2,
+                          observer=groove_observer(split))
+    g = robust_groove_metrics(r["phi"], float(split))
+    hist = [smp.get("dihedral_angle_like_deg", float("nan")) for smp in r["samples"]]
+    m = finalize_metrics(dict(r["metrics"]), phi0, r["phi"], p["dx"], p["dt"] * p["nsteps"],
+                         root_initial_cells=root0["root_x_cells"], groove_metrics=g, angle_history_deg=hist)
+    rmse = max(g["left_fit_rmse"], g["right_fit_rmse"])
+    passed = bool(np.isfinite(m["interface_angle_deg"]) and rmse <= 0.6)
+    return make_case_result(
+        "robust_groove_angle_estimation", passed,
+        f"Robust groove fit produced dihedral-like angle={m['interface_angle_deg']:.2f} deg with fit_rmse={rmse:.2e} cells.",
+        "" if passed else "Robust groove-angle fit remained numerically unstable.", m,
+        {"groove_metrics": g, "angle_history_deg": hist, "fit_rmse": rmse}, case_dir, "main",
+        classify_signal_level(m["interface_angle_deg"], 2.0, 5.0, passed))
+
 CASES = {
     "basic_constraints": case_basic_constraints,
     "kmax_overflow_detection": case_kmax_overflow_detection,
@@ -657,6 +1256,15 @@ CASES = {
     "grain_competition": case_grain_competition,
     "grain_boundary_groove_trijunction": case_grain_boundary_groove,
     "convergence_grid_dt_delta": case_convergence_sweep,
+    "single_grain_preferred_growth_benchmark": case_single_grain_preferred_growth_benchmark,
+    "directional_preference_map": case_directional_preference_map,
+    "anisotropy_threshold_test": case_anisotropy_threshold_test,
+    "bicrystal_competition_low_high_driving": case_bicrystal_competition_low_high_driving,
+    "bicrystal_interfacial_energy_dominated_regime_test": case_bicrystal_interfacial_energy_dominated_regime,
+    "bicrystal_kinetic_dominated_regime_test": case_bicrystal_kinetic_dominated_regime,
+    "multigrain_competition_extension": case_multigrain_competition_extension,
+    "groove_depth_only": case_groove_depth_only,
+    "robust_groove_angle_estimation": case_robust_groove_angle_estimation,
 }
 
 
