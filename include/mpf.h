@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define MPF_ABI_VERSION 1
+#define MPF_ABI_VERSION 2
 
 enum { MPF_FP32 = 0, MPF_FP64 = 1 };
 enum {
@@ -86,6 +86,9 @@ typedef struct mpf_stats {
     int kernel_steps;               /* steps timed per kernel (mpf_set_kernel_timing) */
     double kernel_ms[4];            /* summed GPU time of the step's kernels over those steps:
                                        classification, merge, interface, many-phase */
+    unsigned long long light_cells; /* cells of the tile planes the classification pass computed,
+                                       summed over steps; the other planes held liquid 1 with liquid
+                                       1 around them and were left unchanged */
 } mpf_stats;
 
 /* ---- resident solver ---------------------------------------------------- */

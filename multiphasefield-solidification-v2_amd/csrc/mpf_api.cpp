@@ -104,7 +104,8 @@ struct mpf_handle {
     size_t nT = 0;
     void *tab = nullptr; // per instance: wij, aij, mij, n111 contiguous
     int *wl = nullptr;        // NWL work lists of ncell entries (mpf_state.h)
-    int *wl_count = nullptr;  // [NWL][2]
+    int *wl_count = nullptr;  // [NWC][2]
+    int *tiles = nullptr;     // tile list of the classification pass (n_tiles entries)
     Stats *stats = nullptr;
     int cur = 0, tcur = 0;
     int step_counter = 0;
@@ -187,6 +188,7 @@ extern "C" void mpf_destroy(mpf_handle *h)
     hipFree(h->tab);
     hipFree(h->wl);
     hipFree(h->wl_count);
+    hipFree(h->tiles);
     hipFree(h->stats);
     hipFree(h->inst_d);
     drop_graphs(h);
@@ -207,7 +209,7 @@ static int reset_stats_dev(mpf_handle *h)
     memset(&s, 0, sizeof(s));
     s.halt_step = NO_HALT;
     HIP_TRY(hipMemcpyAsync(h->stats, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemsetAsync(h->wl_count, 0, 2 * NWL * sizeof(int), h->stream));
+    HIP_TRY(hipMemsetAsync(h->wl_count, 0, 2 * NWC * sizeof(int), h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     return 0;
 }
@@ -300,7 +302,8 @@ static int create_handle(const mpf_config &c, const std::vector<Inst> &grids, in
     for (int i = 0; i < 2; ++i) TRY_RC(dev_alloc(h, (void **)&h->T[i], h->nT * sizeof(double)));
     TRY_RC(dev_alloc(h, &h->tab, inst.size() * h->tab_stride * h->rsz));
     TRY_RC(dev_alloc(h, (void **)&h->wl, (size_t)NWL * ncell * sizeof(int)));
-    TRY_RC(dev_alloc(h, (void **)&h->wl_count, 2 * NWL * sizeof(int)));
+    TRY_RC(dev_alloc(h, (void **)&h->wl_count, 2 * NWC * sizeof(int)));
+    TRY_RC(dev_alloc(h, (void **)&h->tiles, (size_t)h->n_tiles * sizeof(int)));
     TRY_RC(dev_alloc(h, (void **)&h->stats, sizeof(Stats)));
     {
         TRY_RC(dev_alloc(h, (void **)&h->inst_d, inst.size() * sizeof(Inst)));
@@ -628,6 +631,9 @@ static hipError_t launch_one_step(mpf_handle *h, const int *grids, hipEvent_t *t
     a.cooling = c.cooling_rate;
     a.wl_base = h->wl;
     a.wl_count = h->wl_count;
+    a.tiles = h->tiles;
+    a.n_tiles = h->n_tiles;
+    a.n_temp = h->nT;
     a.step = h->step_counter;
     a.halt_mode = c.halt_on_kmax;
     a.tab_class = h->tab_class;
@@ -640,7 +646,6 @@ static hipError_t launch_one_step(mpf_handle *h, const int *grids, hipEvent_t *t
         a.inst = h->inst_d;
         a.n_inst = (int)h->inst.size();
         a.tab_stride = h->tab_stride;
-        a.n_tiles = h->n_tiles;
     }
     hipStream_t side = h->use_side ? h->side : nullptr;
     if (sizeof(R) == 8)
@@ -701,7 +706,7 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
     HIP_TRY(hipStreamSynchronize(h->stream));
     int m1 = NO_HALT;
     HIP_TRY(hipMemcpy(&h->stats->halt_step, &m1, sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(h->wl_count, 0, 2 * NWL * sizeof(int)));
+    HIP_TRY(hipMemset(h->wl_count, 0, 2 * NWC * sizeof(int)));
     const int s0 = h->step_counter, cur0 = h->cur, tcur0 = h->tcur;
     int grids[3];
     step_grids(h, grids);
@@ -868,6 +873,7 @@ extern "C" int mpf_get_stats(mpf_handle *h, mpf_stats *st)
     st->last_step_ms = h->last_ms;
     st->merge_cells = s.merge_cells;
     st->front_cells = s.front_cells;
+    st->light_cells = s.light_cells;
     st->class_tables = h->tab_class;
     for (int j = 0; j < 4; ++j) st->kernel_ms[j] = h->kern_ms[j];
     st->kernel_steps = h->kern_steps;

@@ -177,26 +177,125 @@ __device__ __forceinline__ Where locate(const StepArgs<Real> &a, const uint32_t 
     return w;
 }
 
-// Pass 1 over every cell: temperature, classification and the cells whose
-// stencil holds only liquid (no stored grain at any of the 7 points) or one
-// grain with zero liquid everywhere.  Everything else is queued for k_merge.
+// Batch-wide tile t: the arguments as its instance sees them, the first cell
+// of that instance (batch-wide list entries) and the tile's index inside it.
+template <int B>
+__device__ __forceinline__ int enter_tile(StepArgs<Real> &a, int t, size_t &cell_base)
+{
+    cell_base = 0;
+    if constexpr (B != 0) {
+        int lo = 0, hi = a.n_inst - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if ((int)a.inst[mid].tile_off <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const Inst d = a.inst[lo];
+        instance_args(a, d, lo);
+        a.ntx = (d.nx + LTX - 1) / LTX;
+        a.nty = (d.ny + LTY - 1) / LTY;
+        cell_base = d.cell_off;
+        return t - (int)d.tile_off;
+    }
+    return t;
+}
+
+// Planes of tile (tx, ty, tz) that the classification pass computes (bit p:
+// plane tz*LZC + p), and the occupancy of its own column over planes
+// k0-1 .. k0+LZC in *colmask (bit kk-k0+1).
 //
-// A block owns an LTX x LTY column tile and marches up LZC z-planes.  The
-// cell's own column is carried in registers across planes (z-1, z, z+1); the
-// x/y neighbours of plane z come from an LDS tile with a one-cell halo.  The
-// queued cells of the whole block are collected in LDS and appended to the
-// work list with a single atomic.
+// An occupancy bit that is clear means that every cell of the tile plane
+// holds liquid exactly 1 and no grain.  A cell whose whole stencil is clear
+// becomes liquid 1 again (its liquid phase is active, 1 + dt*0 = 1, 1/1 = 1);
+// when the output buffer's bit for the plane is clear as well, that buffer
+// already holds this result, so the plane is neither read nor written.
+template <int DIM>
+__device__ __forceinline__ uint32_t tile_busy(const StepArgs<Real> &a, int tx, int ty, int tz, uint32_t &colmask)
+{
+    const Consts<Real> &K = a.K;
+    const int nzc = (DIM == 3) ? (K.nz + LZC - 1) / LZC : 1;
+    const int k0 = tz * LZC;
+    const int np = (DIM == 3) ? min(LZC, K.nz - k0) : 1;
+    const int l0 = tx * LTX, m0 = ty * LTY;
+    const int w = min(LTX, K.nx - l0), h = min(LTY, K.ny - m0);
+    auto tile_of = [&](int ttx, int tty, int ttz) { return ((size_t)ttz * a.nty + tty) * a.ntx + ttx; };
+    uint32_t cm = (uint32_t)a.occ_in[tile_of(tx, ty, tz)] << 1;
+    if (DIM == 3 && tz > 0) cm |= ((uint32_t)a.occ_in[tile_of(tx, ty, tz - 1)] >> (LZC - 1)) & 1u;
+    if (DIM == 3 && tz + 1 < nzc) cm |= ((uint32_t)a.occ_in[tile_of(tx, ty, tz + 1)] & 1u) << (LZC + 1);
+    colmask = cm;
+    // tiles holding the x and y halo (x periodic; y periodic in 3D, mirror in 2D)
+    const int xm = bc_pm(l0, K.nx) / LTX, xp = bc_pp(l0 + w - 1, K.nx) / LTX;
+    const int ym = (DIM == 3 ? bc_pm(m0, K.ny) : bc_mm(m0, K.ny)) / LTY;
+    const int yp = (DIM == 3 ? bc_pp(m0 + h - 1, K.ny) : bc_mp(m0 + h - 1, K.ny)) / LTY;
+    const uint32_t nbr = (uint32_t)a.occ_in[tile_of(xm, ty, tz)] | (uint32_t)a.occ_in[tile_of(xp, ty, tz)] |
+                         (uint32_t)a.occ_in[tile_of(tx, ym, tz)] | (uint32_t)a.occ_in[tile_of(tx, yp, tz)];
+    return (cm | (cm >> 1) | (cm >> 2) | nbr | (uint32_t)a.occ_out[tile_of(tx, ty, tz)]) & ((1u << np) - 1u);
+}
+
+// Pass 0: one lane per tile lists the tiles with a plane to compute for
+// k_light, counts their cells, and records that the liquid phase was active
+// on the planes left as they are.
+template <int DIM, int B>
+__global__ __launch_bounds__(256) void k_tiles(StepArgs<Real> a0)
+{
+    if (a0.halt_mode && a0.stats->halt_step < a0.step) return;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    uint32_t busy = 0;
+    bool left = false;
+    unsigned long long cells = 0;
+    if (t < a0.n_tiles) {
+        StepArgs<Real> a = a0;
+        size_t cb;
+        const int lb = enter_tile<B>(a, t, cb);
+        const int tx = lb % a.ntx, ty = (lb / a.ntx) % a.nty, tz = lb / (a.ntx * a.nty);
+        uint32_t colmask;
+        busy = tile_busy<DIM>(a, tx, ty, tz, colmask);
+        const int np = (DIM == 3) ? min(LZC, a.K.nz - tz * LZC) : 1;
+        left = busy != ((1u << np) - 1u);
+        cells = (unsigned long long)__popc(busy) * (unsigned)(min(LTX, a.K.nx - tx * LTX) * min(LTY, a.K.ny - ty * LTY));
+    }
+    const int lane = threadIdx.x & 63;
+    const unsigned long long ball = __ballot(busy != 0u);
+    if (ball != 0ull) {
+        const int leader = __ffsll((long long)ball) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(wl_counter(a0, WC_TILES, a0.step), __popcll(ball));
+        base = __shfl(base, leader, 64);
+        if (busy != 0u) a0.tiles[base + __popcll(ball & ((1ull << lane) - 1ull))] = t;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cells += __shfl_xor(cells, o, 64);
+    if (lane == 0 && cells) atomicAdd(&a0.stats->light_cells, cells);
+    if (__ballot(left) != 0ull && lane == 0 && *((volatile int *)&a0.stats->max_active) < 1)
+        atomicMax(&a0.stats->max_active, 1);
+}
+
+// Pass 1 over the listed tiles: classification and the cells whose stencil
+// holds only liquid (no stored grain at any of the 7 points) or one grain
+// with zero liquid everywhere.  Everything else is queued for k_merge.  The
+// same launch advances the temperature (per plane or per cell).
+//
+// A block takes an LTX x LTY column tile at a time and marches up its LZC
+// z-planes.  The cell's own column is carried in registers across planes
+// (z-1, z, z+1); the x/y neighbours of plane z come from an LDS tile with a
+// one-cell halo.  The queued cells of a tile are collected in LDS and
+// appended to the work list with a single atomic.  The blocks of one XCD
+// take a contiguous part of the tile list (listed in occupancy order, so
+// neighbouring tiles mostly share an L2).
 //
 // The loads of the next plane are issued before the current plane is
 // processed.  Every lane issues the same loads on every plane (lanes outside
 // the grid read a clamped cell, lanes without a halo role read their own
-// cell, and planes whose occupancy bit is clear read slot 0 of the first 64
-// cells, a line that stays cached), so the number of memory operations in
-// flight is static and the compiler waits only for the plane it needs.  The
-// march is unrolled by four so that the column ring (z-1, z, z+1, z+2) and
-// the halo double buffer are fixed registers.  Slot 0 is read together with
-// the count and dropped when the count is zero.  Offsets are cell indices of
-// type Off (32-bit when every byte offset of the arrays fits in 32 bits).
+// cell, and planes whose occupancy bit is clear read the first 64 cells, a
+// line that stays cached, and substitute the pure-liquid cell), so the number
+// of memory operations in flight is static and the compiler waits only for
+// the plane it needs.  The march is unrolled by four so that the column ring
+// (z-1, z, z+1, z+2) and the halo double buffer are fixed registers.  Slot 0
+// is read together with the count and dropped when the count is zero.
+// Offsets are cell indices of type Off (32-bit when every byte offset of the
+// arrays fits in 32 bits).
 struct ColCell {
     uint8_t cnt;
     Real liq;
@@ -215,16 +314,23 @@ __device__ __forceinline__ T &at(T *base, Off i)
         return base[i];
 }
 
-// c: cell for count and liquid; cs: cell whose slot 0 is read
 template <typename Off>
-__device__ __forceinline__ ColCell load_cell(const SlotView<Real> &S, Off c, Off cs)
+__device__ __forceinline__ ColCell load_cell(const SlotView<Real> &S, Off c)
 {
     ColCell r;
     r.cnt = at(S.cnt, c);
     r.liq = at(S.liq, c);
-    r.sid = at(S.sid, cs);
-    r.sv = at(S.sval, cs);
+    r.sid = at(S.sid, c);
+    r.sv = at(S.sval, c);
     return r;
+}
+
+// Cell of a tile plane whose occupancy bit is clear: liquid exactly 1, no
+// grain.  occ: the bit; the loaded values are kept when it is set.
+__device__ __forceinline__ void pristine_unless(ColCell &r, bool occ)
+{
+    r.cnt = occ ? r.cnt : (uint8_t)0;
+    r.liq = occ ? r.liq : Real(1.0);
 }
 
 __device__ __forceinline__ void finish_cell(ColCell &r)
@@ -233,33 +339,11 @@ __device__ __forceinline__ void finish_cell(ColCell &r)
     r.sid = e ? (uint16_t)NO_ID : r.sid;
     r.sv = e ? Real(0.0) : r.sv;
 }
-
-template <int DIM, int TD, typename Off, int B>
+template <int DIM, typename Off, int B>
 __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
 {
     if (a0.halt_mode && a0.stats->halt_step < a0.step) return;
-    if (blockIdx.x == 0 && threadIdx.x < NWL) a0.wl_count[threadIdx.x * 2 + ((a0.step + 1) & 1)] = 0;
-    StepArgs<Real> a = a0;
-    int lb = xcd_remap(blockIdx.x, gridDim.x);
-    size_t cell_base = 0; // first cell of this block's grid (batch-wide list entries)
-    if constexpr (B != 0) {
-        // the instance whose tile range holds this block (uniform per block)
-        int lo = 0, hi = a.n_inst - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if ((int)a.inst[mid].tile_off <= lb)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        const Inst d = a.inst[lo];
-        instance_args(a, d, lo);
-        a.ntx = (d.nx + LTX - 1) / LTX;
-        a.nty = (d.ny + LTY - 1) / LTY;
-        lb -= (int)d.tile_off;
-        cell_base = d.cell_off;
-    }
-    const Consts<Real> &K = a.K;
+    if (blockIdx.x == 0 && threadIdx.x < NWC) a0.wl_count[threadIdx.x * 2 + ((a0.step + 1) & 1)] = 0;
     constexpr int NP = (DIM == 3) ? 7 : 5;
     constexpr int ZC = (DIM == 3) ? LZC : 1;
     // plane tiles, double-buffered so one barrier per plane suffices
@@ -271,216 +355,236 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
     __shared__ int s_qn, s_qbase, s_any;
     __shared__ uint32_t s_occ;
 
-    const int tx = lb % a.ntx;
-    const int rest = lb / a.ntx;
-    const int ty = rest % a.nty;
-    const int tz = rest / a.nty;
-    const int nzc = (DIM == 3) ? (K.nz + LZC - 1) / LZC : 1;
-    const int lx = threadIdx.x % LTX, ly = threadIdx.x / LTX;
-    const int l0 = tx * LTX, m0 = ty * LTY;
-    const int w = min(LTX, K.nx - l0), h = min(LTY, K.ny - m0);
-    const int l = l0 + lx, m = m0 + ly;
-    const bool valid = (lx < w) && (ly < h);
-    const int k0 = tz * ZC;
-    const int k1 = (DIM == 3) ? min(k0 + ZC, K.nz) : 1;
-    const Off plane = (Off)K.nx * (Off)K.ny;
-    if (threadIdx.x == 0) {
-        s_qn = 0;
-        s_any = 0;
-        s_occ = 0;
+    // this block's share of the tile list: XCD x = blockIdx % 8 takes the
+    // x-th eighth, its blocks stride through it
+    const int nlist = *wl_counter(a0, WC_TILES, a0.step);
+    int i0 = blockIdx.x, i1 = nlist, di = gridDim.x;
+    if ((gridDim.x & 7) == 0) {
+        const int x = blockIdx.x & 7;
+        i0 = (int)((long long)nlist * x / 8) + (blockIdx.x >> 3);
+        i1 = (int)((long long)nlist * (x + 1) / 8);
+        di = gridDim.x >> 3;
     }
-    auto tile_of = [&](int ttx, int tty, int ttz) { return ((size_t)ttz * a.nty + tty) * a.ntx + ttx; };
-    // occupancy of the own column over planes k0-1 .. k0+LZC (bit kk-k0+1)
-    uint32_t colmask = (uint32_t)a.occ_in[tile_of(tx, ty, tz)] << 1;
-    if (DIM == 3 && tz > 0) colmask |= ((uint32_t)a.occ_in[tile_of(tx, ty, tz - 1)] >> (LZC - 1)) & 1u;
-    if (DIM == 3 && tz + 1 < nzc) colmask |= ((uint32_t)a.occ_in[tile_of(tx, ty, tz + 1)] & 1u) << (LZC + 1);
-
-    // halo cells this thread loads each plane: x halo of its row, y halo of
-    // its column (x periodic; y periodic in 3D, mirror in 2D)
-    int hxl = -1, hxs = 0;
-    if (ly < h && lx < 2) {
-        hxs = (lx == 0) ? 0 : w + 1;
-        hxl = (lx == 0) ? bc_pm(l0, K.nx) : bc_pp(l0 + w - 1, K.nx);
-    }
-    int hym = -1, hys = 0;
-    if (lx < w && ly < 2) {
-        hys = (ly == 0) ? 0 : h + 1;
-        if (DIM == 3)
-            hym = (ly == 0) ? bc_pm(m0, K.ny) : bc_pp(m0 + h - 1, K.ny);
-        else
-            hym = (ly == 0) ? bc_mm(m0, K.ny) : bc_mp(m0 + h - 1, K.ny);
-    }
-    const uint32_t occx = hxl >= 0 ? a.occ_in[tile_of(hxl / LTX, ty, tz)] : 0u;
-    const uint32_t occy = hym >= 0 ? a.occ_in[tile_of(tx, hym / LTY, tz)] : 0u;
-    // in-plane cell offsets: own (clamped into the grid), halo x, halo y
-    // (lanes without a halo role use their own cell)
-    const Off own_xy = (Off)min(m, K.ny - 1) * (Off)K.nx + (Off)min(l, K.nx - 1);
-    const Off hx_xy = hxl >= 0 ? (Off)m * (Off)K.nx + (Off)hxl : own_xy;
-    const Off hy_xy = hym >= 0 ? (Off)hym * (Off)K.nx + (Off)l : own_xy;
-    const Off dum = (Off)(threadIdx.x & 63) < plane * (Off)(DIM == 3 ? K.nz : 1) ? (Off)(threadIdx.x & 63) : (Off)0;
-
-    auto own_cell = [&](int kk) {
-        const bool occ = (colmask >> (kk - k0 + 1)) & 1u;
-        const Off c = own_xy + (Off)kk * plane;
-        return load_cell<Off>(a.in, c, occ ? c : dum);
-    };
-    auto halo_x = [&](int kk) {
-        const bool occ = (occx >> (kk - k0)) & 1u;
-        const Off c = hx_xy + (Off)kk * plane;
-        return load_cell<Off>(a.in, c, occ ? c : dum);
-    };
-    auto halo_y = [&](int kk) {
-        const bool occ = (occy >> (kk - k0)) & 1u;
-        const Off c = hy_xy + (Off)kk * plane;
-        return load_cell<Off>(a.in, c, occ ? c : dum);
-    };
-    // per-plane temperature: one lane of the tiles at x = y = 0 updates the
-    // planes of its z-range
-    if (!TD && tx == 0 && ty == 0 && threadIdx.x == 0) {
-        for (int k = k0; k < k1; ++k) {
-            const double Tin = a.T_in[k];
-            a.T_out[k] = a.temp_f32 ? (double)((float)Tin - (float)a.cooling) : Tin - a.cooling;
+    for (int li = i0; li < i1; li += di) {
+        StepArgs<Real> a = a0;
+        size_t cell_base;
+        const int lb = enter_tile<B>(a, a0.tiles[li], cell_base);
+        const Consts<Real> &K = a.K;
+        const int tx = lb % a.ntx, ty = (lb / a.ntx) % a.nty, tz = lb / (a.ntx * a.nty);
+        const int lx = threadIdx.x % LTX, ly = threadIdx.x / LTX;
+        const int l0 = tx * LTX, m0 = ty * LTY;
+        const int w = min(LTX, K.nx - l0), h = min(LTY, K.ny - m0);
+        const int l = l0 + lx, m = m0 + ly;
+        const bool valid = (lx < w) && (ly < h);
+        const int k0 = tz * ZC;
+        const int k1 = (DIM == 3) ? min(k0 + ZC, K.nz) : 1;
+        const Off plane = (Off)K.nx * (Off)K.ny;
+        if (threadIdx.x == 0) {
+            s_qn = 0;
+            s_any = 0;
+            s_occ = 0;
         }
-    }
+        auto tile_of = [&](int ttx, int tty, int ttz) { return ((size_t)ttz * a.nty + tty) * a.ntx + ttx; };
+        uint32_t colmask;
+        const uint32_t busy = tile_busy<DIM>(a, tx, ty, tz, colmask);
 
-    // ring[r & 3] holds plane k0 + r - 1 of the own column; hr/yr/tr[r & 1]
-    // hold the halo and temperature of plane k0 + r
-    ColCell ring[4], hr[2], yr[2];
-    double tr[2] = {0.0, 0.0};
-    ring[0] = (DIM == 3) ? own_cell(bc_mm(k0, K.nz)) : ColCell{};
-    ring[1] = own_cell(k0);
-    ring[2] = (DIM == 3) ? own_cell(bc_mp(k0, K.nz)) : ColCell{};
-    ring[3] = ColCell{};
-    hr[0] = halo_x(k0);
-    yr[0] = halo_y(k0);
-    hr[1] = yr[1] = ColCell{};
-    if (TD) tr[0] = at(a.T_in, own_xy + (Off)k0 * plane);
-    finish_cell(ring[0]);
-    finish_cell(ring[1]);
-    uint32_t occ_bits = 0;
+        // halo cells this thread loads each plane: x halo of its row, y halo
+        // of its column (x periodic; y periodic in 3D, mirror in 2D)
+        int hxl = -1, hxs = 0;
+        if (ly < h && lx < 2) {
+            hxs = (lx == 0) ? 0 : w + 1;
+            hxl = (lx == 0) ? bc_pm(l0, K.nx) : bc_pp(l0 + w - 1, K.nx);
+        }
+        int hym = -1, hys = 0;
+        if (lx < w && ly < 2) {
+            hys = (ly == 0) ? 0 : h + 1;
+            if (DIM == 3)
+                hym = (ly == 0) ? bc_pm(m0, K.ny) : bc_pp(m0 + h - 1, K.ny);
+            else
+                hym = (ly == 0) ? bc_mm(m0, K.ny) : bc_mp(m0 + h - 1, K.ny);
+        }
+        const uint32_t occx = hxl >= 0 ? a.occ_in[tile_of(hxl / LTX, ty, tz)] : 0u;
+        const uint32_t occy = hym >= 0 ? a.occ_in[tile_of(tx, hym / LTY, tz)] : 0u;
+        // in-plane cell offsets: own (clamped into the grid), halo x, halo y
+        // (lanes without a halo role use their own cell)
+        const Off own_xy = (Off)min(m, K.ny - 1) * (Off)K.nx + (Off)min(l, K.nx - 1);
+        const Off hx_xy = hxl >= 0 ? (Off)m * (Off)K.nx + (Off)hxl : own_xy;
+        const Off hy_xy = hym >= 0 ? (Off)hym * (Off)K.nx + (Off)l : own_xy;
+        const Off dum =
+            (Off)(threadIdx.x & 63) < plane * (Off)(DIM == 3 ? K.nz : 1) ? (Off)(threadIdx.x & 63) : (Off)0;
 
-    for (int kb = k0; kb < k1; kb += 4) {
+        auto own_cell = [&](int kk) {
+            const bool occ = (colmask >> (kk - k0 + 1)) & 1u;
+            ColCell r = load_cell<Off>(a.in, occ ? own_xy + (Off)kk * plane : dum);
+            pristine_unless(r, occ);
+            return r;
+        };
+        auto halo_x = [&](int kk) {
+            const bool occ = (occx >> (kk - k0)) & 1u;
+            ColCell r = load_cell<Off>(a.in, occ ? hx_xy + (Off)kk * plane : dum);
+            pristine_unless(r, occ);
+            return r;
+        };
+        auto halo_y = [&](int kk) {
+            const bool occ = (occy >> (kk - k0)) & 1u;
+            ColCell r = load_cell<Off>(a.in, occ ? hy_xy + (Off)kk * plane : dum);
+            pristine_unless(r, occ);
+            return r;
+        };
+
+        // ring[r & 3] holds plane k0 + r - 1 of the own column; hr/yr[r & 1]
+        // hold the halo of plane k0 + r
+        ColCell ring[4], hr[2], yr[2];
+        ring[0] = (DIM == 3) ? own_cell(bc_mm(k0, K.nz)) : ColCell{};
+        ring[1] = own_cell(k0);
+        ring[2] = (DIM == 3) ? own_cell(bc_mp(k0, K.nz)) : ColCell{};
+        ring[3] = ColCell{};
+        hr[0] = halo_x(k0);
+        yr[0] = halo_y(k0);
+        hr[1] = yr[1] = ColCell{};
+        finish_cell(ring[0]);
+        finish_cell(ring[1]);
+        uint32_t occ_bits = 0;
+        int nb = 0; // planes computed so far: their parity selects the LDS buffer
+
+        for (int kb = k0; kb < k1; kb += 4) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int k = kb + j;
-            if (k >= k1) break;
-            const int b = j & 1;
-            ColCell &cp = ring[j & 3], &cc = ring[(j + 1) & 3], &cn = ring[(j + 2) & 3];
-            ColCell &hx = hr[j & 1], &hy = yr[j & 1];
-            // loads of the next plane (the last plane reloads itself; unused)
-            if (DIM == 3) {
-                const int kn = min(k + 1, k1 - 1);
-                ring[(j + 3) & 3] = own_cell(bc_mp(kn, K.nz));
-                hr[(j + 1) & 1] = halo_x(kn);
-                yr[(j + 1) & 1] = halo_y(kn);
-                if (TD) tr[(j + 1) & 1] = at(a.T_in, own_xy + (Off)kn * plane);
-            }
-            finish_cell(cn);
-            finish_cell(hx);
-            finish_cell(hy);
-            if (valid) {
-                s_key[b][ly + 1][lx + 1] = (uint32_t)cc.cnt | ((uint32_t)cc.sid << 8);
-                s_liq[b][ly + 1][lx + 1] = cc.liq;
-                s_sv[b][ly + 1][lx + 1] = cc.sv;
-            }
-            if (hxl >= 0) {
-                s_key[b][ly + 1][hxs] = (uint32_t)hx.cnt | ((uint32_t)hx.sid << 8);
-                s_liq[b][ly + 1][hxs] = hx.liq;
-                s_sv[b][ly + 1][hxs] = hx.sv;
-            }
-            if (hym >= 0) {
-                s_key[b][hys][lx + 1] = (uint32_t)hy.cnt | ((uint32_t)hy.sid << 8);
-                s_liq[b][hys][lx + 1] = hy.liq;
-                s_sv[b][hys][lx + 1] = hy.sv;
-            }
-            __syncthreads();
-            bool queue = false, grains = false;
-            if (valid) {
-                const Off c = own_xy + (Off)k * plane;
-                if (TD) {
-                    const double Tc = tr[j & 1];
-                    at(a.T_out, c) = a.temp_f32 ? (double)((float)Tc - (float)a.cooling) : Tc - a.cooling;
+            for (int j = 0; j < 4; ++j) {
+                const int k = kb + j;
+                if (k >= k1) break;
+                ColCell &cp = ring[j & 3], &cc = ring[(j + 1) & 3], &cn = ring[(j + 2) & 3];
+                ColCell &hx = hr[j & 1], &hy = yr[j & 1];
+                // loads of the next plane (the last plane reloads itself; unused)
+                if (DIM == 3) {
+                    const int kn = min(k + 1, k1 - 1);
+                    ring[(j + 3) & 3] = own_cell(bc_mp(kn, K.nz));
+                    hr[(j + 1) & 1] = halo_x(kn);
+                    yr[(j + 1) & 1] = halo_y(kn);
                 }
-                // stencil: c, x+, x-, y+, y-, z+, z-
-                uint32_t pkey[NP];
-                Real pl[NP];
-                Real psv[NP];
-                const int yy = ly + 1, xx = lx + 1;
-                pkey[P_C] = (uint32_t)cc.cnt | ((uint32_t)cc.sid << 8); pl[P_C] = cc.liq; psv[P_C] = cc.sv;
-                pkey[P_XP] = s_key[b][yy][xx + 1]; pl[P_XP] = s_liq[b][yy][xx + 1]; psv[P_XP] = s_sv[b][yy][xx + 1];
-                pkey[P_XM] = s_key[b][yy][xx - 1]; pl[P_XM] = s_liq[b][yy][xx - 1]; psv[P_XM] = s_sv[b][yy][xx - 1];
-                pkey[P_YP] = s_key[b][yy + 1][xx]; pl[P_YP] = s_liq[b][yy + 1][xx]; psv[P_YP] = s_sv[b][yy + 1][xx];
-                pkey[P_YM] = s_key[b][yy - 1][xx]; pl[P_YM] = s_liq[b][yy - 1][xx]; psv[P_YM] = s_sv[b][yy - 1][xx];
-                if constexpr (DIM == 3) {
-                    pkey[P_ZP] = (uint32_t)cn.cnt | ((uint32_t)cn.sid << 8); pl[P_ZP] = cn.liq; psv[P_ZP] = cn.sv;
-                    pkey[P_ZM] = (uint32_t)cp.cnt | ((uint32_t)cp.sid << 8); pl[P_ZM] = cp.liq; psv[P_ZM] = cp.sv;
+                finish_cell(cn);
+                finish_cell(hx);
+                finish_cell(hy);
+                if (!((busy >> (k - k0)) & 1u)) {
+                    // plane left as it is (block-uniform branch); its liquid
+                    // phase was active
+                    if (threadIdx.x == 0) s_any = 1;
+                    continue;
                 }
-                // one grain everywhere: every key equals the centre's, with count 1
-                int sum = 0;
-                bool single = (pkey[P_C] & 0xffu) == 1u;
-#pragma unroll
-                for (int p = 0; p < NP; ++p) {
-                    sum += (int)(pkey[p] & 0xffu);
-                    single = single && (pkey[p] == pkey[P_C]) && (pl[p] == Real(0.0));
+                const int b = nb & 1;
+                ++nb;
+                if (valid) {
+                    s_key[b][ly + 1][lx + 1] = (uint32_t)cc.cnt | ((uint32_t)cc.sid << 8);
+                    s_liq[b][ly + 1][lx + 1] = cc.liq;
+                    s_sv[b][ly + 1][lx + 1] = cc.sv;
                 }
-                grains = sum > 0;
-                if (sum == 0) {
-                    // only liquid in the stencil: active set {L} or empty
-                    bool act = apt_active<NP>(pl);
-                    Real ln, sn;
-                    single_phase_update(pl[P_C], act, true, K.dt, ln, sn);
-                    at(a.out.liq, c) = ln;
-                    at(a.out.cnt, c) = 0;
-                    if (act) s_any = 1;
-                } else if (single) {
-                    // one grain, no liquid: active set {g} or empty
-                    bool act = apt_active<NP>(psv);
-                    Real ln, sn;
-                    single_phase_update(psv[P_C], act, false, K.dt, ln, sn);
-                    at(a.out.liq, c) = ln;
-                    if (sn != Real(0.0)) {
-                        at(a.out.sid, c) = (uint16_t)cc.sid;
-                        at(a.out.sval, c) = sn;
-                        at(a.out.cnt, c) = (uint8_t)1;
-                    } else {
-                        at(a.out.cnt, c) = (uint8_t)0;
+                if (hxl >= 0) {
+                    s_key[b][ly + 1][hxs] = (uint32_t)hx.cnt | ((uint32_t)hx.sid << 8);
+                    s_liq[b][ly + 1][hxs] = hx.liq;
+                    s_sv[b][ly + 1][hxs] = hx.sv;
+                }
+                if (hym >= 0) {
+                    s_key[b][hys][lx + 1] = (uint32_t)hy.cnt | ((uint32_t)hy.sid << 8);
+                    s_liq[b][hys][lx + 1] = hy.liq;
+                    s_sv[b][hys][lx + 1] = hy.sv;
+                }
+                __syncthreads();
+                bool queue = false, dirty = false;
+                if (valid) {
+                    const Off c = own_xy + (Off)k * plane;
+                    // stencil: c, x+, x-, y+, y-, z+, z-
+                    uint32_t pkey[NP];
+                    Real pl[NP];
+                    Real psv[NP];
+                    const int yy = ly + 1, xx = lx + 1;
+                    pkey[P_C] = (uint32_t)cc.cnt | ((uint32_t)cc.sid << 8); pl[P_C] = cc.liq; psv[P_C] = cc.sv;
+                    pkey[P_XP] = s_key[b][yy][xx + 1]; pl[P_XP] = s_liq[b][yy][xx + 1]; psv[P_XP] = s_sv[b][yy][xx + 1];
+                    pkey[P_XM] = s_key[b][yy][xx - 1]; pl[P_XM] = s_liq[b][yy][xx - 1]; psv[P_XM] = s_sv[b][yy][xx - 1];
+                    pkey[P_YP] = s_key[b][yy + 1][xx]; pl[P_YP] = s_liq[b][yy + 1][xx]; psv[P_YP] = s_sv[b][yy + 1][xx];
+                    pkey[P_YM] = s_key[b][yy - 1][xx]; pl[P_YM] = s_liq[b][yy - 1][xx]; psv[P_YM] = s_sv[b][yy - 1][xx];
+                    if constexpr (DIM == 3) {
+                        pkey[P_ZP] = (uint32_t)cn.cnt | ((uint32_t)cn.sid << 8); pl[P_ZP] = cn.liq; psv[P_ZP] = cn.sv;
+                        pkey[P_ZM] = (uint32_t)cp.cnt | ((uint32_t)cp.sid << 8); pl[P_ZM] = cp.liq; psv[P_ZM] = cp.sv;
                     }
-                    if (act) s_any = 1;
-                } else {
-                    queue = true;
+                    // one grain everywhere: every key equals the centre's, with count 1
+                    int sum = 0;
+                    bool single = (pkey[P_C] & 0xffu) == 1u;
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) {
+                        sum += (int)(pkey[p] & 0xffu);
+                        single = single && (pkey[p] == pkey[P_C]) && (pl[p] == Real(0.0));
+                    }
+                    dirty = sum > 0;
+                    if (sum == 0) {
+                        // only liquid in the stencil: active set {L} or empty
+                        bool act = apt_active<NP>(pl);
+                        Real ln, sn;
+                        single_phase_update(pl[P_C], act, true, K.dt, ln, sn);
+                        at(a.out.liq, c) = ln;
+                        at(a.out.cnt, c) = 0;
+                        dirty = ln != Real(1.0);
+                        if (act) s_any = 1;
+                    } else if (single) {
+                        // one grain, no liquid: active set {g} or empty
+                        bool act = apt_active<NP>(psv);
+                        Real ln, sn;
+                        single_phase_update(psv[P_C], act, false, K.dt, ln, sn);
+                        at(a.out.liq, c) = ln;
+                        if (sn != Real(0.0)) {
+                            at(a.out.sid, c) = (uint16_t)cc.sid;
+                            at(a.out.sval, c) = sn;
+                            at(a.out.cnt, c) = (uint8_t)1;
+                        } else {
+                            at(a.out.cnt, c) = (uint8_t)0;
+                        }
+                        if (act) s_any = 1;
+                    } else {
+                        queue = true;
+                    }
+                }
+                // output occupancy: an output cell can only hold a grain where
+                // its stencil held one; a liquid-only stencil may leave liquid
+                // below 1
+                if (__ballot(dirty) != 0ull) occ_bits |= 1u << (k - k0);
+                // append queued cells to the tile's LDS list
+                unsigned long long ball = __ballot(queue);
+                if (ball != 0ull) {
+                    int lane = threadIdx.x & 63;
+                    int leader = __ffsll((long long)ball) - 1;
+                    int base = 0;
+                    if (lane == leader) base = atomicAdd(&s_qn, __popcll(ball));
+                    base = __shfl(base, leader, 64);
+                    if (queue)
+                        s_q[base + __popcll(ball & ((1ull << lane) - 1ull))] =
+                            (uint16_t)(((k - k0) * LTY + ly) * LTX + lx);
                 }
             }
-            // an output cell can only hold a grain where its stencil held one
-            if (__ballot(grains) != 0ull) occ_bits |= 1u << (k - k0);
-            // append queued cells to the block's LDS list
-            unsigned long long ball = __ballot(queue);
-            if (ball != 0ull) {
-                int lane = threadIdx.x & 63;
-                int leader = __ffsll((long long)ball) - 1;
-                int base = 0;
-                if (lane == leader) base = atomicAdd(&s_qn, __popcll(ball));
-                base = __shfl(base, leader, 64);
-                if (queue)
-                    s_q[base + __popcll(ball & ((1ull << lane) - 1ull))] =
-                        (uint16_t)(((k - k0) * LTY + ly) * LTX + lx);
-            }
         }
+        if ((threadIdx.x & 63) == 0 && occ_bits) atomicOr(&s_occ, occ_bits);
+        __syncthreads();
+        __syncthreads();
+        const int nq = s_qn;
+        if (threadIdx.x == 0) {
+            if (s_any && *((volatile int *)&a.stats->max_active) < 1) atomicMax(&a.stats->max_active, 1);
+            s_qbase = nq > 0 ? atomicAdd(wl_counter(a, WL_MERGE, a.step), nq) : 0;
+            a.occ_out[tile_of(tx, ty, tz)] = (uint16_t)s_occ;
+        }
+        __syncthreads();
+        int *dst = wl_list(a, WL_MERGE) + s_qbase;
+        for (int i = threadIdx.x; i < nq; i += LTX * LTY) {
+            const int q = s_q[i];
+            const int qx = q % LTX, qy = (q / LTX) % LTY, qz = q / (LTX * LTY);
+            dst[i] = (int)(cell_base + cidx(l0 + qx, m0 + qy, k0 + qz, K.nx, K.ny));
+        }
+        // the next tile resets the LDS counters and list
+        __syncthreads();
     }
-    if ((threadIdx.x & 63) == 0 && occ_bits) atomicOr(&s_occ, occ_bits);
-    __syncthreads();
-    __syncthreads();
-    const int nq = s_qn;
-    if (threadIdx.x == 0) {
-        if (s_any && *((volatile int *)&a.stats->max_active) < 1) atomicMax(&a.stats->max_active, 1);
-        s_qbase = nq > 0 ? atomicAdd(wl_counter(a, WL_MERGE, a.step), nq) : 0;
-        a.occ_out[tile_of(tx, ty, tz)] = (uint16_t)s_occ;
-    }
-    __syncthreads();
-    int *dst = wl_list(a, WL_MERGE) + s_qbase;
-    for (int i = threadIdx.x; i < nq; i += LTX * LTY) {
-        const int q = s_q[i];
-        const int qx = q % LTX, qy = (q / LTX) % LTY, qz = q / (LTX * LTY);
-        dst[i] = (int)(cell_base + cidx(l0 + qx, m0 + qy, k0 + qz, K.nx, K.ny));
+
+    // temperature of the next state: one value per z-plane or per cell, the
+    // arrays of all instances in one range
+    const size_t nth = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a0.n_temp; i += nth) {
+        const double T = a0.T_in[i];
+        a0.T_out[i] = a0.temp_f32 ? (double)((float)T - (float)a0.cooling) : T - a0.cooling;
     }
 }
 
@@ -1159,22 +1263,14 @@ static void launch_step_kernels(const StepArgs<Real> &a, unsigned grid, const in
 {
     hipStream_t gs = side ? side : st;
     if (tev) hipEventRecord(tev[0], st);
-    // 32-bit offsets when every byte offset of the light pass fits (8-byte
-    // temperature per cell is the widest element it indexes per cell)
+    hipLaunchKernelGGL((k_tiles<DIM, B>), dim3(nblk((size_t)a.n_tiles, 256)), dim3(256), 0, st, a);
+    // 32-bit offsets when every byte offset of the light pass fits (4- or
+    // 8-byte values per cell are the widest elements it indexes)
     const bool off32 = a.in.stride * 8 <= 0xffffffffull;
-    if (B != 0) {
-        if (off32)
-            hipLaunchKernelGGL((k_light<DIM, 0, uint32_t, B>), dim3(grid), dim3(LTX * LTY), 0, st, a);
-        else
-            hipLaunchKernelGGL((k_light<DIM, 0, size_t, B>), dim3(grid), dim3(LTX * LTY), 0, st, a);
-    } else if (a.temp_dense && off32)
-        hipLaunchKernelGGL((k_light<DIM, 1, uint32_t, 0>), dim3(grid), dim3(LTX * LTY), 0, st, a);
-    else if (a.temp_dense)
-        hipLaunchKernelGGL((k_light<DIM, 1, size_t, 0>), dim3(grid), dim3(LTX * LTY), 0, st, a);
-    else if (off32)
-        hipLaunchKernelGGL((k_light<DIM, 0, uint32_t, 0>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+    if (off32)
+        hipLaunchKernelGGL((k_light<DIM, uint32_t, B>), dim3(grid), dim3(LTX * LTY), 0, st, a);
     else
-        hipLaunchKernelGGL((k_light<DIM, 0, size_t, 0>), dim3(grid), dim3(LTX * LTY), 0, st, a);
+        hipLaunchKernelGGL((k_light<DIM, size_t, B>), dim3(grid), dim3(LTX * LTY), 0, st, a);
     if (tev) hipEventRecord(tev[1], st);
     hipLaunchKernelGGL((k_merge<DIM, TC, B>), dim3(grids[0]), dim3(256), 0, st, a);
     if (tev) hipEventRecord(tev[2], st);
@@ -1203,9 +1299,11 @@ hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const 
                                      hipEvent_t *tev)
 {
     StepArgs<Real> a = *args;
+    // persistent classification grid: four blocks per CU of the 256, at most
+    // one per tile
+    const unsigned grid = (unsigned)min(a.n_tiles, 1024);
     if (a.n_inst > 0) {
         // a batch of 3D grids: one launch per kernel covers every instance
-        const unsigned grid = (unsigned)a.n_tiles;
         if (a.tab_class)
             launch_step_kernels<3, 1, 1>(a, grid, grids, st, side, ev_merged, ev_general, tev);
         else
@@ -1214,8 +1312,6 @@ hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const 
     }
     a.ntx = (a.K.nx + LTX - 1) / LTX;
     a.nty = (a.K.ny + LTY - 1) / LTY;
-    const int nzc = dim == 3 ? (a.K.nz + LZC - 1) / LZC : 1;
-    const unsigned grid = (unsigned)a.ntx * a.nty * nzc;
     if (dim == 3) {
         if (a.tab_class)
             launch_step_kernels<3, 1, 0>(a, grid, grids, st, side, ev_merged, ev_general, tev);

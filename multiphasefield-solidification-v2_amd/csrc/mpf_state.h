@@ -33,14 +33,18 @@ constexpr int NO_HALT = 0x7fffffff; // halt_step value while no step has overflo
 constexpr int NFB = 4;
 enum { WL_MERGE = 0, WL_GENERAL = 1, WL_FRONT = 2 };
 constexpr int NWL = WL_FRONT + NFB;
+// Counters of a step: one per work list, then the tile list of k_light.
+constexpr int WC_TILES = NWL;
+constexpr int NWC = NWL + 1;
 
 // Tiles of the classification pass: LTX x LTY cells in x/y, LZC z-planes.
 constexpr int LTX = 64, LTY = 8, LZC = 16;
 
 // Occupancy mask: one uint16 per tile (index (tz*nty + ty)*ntx + tx); bit p
-// is set when plane tz*LZC + p of the tile may hold a stored grain.  A clear
-// bit guarantees cnt == 0 on every cell of that tile plane.  Each state
-// buffer has its own mask; states written by the host get all bits set.
+// is set when plane tz*LZC + p of the tile may hold a stored grain or liquid
+// other than 1.  A clear bit guarantees cnt == 0 and liquid exactly 1 on
+// every cell of that tile plane.  Each state buffer has its own mask; states
+// written by the host get all bits set.
 inline size_t occ_words(int nx, int ny, int nz, int dim)
 {
     size_t ntx = (size_t)(nx + LTX - 1) / LTX, nty = (size_t)(ny + LTY - 1) / LTY;
@@ -66,6 +70,7 @@ struct Stats {
     int pad;
     unsigned long long merge_cells;    // cells queued for the merge pass (summed over steps)
     unsigned long long front_cells;    // solid-liquid interface cells (summed over steps)
+    unsigned long long light_cells;    // cells of the tile planes the classification pass computed
 };
 
 // One instance of a batch: its grid and where its cells, classification
@@ -93,14 +98,17 @@ struct StepArgs {
     int temp_dense;  // 1: temperature per cell; 0: one value per z-plane
     int temp_f32;    // 1: the temperature is a float32 quantity
     double cooling;
-    // Work lists of one step (cell indices).  A step runs four kernels: the
-    // classification pass over all cells finishes cells whose stencil holds
+    // Work lists of one step (cell indices).  A step runs five kernels: the
+    // tile pass lists the tiles with planes that can change; the
+    // classification pass over their cells finishes cells whose stencil holds
     // only liquid or only one grain and queues the rest (WL_MERGE); the merge
     // pass finishes cells that need no anisotropy terms and queues
     // solid-liquid interface cells (WL_FRONT) and cells with more than the
     // register capacity of active grains (WL_GENERAL).
     int *wl_base;    // NWL lists of ncell entries each, list i at wl_base + i*ncell
-    int *wl_count;   // [NWL][2] counters, ping-pong on the step parity
+    int *wl_count;   // [NWC][2] counters, ping-pong on the step parity
+    int *tiles;      // tiles with a plane to compute (k_tiles -> k_light), n_tiles entries
+    size_t n_temp;   // temperature entries of all instances (planes, or cells when temp_dense)
     int step;        // step index (for the ping-pong counters and halt)
     int halt_mode;   // stop at the first KMAX overflow
     int tab_class;   // 1: the tables have the class form (K.cls holds the values)
