@@ -13,13 +13,14 @@
 #include "mpf_math.h"
 #include "mpf_consts.h"
 #include "mpf_state.h"
+#include "mpf_streams.h"
 
 using namespace mpf;
 
 // ------------------------------------------------ launchers of both builds
 #define MPF_DECLARE_LAUNCHERS(R, SFX)                                                                    \
-    extern "C" hipError_t mpf_launch_step##SFX(const StepArgs<R> *, int, const int *, hipStream_t, hipStream_t, \
-                                               hipEvent_t, hipEvent_t, hipEvent_t *);                       \
+    extern "C" hipError_t mpf_launch_step##SFX(const StepArgs<R> *, int, const int *, hipStream_t,           \
+                                               const StepStreams *, hipEvent_t *);                          \
     extern "C" hipError_t mpf_launch_apt_slots##SFX(const Consts<R> *, SlotView<R>, int, int32_t *,      \
                                                     int32_t *, int32_t *, int, hipStream_t);             \
     extern "C" hipError_t mpf_launch_pack##SFX(const void *, int, int, int, int, int, SlotView<R>, int,  \
@@ -94,7 +95,8 @@ struct mpf_handle {
     std::vector<double> cls_vals;   // per instance: 6 class values
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;            // many-phase kernel, concurrent with the interface kernel
-    hipEvent_t ev_merged = nullptr, ev_general = nullptr;
+    hipStream_t side2 = nullptr;           // interface cells with 2-4 solids, concurrent as well
+    hipEvent_t ev_merged = nullptr, ev_general = nullptr, ev_front = nullptr;
     int kernel_timing = 0;                 // per-kernel events (mpf_set_kernel_timing)
     std::vector<hipEvent_t> tev;           // 6 per step of the current call
     double kern_ms[4] = {0, 0, 0, 0};      // k_light, k_merge, k_front, k_step_general
@@ -193,10 +195,13 @@ extern "C" void mpf_destroy(mpf_handle *h)
     hipFree(h->inst_d);
     drop_graphs(h);
     if (h->side) hipStreamSynchronize(h->side);
+    if (h->side2) hipStreamSynchronize(h->side2);
     for (hipEvent_t e : h->tev) hipEventDestroy(e);
     if (h->ev_merged) hipEventDestroy(h->ev_merged);
     if (h->ev_general) hipEventDestroy(h->ev_general);
+    if (h->ev_front) hipEventDestroy(h->ev_front);
     if (h->side) hipStreamDestroy(h->side);
+    if (h->side2) hipStreamDestroy(h->side2);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->stream) hipStreamDestroy(h->stream);
@@ -325,9 +330,10 @@ static int create_handle(const mpf_config &c, const std::vector<Inst> &grids, in
         }
     }
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
-        hipStreamCreate(&h->side) != hipSuccess ||
+        hipStreamCreate(&h->side) != hipSuccess || hipStreamCreate(&h->side2) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_merged, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&h->ev_general, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&h->ev_general, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_front, hipEventDisableTiming) != hipSuccess) {
         mpf_destroy(h);
         return fail(MPF_EHIP, "hipEventCreate failed");
     }
@@ -647,12 +653,10 @@ static hipError_t launch_one_step(mpf_handle *h, const int *grids, hipEvent_t *t
         a.n_inst = (int)h->inst.size();
         a.tab_stride = h->tab_stride;
     }
-    hipStream_t side = h->use_side ? h->side : nullptr;
+    StepStreams ss{h->use_side ? h->side : nullptr, h->side2, h->ev_merged, h->ev_general, h->ev_front};
     if (sizeof(R) == 8)
-        return mpf_launch_step_f64((const StepArgs<double> *)&a, c.dim, grids, h->stream, side, h->ev_merged,
-                                   h->ev_general, tev);
-    return mpf_launch_step_f32((const StepArgs<float> *)&a, c.dim, grids, h->stream, side, h->ev_merged,
-                               h->ev_general, tev);
+        return mpf_launch_step_f64((const StepArgs<double> *)&a, c.dim, grids, h->stream, &ss, tev);
+    return mpf_launch_step_f32((const StepArgs<float> *)&a, c.dim, grids, h->stream, &ss, tev);
 }
 
 // Launch shape of the work-list kernels: the requested block counts, or by
@@ -735,6 +739,7 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
     h->last_ms = ms;
     if (h->kernel_timing) {
         HIP_TRY(hipStreamSynchronize(h->side));
+        HIP_TRY(hipStreamSynchronize(h->side2));
         if (!h->use_side) HIP_TRY(hipStreamSynchronize(h->stream));
         for (int i = 0; i < nsteps; ++i) {
             hipEvent_t *t = h->tev.data() + (size_t)i * 6;

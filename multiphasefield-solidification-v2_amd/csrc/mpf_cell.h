@@ -533,10 +533,15 @@ struct Cell {
 // Cells with many active phases (A large) keep the arrays in scratch memory.
 // ANISO_TERMS = false compiles the anisotropy terms out; the caller then
 // guarantees the cell has no active solid-liquid pair or anisotropy is off.
-template <typename Real, int DIM, int A, bool ANISO_TERMS = true, int TC = 0, typename LiqAt>
+// EXACT = true: the caller guarantees an active liquid and exactly A active
+// solids, so every presence test folds at compile time.
+template <typename Real, int DIM, int A, bool ANISO_TERMS = true, int TC = 0, bool EXACT = false,
+          typename LiqAt>
 MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, int k,
                                             double Tcell, Cell<Real, DIM, A> &S)
 {
+    const bool s_liq = EXACT ? true : S.liq;
+    const int s_nsol = EXACT ? A : S.nsol;
     constexpr int NA = A + 1;
     constexpr int NP = Cell<Real, DIM, A>::NP;
     constexpr bool SMALL = (A <= 8);
@@ -548,9 +553,9 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
         if constexpr (SMALL)
             return NA;
         else
-            return S.nsol + 1;
+            return s_nsol + 1;
     }();
-    const int n_act = (S.liq ? 1 : 0) + S.nsol;
+    const int n_act = (s_liq ? 1 : 0) + s_nsol;
     Real lap_sl[NA], b_sl[NA], lap[NA];
 #pragma unroll UNR_NA
     for (int t = 0; t < TMAX; ++t) {
@@ -572,7 +577,7 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
     // Per-solid terms are read only in solid-liquid pairs, so they are needed
     // only when the liquid is active (the reference computes them regardless).
     Real w_sl = K.w0_sl;
-    if (S.liq && S.nsol > 0) {
+    if (s_liq && s_nsol > 0) {
         // The dominant solid's gradient is taken first so that the neighbour
         // values of the other phases are dead before the liquid context.
         // dominant solid and anisotropic w_sl (:579-595; 2D :410-436)
@@ -583,7 +588,7 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
                 Real maxv = Real(-1.0);
 #pragma unroll UNR_A
                 for (int t = 1; t < NA; ++t) {
-                    if (t > S.nsol) break;
+                    if (t > s_nsol) break;
                     if (S.v[t][P_C] > maxv) {
                         maxv = S.v[t][P_C];
                         ts = t;
@@ -619,7 +624,7 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
             Real maxv = Real(0.0);
 #pragma unroll UNR_A
             for (int t = 1; t < NA; ++t) {
-                if (t > S.nsol) break;
+                if (t > s_nsol) break;
                 if (S.v[t][P_C] > maxv) {
                     maxv = S.v[t][P_C];
                     ts = t;
@@ -667,7 +672,7 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
 #pragma unroll 1
                     for (int it = 0; it < A; ++it) {
                         Real ls = Real(0.0), bs = Real(1.0);
-                        if (it < S.nsol) solid_liquid_terms<Real, DIM>(K, X, gq[0], ls, bs);
+                        if (it < s_nsol) solid_liquid_terms<Real, DIM>(K, X, gq[0], ls, bs);
 #pragma unroll
                         for (int u = 0; u < A - 1; ++u) {
                             gq[u] = gq[u + 1];
@@ -680,13 +685,13 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
                     }
 #pragma unroll
                     for (int u = 0; u < A; ++u) {
-                        if (u < S.nsol) {
+                        if (u < s_nsol) {
                             lap_sl[u + 1] = lq[u];
                             b_sl[u + 1] = bq[u];
                         }
                     }
                 } else {
-                    for (int t = 1; t <= S.nsol; ++t) {
+                    for (int t = 1; t <= s_nsol; ++t) {
                         Real ls, bs;
                         solid_liquid_terms<Real, DIM>(K, X, S.gid[t], ls, bs);
                         lap_sl[t] = ls;
@@ -698,7 +703,7 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
             // anisotropy off (switchable kernel, :729-731): eps0^2 * lap(phi_L)
 #pragma unroll UNR_A
             for (int t = 1; t < NA; ++t) {
-                if (t > S.nsol) break;
+                if (t > s_nsol) break;
                 lap_sl[t] = K.eps0_sq * lap[0];
                 b_sl[t] = Real(1.0);
             }
@@ -713,13 +718,13 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
 #pragma unroll UNR_NA
     for (int t1 = 0; t1 < TMAX; ++t1) {
         S.out[t1] = Real(0.0);
-        const bool p1 = (t1 == 0) ? S.liq : (t1 <= S.nsol);
+        const bool p1 = (t1 == 0) ? s_liq : (t1 <= s_nsol);
         if (!p1) continue;
         const int i = S.gid[t1];
         Real dpi = Real(0.0);
 #pragma unroll UNR_NA
         for (int t2 = 0; t2 < TMAX; ++t2) {
-            const bool p2 = (t2 == 0) ? S.liq : (t2 <= S.nsol);
+            const bool p2 = (t2 == 0) ? s_liq : (t2 <= s_nsol);
             if (!p2 || t2 == t1) continue;
             const int j = S.gid[t2];
             Real driving_force = Real(0.0);
@@ -730,7 +735,7 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
             Real ppp = Real(0.0);
 #pragma unroll UNR_NA
             for (int t3 = 0; t3 < TMAX; ++t3) {
-                const bool p3 = (t3 == 0) ? S.liq : (t3 <= S.nsol);
+                const bool p3 = (t3 == 0) ? s_liq : (t3 <= s_nsol);
                 if (!p3) continue;
                 const int pk = S.gid[t3];
                 const Real phi_k = S.v[t3][P_C];
@@ -771,7 +776,7 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
     Real s = Real(0.0);
 #pragma unroll UNR_NA
     for (int t = 0; t < TMAX; ++t) {
-        const bool pr = (t == 0) ? S.liq : (t <= S.nsol);
+        const bool pr = (t == 0) ? s_liq : (t <= s_nsol);
         if (!pr) continue;
         Real v = S.out[t];
         if (v < Real(0.0)) v = Real(0.0);
@@ -783,7 +788,7 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
         Real invs = Real(1.0) / s;
 #pragma unroll UNR_NA
         for (int t = 0; t < TMAX; ++t) {
-            const bool pr = (t == 0) ? S.liq : (t <= S.nsol);
+            const bool pr = (t == 0) ? s_liq : (t <= s_nsol);
             if (pr) S.out[t] *= invs;
         }
     } else {

@@ -27,6 +27,7 @@
 
 #include "mpf_gather.h"
 #include "mpf_state.h"
+#include "mpf_streams.h"
 
 #ifndef MPF_REAL
 #error "MPF_REAL must be float or double"
@@ -351,9 +352,15 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
     __shared__ uint32_t s_key[2][LTY + 2][LTX + 2];
     __shared__ Real s_liq[2][LTY + 2][LTX + 2];
     __shared__ Real s_sv[2][LTY + 2][LTX + 2];
-    __shared__ uint16_t s_q[LTX * LTY * ZC]; // queued cells as (z-plane, row, column) in the tile
-    __shared__ int s_qn, s_qbase, s_any;
+    // queued cells of a tile as (z-plane, row, column): the merge list grows
+    // from the start, the interface list (entries tagged with their bucket
+    // above bit 13) from the end; a cell enters at most one of them
+    constexpr int QCAP = LTX * LTY * ZC;
+    __shared__ uint16_t s_q[QCAP];
+    __shared__ int s_qn, s_qbase, s_any, s_fn;
+    __shared__ int s_bn[NFB], s_fbase[NFB], s_wc[NFB * LTX * LTY / 64];
     __shared__ uint32_t s_occ;
+    const bool need_terms = a0.K.aniso || DIM == 2;
 
     // this block's share of the tile list: XCD x = blockIdx % 8 takes the
     // x-th eighth, its blocks stride through it
@@ -381,9 +388,11 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
         const Off plane = (Off)K.nx * (Off)K.ny;
         if (threadIdx.x == 0) {
             s_qn = 0;
+            s_fn = 0;
             s_any = 0;
             s_occ = 0;
         }
+        if (threadIdx.x < NFB) s_bn[threadIdx.x] = 0;
         auto tile_of = [&](int ttx, int tty, int ttz) { return ((size_t)ttz * a.nty + tty) * a.ntx + ttx; };
         uint32_t colmask;
         const uint32_t busy = tile_busy<DIM>(a, tx, ty, tz, colmask);
@@ -489,6 +498,7 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
                 }
                 __syncthreads();
                 bool queue = false, dirty = false;
+                int fb = -1; // interface bucket
                 if (valid) {
                     const Off c = own_xy + (Off)k * plane;
                     // stencil: c, x+, x-, y+, y-, z+, z-
@@ -538,44 +548,120 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
                         }
                         if (act) s_any = 1;
                     } else {
-                        queue = true;
+                        // A solid-liquid interface cell whose stencil stores at
+                        // most one grain per point, each with a positive value,
+                        // has exactly the distinct stored grains as active
+                        // solids; within the register capacity and KMAX it goes
+                        // straight to the interface pass, in the bucket of its
+                        // solid count (the merge pass would route it there).
+                        int nd = 0;
+                        bool direct = need_terms && apt_active<NP>(pl);
+#pragma unroll
+                        for (int p = 0; p < NP; ++p) {
+                            const uint32_t cp = pkey[p] & 0xffu;
+                            direct = direct && (cp == 0u || (cp == 1u && psv[p] > Real(0.0)));
+                            bool first = cp == 1u;
+#pragma unroll
+                            for (int q = 0; q < p; ++q)
+                                first = first && !((pkey[q] & 0xffu) == 1u && pkey[q] == pkey[p]);
+                            nd += first ? 1 : 0;
+                        }
+                        if (direct && nd <= NFB && nd + 1 <= K.kmax)
+                            fb = nd - 1;
+                        else
+                            queue = true;
                     }
                 }
                 // output occupancy: an output cell can only hold a grain where
                 // its stencil held one; a liquid-only stencil may leave liquid
                 // below 1
                 if (__ballot(dirty) != 0ull) occ_bits |= 1u << (k - k0);
-                // append queued cells to the tile's LDS list
+                // append queued cells to the tile's LDS lists
+                const int lane = threadIdx.x & 63;
+                const unsigned long long lt = (1ull << lane) - 1ull;
+                const uint16_t qe = (uint16_t)(((k - k0) * LTY + ly) * LTX + lx);
                 unsigned long long ball = __ballot(queue);
                 if (ball != 0ull) {
-                    int lane = threadIdx.x & 63;
                     int leader = __ffsll((long long)ball) - 1;
                     int base = 0;
                     if (lane == leader) base = atomicAdd(&s_qn, __popcll(ball));
                     base = __shfl(base, leader, 64);
-                    if (queue)
-                        s_q[base + __popcll(ball & ((1ull << lane) - 1ull))] =
-                            (uint16_t)(((k - k0) * LTY + ly) * LTX + lx);
+                    if (queue) s_q[base + __popcll(ball & lt)] = qe;
+                }
+                ball = __ballot(fb >= 0);
+                if (ball != 0ull) {
+                    int leader = __ffsll((long long)ball) - 1;
+                    int base = 0;
+                    if (lane == leader) base = atomicAdd(&s_fn, __popcll(ball));
+                    base = __shfl(base, leader, 64);
+                    if (fb >= 0) s_q[QCAP - 1 - (base + __popcll(ball & lt))] = (uint16_t)(qe | (fb << 13));
+#pragma unroll
+                    for (int q = 0; q < NFB; ++q) {
+                        const unsigned long long bq = __ballot(fb == q);
+                        if (lane == 0 && bq != 0ull) atomicAdd(&s_bn[q], __popcll(bq));
+                    }
                 }
             }
         }
         if ((threadIdx.x & 63) == 0 && occ_bits) atomicOr(&s_occ, occ_bits);
         __syncthreads();
         __syncthreads();
-        const int nq = s_qn;
+        const int nq = s_qn, nf = s_fn;
         if (threadIdx.x == 0) {
             if (s_any && *((volatile int *)&a.stats->max_active) < 1) atomicMax(&a.stats->max_active, 1);
             s_qbase = nq > 0 ? atomicAdd(wl_counter(a, WL_MERGE, a.step), nq) : 0;
             a.occ_out[tile_of(tx, ty, tz)] = (uint16_t)s_occ;
         }
-        __syncthreads();
-        int *dst = wl_list(a, WL_MERGE) + s_qbase;
-        for (int i = threadIdx.x; i < nq; i += LTX * LTY) {
-            const int q = s_q[i];
-            const int qx = q % LTX, qy = (q / LTX) % LTY, qz = q / (LTX * LTY);
-            dst[i] = (int)(cell_base + cidx(l0 + qx, m0 + qy, k0 + qz, K.nx, K.ny));
+        if (threadIdx.x >= 64 && threadIdx.x < 64 + NFB) {
+            const int q = threadIdx.x - 64;
+            s_fbase[q] = s_bn[q] > 0 ? atomicAdd(wl_counter(a, WL_FRONT + q, a.step), s_bn[q]) : 0;
         }
-        // the next tile resets the LDS counters and list
+        __syncthreads();
+        auto cell_of = [&](int q) {
+            q &= (1 << 13) - 1;
+            const int qx = q % LTX, qy = (q / LTX) % LTY, qz = q / (LTX * LTY);
+            return (int)(cell_base + cidx(l0 + qx, m0 + qy, k0 + qz, K.nx, K.ny));
+        };
+        int *dst = wl_list(a, WL_MERGE) + s_qbase;
+        for (int i = threadIdx.x; i < nq; i += LTX * LTY) dst[i] = cell_of(s_q[i]);
+        // interface cells, compacted into their buckets: per chunk of the
+        // list, each wave counts its entries per bucket, and an entry's place
+        // is the bucket's base, the entries of the chunks before and of the
+        // waves before, and its rank in its wave (block-uniform loop)
+        constexpr int NW = LTX * LTY / 64;
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        int run[NFB];
+#pragma unroll
+        for (int q = 0; q < NFB; ++q) run[q] = s_fbase[q];
+        for (int i0 = 0; i0 < nf; i0 += LTX * LTY) {
+            const int i = i0 + threadIdx.x;
+            const int e = i < nf ? (int)s_q[QCAP - 1 - i] : 0;
+            const int mq = i < nf ? (e >> 13) : -1;
+            unsigned long long mb = 0ull;
+#pragma unroll
+            for (int q = 0; q < NFB; ++q) {
+                const unsigned long long bq = __ballot(mq == q);
+                if (mq == q) mb = bq;
+                if (lane == 0) s_wc[q * NW + wave] = __popcll(bq);
+            }
+            __syncthreads();
+            int off = 0;
+#pragma unroll
+            for (int q = 0; q < NFB; ++q) {
+                int o = run[q], tot = 0;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) {
+                    const int v = s_wc[q * NW + w];
+                    o += w < wave ? v : 0;
+                    tot += v;
+                }
+                if (mq == q) off = o;
+                run[q] += tot;
+            }
+            if (mq >= 0) wl_list(a, WL_FRONT + mq)[off + __popcll(mb & ((1ull << lane) - 1ull))] = cell_of(e);
+            __syncthreads();
+        }
+        // the next tile resets the LDS counters and lists
         __syncthreads();
     }
 
@@ -662,51 +748,85 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
     }
 }
 
-// Pass 3: solid-liquid interface cells, compacted so that every lane of a
-// wavefront evaluates the anisotropy terms.
+// Pass 3: solid-liquid interface cells by bucket: the cells of bucket
+// NS - 1 have an active liquid and exactly NS active solids (both the
+// classification and the merge pass bucket by that count), so their update is
+// compiled for that set with no presence tests and only the registers it
+// needs.  k_front1 takes the one-solid bucket (nearly all interface cells);
+// k_front_multi the buckets of 2, 3 and 4 solids (triple lines), in block
+// ranges of one launch, concurrently with k_front1.
+static_assert(NFB == FAST_A, "interface buckets cover the register path's solid counts");
+template <int DIM, int TC, int B, int NS>
+__device__ __forceinline__ void front_bucket(const StepArgs<Real> &a, const uint32_t *s_coff, int blk, int nblk)
+{
+    const Consts<Real> &K = a.K;
+    const int n = *wl_counter(a, WL_FRONT + NS - 1, a.step);
+    const int *list = wl_list(a, WL_FRONT + NS - 1);
+    const size_t ncell = a.in.stride; // slot stride
+    if (blk == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->front_cells, (unsigned long long)n);
+    for (int i = blk * blockDim.x + threadIdx.x; i < n; i += nblk * blockDim.x) {
+        const size_t c = (size_t)list[i];
+        const Where W = locate<B>(a, s_coff, c);
+        Cell<Real, DIM, NS> C;
+        int ntot;
+        bool fits;
+        gather<Real, DIM, NS>(W.K, W.in, ncell, W.l, W.m, W.k, C, &ntot, &fits);
+        note_active(a.stats, ntot, K.kmax, a.step, a.halt_mode);
+        size_t ti = a.temp_dense ? W.c : (size_t)W.k;
+        SlotLiq<Real> L{W.in.liq, W.K.nx, W.K.ny};
+        cell_update<Real, DIM, NS, true, TC, true>(W.K, L, W.l, W.m, W.k, W.T[ti], C);
+        write_cell<Real, DIM, NS>(W.out, ncell, W.c, C);
+    }
+}
+
 template <int DIM, int TC, int B>
-__global__ __launch_bounds__(128) void k_front(StepArgs<Real> a)
+__global__ __launch_bounds__(128) void k_front1(StepArgs<Real> a)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
     __shared__ uint32_t s_coff[B ? MAX_INST : 1];
     load_cell_offsets<B>(a, s_coff);
-    const Consts<Real> &K = a.K;
-    // the buckets are processed as one sequence, fewest solids first
-    int nb[NFB];
-    int n = 0;
-#pragma unroll
-    for (int b = 0; b < NFB; ++b) {
-        nb[b] = *wl_counter(a, WL_FRONT + b, a.step);
-        n += nb[b];
-    }
-    const size_t ncell = a.in.stride; // slot stride
-    if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->front_cells, (unsigned long long)n);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        int b = 0, off = i;
-#pragma unroll
-        for (int q = 0; q < NFB - 1; ++q) {
-            if (b == q && off >= nb[q]) {
-                off -= nb[q];
-                b = q + 1;
-            }
-        }
-        const size_t c = (size_t)wl_list(a, WL_FRONT + b)[off];
-        const Where W = locate<B>(a, s_coff, c);
-        Cell<Real, DIM, FAST_A> C;
-        int ntot;
-        bool fits;
-        gather<Real, DIM, FAST_A>(W.K, W.in, ncell, W.l, W.m, W.k, C, &ntot, &fits);
-        note_active(a.stats, ntot, K.kmax, a.step, a.halt_mode);
-        size_t ti = a.temp_dense ? W.c : (size_t)W.k;
-        SlotLiq<Real> L{W.in.liq, W.K.nx, W.K.ny};
-        cell_update<Real, DIM, FAST_A, true, TC>(W.K, L, W.l, W.m, W.k, W.T[ti], C);
-        write_cell<Real, DIM, FAST_A>(W.out, ncell, W.c, C);
-    }
+    front_bucket<DIM, TC, B, 1>(a, s_coff, blockIdx.x, gridDim.x);
 }
 
-// Pass 4: cells with more active grains than the merge pass holds.  The
-// arrays live in scratch memory with room for KMAX phases; the loops run over
-// the occupied entries only.
+// The grid (at least 4 blocks) is split over the buckets of 2, 3 and 4
+// solids in proportion to cells x solids, with at least one block for a
+// non-empty bucket; every block computes the same split from the counts.
+template <int DIM, int TC, int B>
+__global__ __launch_bounds__(128) void k_front_multi(StepArgs<Real> a)
+{
+    if (a.halt_mode && a.stats->halt_step < a.step) return;
+    __shared__ uint32_t s_coff[B ? MAX_INST : 1];
+    load_cell_offsets<B>(a, s_coff);
+    const int g = gridDim.x;
+    long long w[3], wt = 0;
+    int gb[3], sum = 0, big = 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        w[q] = (long long)(q + 2) * *wl_counter(a, WL_FRONT + q + 1, a.step);
+        wt += w[q];
+    }
+    if (wt == 0) return;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        gb[q] = w[q] > 0 ? max(1, (int)(g * w[q] / wt)) : 0;
+        sum += gb[q];
+        if (w[q] > w[big]) big = q;
+    }
+    gb[big] += g - sum; // stays >= 1: the largest bucket holds at least a third of g
+    const int b = blockIdx.x;
+    if (b < gb[0])
+        front_bucket<DIM, TC, B, 2>(a, s_coff, b, gb[0]);
+    else if (b < gb[0] + gb[1])
+        front_bucket<DIM, TC, B, 3>(a, s_coff, b - gb[0], gb[1]);
+    else
+        front_bucket<DIM, TC, B, 4>(a, s_coff, b - gb[0] - gb[1], gb[2]);
+}
+
+// Pass 4: cells with more active grains than the merge pass holds.  Cells
+// with up to MID_A active grains take unrolled register arrays; beyond that
+// the arrays live in scratch memory with room for KMAX phases and the loops
+// run over the occupied entries only.
+constexpr int MID_A = 8;
 template <int DIM, int TC, int B>
 __global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
 {
@@ -722,11 +842,20 @@ __global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
         const Where W = locate<B>(a, s_coff, c);
         size_t ti = a.temp_dense ? W.c : (size_t)W.k;
         double Tn = W.T[ti];
-        Cell<Real, DIM, KCAP> C;
+        SlotLiq<Real> L{W.in.liq, W.K.nx, W.K.ny};
         int ntot;
         bool fits;
+        {
+            Cell<Real, DIM, MID_A> C;
+            gather<Real, DIM, MID_A>(W.K, W.in, ncell, W.l, W.m, W.k, C, &ntot, &fits);
+            if (fits) {
+                cell_update<Real, DIM, MID_A, true, TC>(W.K, L, W.l, W.m, W.k, Tn, C);
+                write_cell<Real, DIM, MID_A>(W.out, ncell, W.c, C);
+                continue;
+            }
+        }
+        Cell<Real, DIM, KCAP> C;
         gather<Real, DIM, KCAP>(W.K, W.in, ncell, W.l, W.m, W.k, C, &ntot, &fits);
-        SlotLiq<Real> L{W.in.liq, W.K.nx, W.K.ny};
         cell_update<Real, DIM, KCAP, true, TC>(W.K, L, W.l, W.m, W.k, Tn, C);
         write_cell<Real, DIM, KCAP>(W.out, ncell, W.c, C);
     }
@@ -1251,17 +1380,18 @@ using namespace mpf::MPF_CAT(impl, MPF_SFX);
 
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
-// One step.  With a side stream, the many-phase kernel runs there,
-// concurrently with the interface kernel (both only read the input state and
-// write disjoint cells); the main stream waits for it before the step ends.
-// tev (optional): 6 timing events, recorded around the kernels: [0] before
-// k_light, [1] after it, [2] after k_merge, [3] after k_front, and [4]/[5]
-// around k_step_general.
+// One step.  With side streams, the many-phase kernel and the interface
+// cells with 2-4 solids run concurrently with the one-solid interface kernel
+// (all of them only read the input state and write disjoint cells); the
+// solver stream waits for both before the step ends.  tev (optional): 6
+// timing events, recorded around the kernels: [0] before k_tiles, [1] after
+// k_light, [2] after k_merge, [3] after k_front1 (and, without side streams,
+// k_front_multi), and [4]/[5] around k_step_general.
 template <int DIM, int TC, int B>
 static void launch_step_kernels(const StepArgs<Real> &a, unsigned grid, const int *grids, hipStream_t st,
-                                hipStream_t side, hipEvent_t ev_merged, hipEvent_t ev_general, hipEvent_t *tev)
+                                const StepStreams &S, hipEvent_t *tev)
 {
-    hipStream_t gs = side ? side : st;
+    const int gm = max(grids[1] / 2, 4);
     if (tev) hipEventRecord(tev[0], st);
     hipLaunchKernelGGL((k_tiles<DIM, B>), dim3(nblk((size_t)a.n_tiles, 256)), dim3(256), 0, st, a);
     // 32-bit offsets when every byte offset of the light pass fits (4- or
@@ -1274,29 +1404,34 @@ static void launch_step_kernels(const StepArgs<Real> &a, unsigned grid, const in
     if (tev) hipEventRecord(tev[1], st);
     hipLaunchKernelGGL((k_merge<DIM, TC, B>), dim3(grids[0]), dim3(256), 0, st, a);
     if (tev) hipEventRecord(tev[2], st);
-    if (side) {
-        hipEventRecord(ev_merged, st);
-        hipStreamWaitEvent(side, ev_merged, 0);
+    if (S.side) {
+        hipEventRecord(S.ev_merged, st);
+        hipStreamWaitEvent(S.side, S.ev_merged, 0);
+        hipStreamWaitEvent(S.side2, S.ev_merged, 0);
+        if (tev) hipEventRecord(tev[4], S.side);
+        hipLaunchKernelGGL((k_step_general<DIM, TC, B>), dim3(grids[2]), dim3(64), 0, S.side, a);
+        if (tev) hipEventRecord(tev[5], S.side);
+        hipEventRecord(S.ev_general, S.side);
+        hipLaunchKernelGGL((k_front_multi<DIM, TC, B>), dim3(gm), dim3(128), 0, S.side2, a);
+        hipEventRecord(S.ev_front, S.side2);
+        hipLaunchKernelGGL((k_front1<DIM, TC, B>), dim3(grids[1]), dim3(128), 0, st, a);
+        if (tev) hipEventRecord(tev[3], st);
+        hipStreamWaitEvent(st, S.ev_general, 0);
+        hipStreamWaitEvent(st, S.ev_front, 0);
     } else {
-        hipLaunchKernelGGL((k_front<DIM, TC, B>), dim3(grids[1]), dim3(128), 0, st, a);
+        hipLaunchKernelGGL((k_front1<DIM, TC, B>), dim3(grids[1]), dim3(128), 0, st, a);
+        hipLaunchKernelGGL((k_front_multi<DIM, TC, B>), dim3(gm), dim3(128), 0, st, a);
         if (tev) hipEventRecord(tev[3], st);
-    }
-    if (tev) hipEventRecord(tev[4], gs);
-    hipLaunchKernelGGL((k_step_general<DIM, TC, B>), dim3(grids[2]), dim3(64), 0, gs, a);
-    if (tev) hipEventRecord(tev[5], gs);
-    if (side) {
-        hipEventRecord(ev_general, side);
-        hipLaunchKernelGGL((k_front<DIM, TC, B>), dim3(grids[1]), dim3(128), 0, st, a);
-        if (tev) hipEventRecord(tev[3], st);
-        hipStreamWaitEvent(st, ev_general, 0);
+        if (tev) hipEventRecord(tev[4], st);
+        hipLaunchKernelGGL((k_step_general<DIM, TC, B>), dim3(grids[2]), dim3(64), 0, st, a);
+        if (tev) hipEventRecord(tev[5], st);
     }
 }
 
 extern "C" {
 
 hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const int *grids, hipStream_t st,
-                                     hipStream_t side, hipEvent_t ev_merged, hipEvent_t ev_general,
-                                     hipEvent_t *tev)
+                                     const StepStreams *ss, hipEvent_t *tev)
 {
     StepArgs<Real> a = *args;
     // persistent classification grid: four blocks per CU of the 256, at most
@@ -1305,23 +1440,23 @@ hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const 
     if (a.n_inst > 0) {
         // a batch of 3D grids: one launch per kernel covers every instance
         if (a.tab_class)
-            launch_step_kernels<3, 1, 1>(a, grid, grids, st, side, ev_merged, ev_general, tev);
+            launch_step_kernels<3, 1, 1>(a, grid, grids, st, *ss, tev);
         else
-            launch_step_kernels<3, 0, 1>(a, grid, grids, st, side, ev_merged, ev_general, tev);
+            launch_step_kernels<3, 0, 1>(a, grid, grids, st, *ss, tev);
         return hipGetLastError();
     }
     a.ntx = (a.K.nx + LTX - 1) / LTX;
     a.nty = (a.K.ny + LTY - 1) / LTY;
     if (dim == 3) {
         if (a.tab_class)
-            launch_step_kernels<3, 1, 0>(a, grid, grids, st, side, ev_merged, ev_general, tev);
+            launch_step_kernels<3, 1, 0>(a, grid, grids, st, *ss, tev);
         else
-            launch_step_kernels<3, 0, 0>(a, grid, grids, st, side, ev_merged, ev_general, tev);
+            launch_step_kernels<3, 0, 0>(a, grid, grids, st, *ss, tev);
     } else {
         if (a.tab_class)
-            launch_step_kernels<2, 1, 0>(a, grid, grids, st, side, ev_merged, ev_general, tev);
+            launch_step_kernels<2, 1, 0>(a, grid, grids, st, *ss, tev);
         else
-            launch_step_kernels<2, 0, 0>(a, grid, grids, st, side, ev_merged, ev_general, tev);
+            launch_step_kernels<2, 0, 0>(a, grid, grids, st, *ss, tev);
     }
     return hipGetLastError();
 }

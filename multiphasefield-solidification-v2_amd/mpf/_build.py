@@ -9,6 +9,7 @@ from __future__ import annotations
 import os
 import shutil
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 import sys
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -46,11 +47,12 @@ def _run(cmd, verbose):
 
 
 def build(verbose: bool = False) -> str:
-    """Compile every HIP source and link mpf/libmpf.so.  Always rebuilds."""
+    """Compile every HIP source (the translation units in parallel) and link
+    mpf/libmpf.so.  Always rebuilds."""
     os.makedirs(BUILD, exist_ok=True)
     cc = hipcc()
     kern = os.path.join(CSRC, "mpf_kernels.hip")
-    objs = []
+    steps = []
     # FP64: IEEE operations in source order (bit-exact with the oracle).
     # FP32: contraction and the hardware's approximate (1-2 ulp) square root
     # and division instead of the correctly rounded sequences.
@@ -58,15 +60,15 @@ def build(verbose: bool = False) -> str:
                 ("float", "_f32", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]))
     for real, sfx, flags in variants:
         o = os.path.join(BUILD, f"mpf_kernels{sfx}.o")
-        _run([cc, *_common(), f"-DMPF_REAL={real}", f"-DMPF_SFX={sfx}", *flags, "-c", kern, "-o", o],
-             verbose)
-        objs.append(o)
+        steps.append(([cc, *_common(), f"-DMPF_REAL={real}", f"-DMPF_SFX={sfx}", *flags, "-c", kern, "-o", o], o))
     nn = os.path.join(BUILD, "mpf_nn.o")
-    _run([cc, *_common(), "-ffp-contract=fast", "-c", os.path.join(CSRC, "mpf_nn.hip"), "-o", nn], verbose)
-    objs.append(nn)
+    steps.append(([cc, *_common(), "-ffp-contract=fast", "-c", os.path.join(CSRC, "mpf_nn.hip"), "-o", nn], nn))
     api = os.path.join(BUILD, "mpf_api.o")
-    _run([cc, *_common(), "-x", "hip", "-c", os.path.join(CSRC, "mpf_api.cpp"), "-o", api], verbose)
-    objs.append(api)
+    steps.append(([cc, *_common(), "-x", "hip", "-c", os.path.join(CSRC, "mpf_api.cpp"), "-o", api], api))
+    with ThreadPoolExecutor(max_workers=len(steps)) as ex:
+        for f in [ex.submit(_run, cmd, verbose) for cmd, _ in steps]:
+            f.result()
+    objs = [o for _, o in steps]
     tmp = LIB + ".tmp"
     _run([cc, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
