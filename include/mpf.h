@@ -89,6 +89,8 @@ typedef struct mpf_stats {
     unsigned long long light_cells; /* cells of the tile planes the classification pass computed,
                                        summed over steps; the other planes held liquid 1 with liquid
                                        1 around them and were left unchanged */
+    int n111_pairs;                 /* 1: every grain's n111 rows come in opposite pairs (row 7-t = -row t,
+                                       src/orientation_utils.py:182-212) and the facet search scans four */
 } mpf_stats;
 
 /* ---- resident solver ---------------------------------------------------- */

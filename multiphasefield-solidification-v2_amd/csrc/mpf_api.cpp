@@ -92,6 +92,7 @@ struct mpf_handle {
     size_t tab_stride = 0;          // table elements per instance (batches)
     std::vector<int> tab_set;       // per instance: tables uploaded
     std::vector<int> tab_cls;       // per instance: tables have the class form
+    std::vector<int> tab_pairs;     // per instance: n111 rows in opposite pairs
     std::vector<double> cls_vals;   // per instance: 6 class values
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;            // many-phase kernel, concurrent with the interface kernel
@@ -114,6 +115,7 @@ struct mpf_handle {
     int steps_done = 0;
     int tables_set = 0;
     int tab_class = 0;    // every instance's tables have the same class form
+    int n111_pairs = 0;   // every instance's n111 rows come in opposite pairs
     double cls[3][2] = {}; // its values, exact in the solver precision
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
@@ -268,6 +270,7 @@ static int create_handle(const mpf_config &c, const std::vector<Inst> &grids, in
     h->tab_stride = 3 * nn + (size_t)c.n_phase * 24;
     h->tab_set.assign(inst.size(), 0);
     h->tab_cls.assign(inst.size(), 0);
+    h->tab_pairs.assign(inst.size(), 0);
     h->cls_vals.assign(inst.size() * 6, 0.0);
     int cap = c.kmax < c.n_phase - 1 ? c.kmax : c.n_phase - 1;
     if (c.slot_capacity > 0) {
@@ -393,12 +396,13 @@ static int set_tables_inst(mpf_handle *h, int ii, const double *wij, const doubl
     memcpy(all.data() + 2 * nn, mij, nn * sizeof(double));
     memcpy(all.data() + 3 * nn, n111, (size_t)N * 24 * sizeof(double));
     char *dst = (char *)h->tab + (size_t)ii * tot * h->rsz;
-    int cls_ok = 0;
+    int cls_ok = 0, pairs = 0;
     double cv[6];
     if (h->rsz == 8) {
         HIP_TRY(hipMemcpy(dst, all.data(), tot * sizeof(double), hipMemcpyHostToDevice));
         double cls[3][2];
         cls_ok = class_tables<double>(all.data(), N, cls);
+        pairs = n111_pairs<double>(all.data() + 3 * nn, N);
         for (int t = 0; t < 6; ++t) cv[t] = cls[t / 2][t % 2];
     } else {
         std::vector<float> f(tot);
@@ -406,20 +410,24 @@ static int set_tables_inst(mpf_handle *h, int ii, const double *wij, const doubl
         HIP_TRY(hipMemcpy(dst, f.data(), tot * sizeof(float), hipMemcpyHostToDevice));
         float cls[3][2];
         cls_ok = class_tables<float>(f.data(), N, cls);
+        pairs = n111_pairs<float>(f.data() + 3 * nn, N);
         for (int t = 0; t < 6; ++t) cv[t] = cls[t / 2][t % 2];
     }
     drop_graphs(h); // the class form and values are part of a captured step
     h->tab_set[ii] = 1;
     h->tab_cls[ii] = cls_ok;
+    h->tab_pairs[ii] = pairs;
     for (int t = 0; t < 6; ++t) h->cls_vals[(size_t)ii * 6 + t] = cv[t];
-    int all_set = 1, all_cls = 1;
+    int all_set = 1, all_cls = 1, all_pairs = 1;
     for (size_t i = 0; i < h->inst.size(); ++i) {
         all_set &= h->tab_set[i];
+        all_pairs &= h->tab_set[i] && h->tab_pairs[i];
         all_cls &= h->tab_set[i] && h->tab_cls[i] &&
                    memcmp(&h->cls_vals[i * 6], &h->cls_vals[0], 6 * sizeof(double)) == 0;
     }
     h->tables_set = all_set;
     h->tab_class = all_cls;
+    h->n111_pairs = all_pairs;
     for (int t = 0; t < 6; ++t) h->cls[t / 2][t % 2] = h->cls_vals[t];
     return 0;
 }
@@ -643,6 +651,7 @@ static hipError_t launch_one_step(mpf_handle *h, const int *grids, hipEvent_t *t
     a.step = h->step_counter;
     a.halt_mode = c.halt_on_kmax;
     a.tab_class = h->tab_class;
+    a.K.n111_pairs = h->n111_pairs;
     for (int t = 0; t < 3; ++t) {
         a.K.cls[t][0] = (R)h->cls[t][0];
         a.K.cls[t][1] = (R)h->cls[t][1];
@@ -879,6 +888,7 @@ extern "C" int mpf_get_stats(mpf_handle *h, mpf_stats *st)
     st->merge_cells = s.merge_cells;
     st->front_cells = s.front_cells;
     st->light_cells = s.light_cells;
+    st->n111_pairs = h->n111_pairs;
     st->class_tables = h->tab_class;
     for (int j = 0; j < 4; ++j) st->kernel_ms[j] = h->kern_ms[j];
     st->kernel_steps = h->kern_steps;

@@ -58,6 +58,10 @@ struct Consts {
     // cls[table][0] for solid-liquid pairs, cls[table][1] for grain-grain
     // pairs; table 0 = wij, 1 = aij, 2 = mij.
     Real cls[3][2];
+    // 1 when every grain's n111 rows come in opposite pairs, row 7 - t equal
+    // to row t with every sign bit flipped (the rotated (+-1,+-1,+-1)/sqrt(3)
+    // of compute_rotated_n111 in its order); checked bit for bit by the host.
+    int n111_pairs;
 };
 
 // Interaction-table lookup.  TC = 1: the tables have the class form that
@@ -91,6 +95,29 @@ MPF_DEV double uniform_in_vgpr(double x)
     return r;
 #else
     return x;
+#endif
+}
+
+// Lane of the wavefront and a value read from another lane (lane groups of
+// cell_update).  The host build runs one cell per call and never groups
+// lanes, so there they are lane 0 and the identity.
+MPF_DEV int lane_id()
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (int)__lane_id();
+#else
+    return 0;
+#endif
+}
+
+template <typename T>
+MPF_DEV T lane_read(T v, int src)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __shfl(v, src, 64);
+#else
+    (void)src;
+    return v;
 #endif
 }
 
@@ -184,8 +211,33 @@ struct Facet {
 
 // facet_cos_and_nxyz_from_grad (src/gpu_kernels_3d.py:251-301) given the unit
 // gradient; best_cos_from_grad_3d (:110-134) returns the same |cos|.
+//
+// The scan keeps the first facet of largest |cos| (strictly greater
+// replaces).  When the rows come in opposite pairs (n111_pairs), rows 4..7
+// give the |cos| of rows 3..0 again (negating a dot product only flips its
+// sign), so they can never replace and the first four rows decide: n_lo holds
+// rows 0..3 (12 values), rows 4..7 are read from n_hi (the table's row 4)
+// only when pairs is false.
 template <typename Real, int DIM>
-MPF_DEV Facet<Real> facet_search(const Real *n, Real gx, Real gy, Real gz)
+MPF_DEV void facet_visit(Facet<Real> &f, Real ux, Real uy, Real uz, Real gx, Real gy, Real gz)
+{
+    Real dot;
+    if constexpr (DIM == 3)
+        dot = gx * ux + gy * uy + gz * uz;
+    else
+        dot = gx * ux + gy * uy;
+    Real ad = m_abs(dot);
+    if (ad > f.ca) {
+        f.ca = ad;
+        f.cs = dot;
+        f.nx = ux;
+        f.ny = uy;
+        f.nz = uz;
+    }
+}
+
+template <typename Real, int DIM>
+MPF_DEV Facet<Real> facet_search(const Real *n_lo, const Real *n_hi, bool pairs, Real gx, Real gy, Real gz)
 {
     Facet<Real> f;
     f.cs = Real(0.0);
@@ -194,21 +246,11 @@ MPF_DEV Facet<Real> facet_search(const Real *n, Real gx, Real gy, Real gz)
     f.ny = Real(0.0);
     f.nz = Real(0.0);
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        Real ux = n[t * 3 + 0], uy = n[t * 3 + 1], uz = n[t * 3 + 2];
-        Real dot;
-        if constexpr (DIM == 3)
-            dot = gx * ux + gy * uy + gz * uz;
-        else
-            dot = gx * ux + gy * uy;
-        Real ad = m_abs(dot);
-        if (ad > f.ca) {
-            f.ca = ad;
-            f.cs = dot;
-            f.nx = ux;
-            f.ny = uy;
-            f.nz = uz;
-        }
+    for (int t = 0; t < 4; ++t) facet_visit<Real, DIM>(f, n_lo[t * 3], n_lo[t * 3 + 1], n_lo[t * 3 + 2], gx, gy, gz);
+    if (!pairs) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            facet_visit<Real, DIM>(f, n_hi[t * 3], n_hi[t * 3 + 1], n_hi[t * 3 + 2], gx, gy, gz);
     }
     if (f.ca > Real(1.0)) f.ca = Real(1.0);
     if (f.cs > Real(1.0)) f.cs = Real(1.0);
@@ -377,16 +419,19 @@ MPF_DEV void solid_liquid_terms(const Consts<Real> &K, const LiqCtx<Real, DIM> &
                                 Real &b_sl)
 {
     constexpr int NP = LiqCtx<Real, DIM>::NP;
-    Real nrm[24];
+    const Real *nrow = K.n111() + (size_t)s * 24;
+    Real nrm[12];
 #pragma unroll
-    for (int u = 0; u < 24; ++u) nrm[u] = K.n111()[(size_t)s * 24 + u];
+    for (int u = 0; u < 12; ++u) nrm[u] = nrow[u];
+    const bool pairs = K.n111_pairs != 0;
     // One stencil point: facet search, eps^2 and A12 (written out per point so
     // that every access to the context is static).
     auto point = [&](const LGrad<Real> &G, Real &e2p, Facet<Real> &f, A12v<Real> &r) {
         f.cs = f.ca = f.ny = f.nz = Real(0.0);
         f.nx = Real(1.0);
         // eps2_at_cell_from_liquid_3d (:137-146): best_cos is 0 below the floor
-        if (lgrad_ok(K, G)) f = facet_search<Real, DIM>(nrm, G.gx * G.inv_g, G.gy * G.inv_g, G.gz * G.inv_g);
+        if (lgrad_ok(K, G))
+            f = facet_search<Real, DIM>(nrm, nrow + 12, pairs, G.gx * G.inv_g, G.gy * G.inv_g, G.gz * G.inv_g);
         Real a = calc_a(K, f.ca);
         Real e = K.eps0_sl * a;
         e2p = e * e;
@@ -535,7 +580,11 @@ struct Cell {
 // guarantees the cell has no active solid-liquid pair or anisotropy is off.
 // EXACT = true: the caller guarantees an active liquid and exactly A active
 // solids, so every presence test folds at compile time.
-template <typename Real, int DIM, int A, bool ANISO_TERMS = true, int TC = 0, bool EXACT = false,
+// G > 1 (register path only): G consecutive lanes of the wavefront (G a
+// power of two, G >= A) update the same cell together; lane r of the group
+// evaluates the solid-liquid terms of solid r + 1 and the group exchanges
+// them, which shortens the serial chain of a cell with several solids.
+template <typename Real, int DIM, int A, bool ANISO_TERMS = true, int TC = 0, bool EXACT = false, int G = 1,
           typename LiqAt>
 MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, int k,
                                             double Tcell, Cell<Real, DIM, A> &S)
@@ -610,7 +659,8 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
                     Real cmax = Real(0.0);
                     if (!(g2 < K.g2_floor)) {
                         Real inv_g = Real(1.0) / m_sqrt(g2);
-                        Facet<Real> f = facet_search<Real, 3>(K.n111() + (size_t)gs * 24, sx * inv_g,
+                        const Real *nr = K.n111() + (size_t)gs * 24;
+                        Facet<Real> f = facet_search<Real, 3>(nr, nr + 12, K.n111_pairs != 0, sx * inv_g,
                                                               sy * inv_g, sz * inv_g);
                         cmax = f.ca;
                     }
@@ -644,7 +694,8 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
             Real cmax = Real(0.0);
             if (gn2 >= K.g2_floor) {
                 Real inv_gn = Real(1.0) / m_sqrt(gn2);
-                Facet<Real> f = facet_search<Real, 2>(K.n111() + (size_t)gs * 24, phix_ * inv_gn,
+                const Real *nr = K.n111() + (size_t)gs * 24;
+                Facet<Real> f = facet_search<Real, 2>(nr, nr + 12, K.n111_pairs != 0, phix_ * inv_gn,
                                                       phiy_ * inv_gn, Real(0.0));
                 cmax = f.ca;
             }
@@ -656,7 +707,26 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
             if constexpr (ANISO_TERMS) {
                 LiqCtx<Real, DIM> X;
                 build_liq_ctx<Real, DIM>(K, L, l, m, k, X);
-                if constexpr (SMALL) {
+                if constexpr (SMALL && G > 1) {
+                    static_assert(G >= A && (G & (G - 1)) == 0 && G <= 64, "one lane per solid");
+                    const int lane = lane_id();
+                    const int r = lane & (G - 1);
+                    int gr = 0;
+#pragma unroll
+                    for (int u = 0; u < A; ++u)
+                        if (u == r) gr = S.gid[u + 1];
+                    Real ls = Real(0.0), bs = Real(1.0);
+                    if (r < s_nsol) solid_liquid_terms<Real, DIM>(K, X, gr, ls, bs);
+#pragma unroll
+                    for (int u = 0; u < A; ++u) {
+                        const Real lu = lane_read(ls, (lane & ~(G - 1)) + u);
+                        const Real bu = lane_read(bs, (lane & ~(G - 1)) + u);
+                        if (u < s_nsol) {
+                            lap_sl[u + 1] = lu;
+                            b_sl[u + 1] = bu;
+                        }
+                    }
+                } else if constexpr (SMALL) {
                     // The solids pass through a rotating register queue: A
                     // iterations (uniform trip count, body emitted once), each
                     // taking the head id and appending its result, so every

@@ -2,7 +2,10 @@
 // Shared by the host API and the CPU emulation tests.
 #pragma once
 
+#include <stdint.h>
 #include <string.h>
+
+#include <type_traits>
 
 #include "mpf.h"
 #include "mpf_cell.h"
@@ -50,7 +53,30 @@ inline Consts<R> make_consts(const mpf_scalars &s, int nx, int ny, int nz, int N
     K.tab = (const R *)tab;
     K.nn = (uint32_t)N * (uint32_t)N;
     for (int t = 0; t < 3; ++t) K.cls[t][0] = K.cls[t][1] = R(0);
+    K.n111_pairs = 0;
     return K;
+}
+
+// True when every grain's n111 rows (grains 1..N-1, [8][3] each, in the
+// solver precision) come in opposite pairs: row 7 - t is row t with every
+// sign bit flipped, as compute_rotated_n111 (src/orientation_utils.py:182-212)
+// produces from (+-1,+-1,+-1)/sqrt(3) in its order.
+template <typename R>
+inline bool n111_pairs(const R *n111, int N)
+{
+    using U = typename std::conditional<sizeof(R) == 8, uint64_t, uint32_t>::type;
+    const U sign = (U)1 << (8 * sizeof(R) - 1);
+    for (int g = 1; g < N; ++g) {
+        const R *row = n111 + (size_t)g * 24;
+        for (int t = 0; t < 4; ++t)
+            for (int c = 0; c < 3; ++c) {
+                U a, b;
+                memcpy(&a, &row[t * 3 + c], sizeof(R));
+                memcpy(&b, &row[(7 - t) * 3 + c], sizeof(R));
+                if ((a ^ sign) != b) return false;
+            }
+    }
+    return true;
 }
 
 // True when the tables tab = {wij, aij, mij} ([N][N] each, in the solver

@@ -756,26 +756,30 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
 // k_front_multi the buckets of 2, 3 and 4 solids (triple lines), in block
 // ranges of one launch, concurrently with k_front1.
 static_assert(NFB == FAST_A, "interface buckets cover the register path's solid counts");
-template <int DIM, int TC, int B, int NS>
+// G lanes per cell (G >= NS; see cell_update): lane 0 of each group notes
+// the statistics and writes the cell.
+template <int DIM, int TC, int B, int NS, int G>
 __device__ __forceinline__ void front_bucket(const StepArgs<Real> &a, const uint32_t *s_coff, int blk, int nblk)
 {
     const Consts<Real> &K = a.K;
     const int n = *wl_counter(a, WL_FRONT + NS - 1, a.step);
     const int *list = wl_list(a, WL_FRONT + NS - 1);
     const size_t ncell = a.in.stride; // slot stride
+    const bool lead = (threadIdx.x & (G - 1)) == 0;
     if (blk == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->front_cells, (unsigned long long)n);
-    for (int i = blk * blockDim.x + threadIdx.x; i < n; i += nblk * blockDim.x) {
+    const int per = blockDim.x / G;
+    for (int i = blk * per + (int)threadIdx.x / G; i < n; i += nblk * per) {
         const size_t c = (size_t)list[i];
         const Where W = locate<B>(a, s_coff, c);
         Cell<Real, DIM, NS> C;
         int ntot;
         bool fits;
         gather<Real, DIM, NS>(W.K, W.in, ncell, W.l, W.m, W.k, C, &ntot, &fits);
-        note_active(a.stats, ntot, K.kmax, a.step, a.halt_mode);
+        if (lead) note_active(a.stats, ntot, K.kmax, a.step, a.halt_mode);
         size_t ti = a.temp_dense ? W.c : (size_t)W.k;
         SlotLiq<Real> L{W.in.liq, W.K.nx, W.K.ny};
-        cell_update<Real, DIM, NS, true, TC, true>(W.K, L, W.l, W.m, W.k, W.T[ti], C);
-        write_cell<Real, DIM, NS>(W.out, ncell, W.c, C);
+        cell_update<Real, DIM, NS, true, TC, true, G>(W.K, L, W.l, W.m, W.k, W.T[ti], C);
+        if (lead) write_cell<Real, DIM, NS>(W.out, ncell, W.c, C);
     }
 }
 
@@ -785,12 +789,14 @@ __global__ __launch_bounds__(128) void k_front1(StepArgs<Real> a)
     if (a.halt_mode && a.stats->halt_step < a.step) return;
     __shared__ uint32_t s_coff[B ? MAX_INST : 1];
     load_cell_offsets<B>(a, s_coff);
-    front_bucket<DIM, TC, B, 1>(a, s_coff, blockIdx.x, gridDim.x);
+    front_bucket<DIM, TC, B, 1, 1>(a, s_coff, blockIdx.x, gridDim.x);
 }
 
 // The grid (at least 4 blocks) is split over the buckets of 2, 3 and 4
 // solids in proportion to cells x solids, with at least one block for a
 // non-empty bucket; every block computes the same split from the counts.
+// (Lane groups per cell measured slower here: the bucket of two solids is
+// large enough to keep every lane busy with a cell of its own.)
 template <int DIM, int TC, int B>
 __global__ __launch_bounds__(128) void k_front_multi(StepArgs<Real> a)
 {
@@ -815,17 +821,17 @@ __global__ __launch_bounds__(128) void k_front_multi(StepArgs<Real> a)
     gb[big] += g - sum; // stays >= 1: the largest bucket holds at least a third of g
     const int b = blockIdx.x;
     if (b < gb[0])
-        front_bucket<DIM, TC, B, 2>(a, s_coff, b, gb[0]);
+        front_bucket<DIM, TC, B, 2, 1>(a, s_coff, b, gb[0]);
     else if (b < gb[0] + gb[1])
-        front_bucket<DIM, TC, B, 3>(a, s_coff, b - gb[0], gb[1]);
+        front_bucket<DIM, TC, B, 3, 1>(a, s_coff, b - gb[0], gb[1]);
     else
-        front_bucket<DIM, TC, B, 4>(a, s_coff, b - gb[0] - gb[1], gb[2]);
+        front_bucket<DIM, TC, B, 4, 1>(a, s_coff, b - gb[0] - gb[1], gb[2]);
 }
 
 // Pass 4: cells with more active grains than the merge pass holds.  Cells
-// with up to MID_A active grains take unrolled register arrays; beyond that
-// the arrays live in scratch memory with room for KMAX phases and the loops
-// run over the occupied entries only.
+// with up to MID_A active grains take unrolled register arrays, MID_A lanes
+// per cell; beyond that one lane per cell keeps the arrays in scratch memory
+// with room for KMAX phases and the loops run over the occupied entries only.
 constexpr int MID_A = 8;
 template <int DIM, int TC, int B>
 __global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
@@ -837,7 +843,12 @@ __global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
     const int n = *wl_counter(a, WL_GENERAL, a.step);
     const size_t ncell = a.in.stride; // slot stride
     if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->slow_cells, (unsigned long long)n);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    // MID_A lanes per cell on the register path (one per solid); lane 0 of
+    // the group writes, and alone takes the scratch path
+    constexpr int G = MID_A;
+    const bool lead = (threadIdx.x & (G - 1)) == 0;
+    const int per = blockDim.x / G;
+    for (int i = blockIdx.x * per + (int)threadIdx.x / G; i < n; i += gridDim.x * per) {
         const size_t c = (size_t)wl_list(a, WL_GENERAL)[i];
         const Where W = locate<B>(a, s_coff, c);
         size_t ti = a.temp_dense ? W.c : (size_t)W.k;
@@ -849,11 +860,12 @@ __global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
             Cell<Real, DIM, MID_A> C;
             gather<Real, DIM, MID_A>(W.K, W.in, ncell, W.l, W.m, W.k, C, &ntot, &fits);
             if (fits) {
-                cell_update<Real, DIM, MID_A, true, TC>(W.K, L, W.l, W.m, W.k, Tn, C);
-                write_cell<Real, DIM, MID_A>(W.out, ncell, W.c, C);
+                cell_update<Real, DIM, MID_A, true, TC, false, G>(W.K, L, W.l, W.m, W.k, Tn, C);
+                if (lead) write_cell<Real, DIM, MID_A>(W.out, ncell, W.c, C);
                 continue;
             }
         }
+        if (!lead) continue;
         Cell<Real, DIM, KCAP> C;
         gather<Real, DIM, KCAP>(W.K, W.in, ncell, W.l, W.m, W.k, C, &ntot, &fits);
         cell_update<Real, DIM, KCAP, true, TC>(W.K, L, W.l, W.m, W.k, Tn, C);
@@ -1391,7 +1403,7 @@ template <int DIM, int TC, int B>
 static void launch_step_kernels(const StepArgs<Real> &a, unsigned grid, const int *grids, hipStream_t st,
                                 const StepStreams &S, hipEvent_t *tev)
 {
-    const int gm = max(grids[1] / 2, 4);
+    const int gm = max(grids[1], 4);
     if (tev) hipEventRecord(tev[0], st);
     hipLaunchKernelGGL((k_tiles<DIM, B>), dim3(nblk((size_t)a.n_tiles, 256)), dim3(256), 0, st, a);
     // 32-bit offsets when every byte offset of the light pass fits (4- or

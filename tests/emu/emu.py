@@ -62,10 +62,11 @@ def unpack(liq, cnt, sid, sval, N, grid):
 
 
 def step(phi, T_cooled, tables, scalars, kmax=50, aniso=True, torque=True, precision="fp64", mode=0,
-         use_class=False):
+         use_class=False, pairs=False):
     """One step of the device code on the CPU.  T_cooled is the dense [nx, ny, nz]
     temperature after this step's cooling.  use_class selects the class-form
-    table code (the tables must have that form).  Returns (phi_new, path
+    table code (the tables must have that form); pairs the four-row facet scan
+    (the n111 rows must come in opposite pairs).  Returns (phi_new, path
     counts, per-cell path)."""
     from mpf import _lib as L
 
@@ -92,7 +93,10 @@ def step(phi, T_cooled, tables, scalars, kmax=50, aniso=True, torque=True, preci
     rc = lib().emu_step(C.c_int(1 if precision == "fp64" else 0), C.c_int(dim), C.byref(s), nx, ny, nz, N, kmax,
                         int(aniso), int(torque), vp(tab), vp(liq), vp(cnt), vp(sid), vp(sval), vp(lo), vp(co),
                         vp(so), vp(vo), T.ctypes.data_as(C.POINTER(C.c_double)), mode,
-                        paths.ctypes.data_as(C.POINTER(C.c_int)), vp(path_of), int(bool(use_class)))
+                        paths.ctypes.data_as(C.POINTER(C.c_int)), vp(path_of),
+                        int(bool(use_class)) | (2 if pairs else 0))
+    if rc == -2:
+        raise ValueError("the n111 rows are not in opposite pairs")
     if rc != 0:
         raise ValueError("the tables do not have the class form")
     po = np.ascontiguousarray(np.transpose(path_of.reshape(tuple(reversed(grid))), tuple(range(dim - 1, -1, -1))))

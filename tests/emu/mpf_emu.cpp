@@ -106,8 +106,10 @@ static int step_any(Consts<R> &K, int dim, int use_class, SlotView<R> in, SlotVi
     return tc ? step<R, 2, 1>(K, in, out, T, mode, paths, path_of) : step<R, 2, 0>(K, in, out, T, mode, paths, path_of);
 }
 
-// use_class: select the class-form table kernels when the tables have that
-// form (as the solver does); returns -1 if asked for but the form is absent.
+// use_class bit 0: select the class-form table kernels when the tables have
+// that form (as the solver does); returns -1 if asked for but the form is
+// absent.  Bit 1: scan four facet rows when the n111 rows come in opposite
+// pairs (as the solver does); returns -2 if asked for but they do not.
 extern "C" int emu_step(int precision, int dim, const mpf_scalars *s, int nx, int ny, int nz, int N, int kmax,
                         int aniso, int torque, const void *tables, const void *liq_in, const uint8_t *cnt_in,
                         const uint16_t *sid_in, const void *sval_in, void *liq_out, uint8_t *cnt_out,
@@ -115,9 +117,13 @@ extern "C" int emu_step(int precision, int dim, const mpf_scalars *s, int nx, in
                         int use_class)
 {
     const size_t ncell = (size_t)nx * ny * (dim == 3 ? nz : 1);
+    const bool want_pairs = (use_class & 2) != 0;
+    use_class &= 1;
     if (precision == MPF_FP64) {
         Consts<double> K = make_consts<double>(*s, nx, ny, nz, N, kmax, dim == 2 ? 1 : aniso, dim == 2 ? 1 : torque,
                                                tables);
+        if (want_pairs && !n111_pairs<double>(K.n111(), N)) return -2;
+        K.n111_pairs = want_pairs ? 1 : 0;
         double cls[3][2];
         if (use_class && !class_tables<double>(K.tab, N, cls)) return -1;
         SlotView<double> in{(double *)liq_in, (uint8_t *)cnt_in, (uint16_t *)sid_in, (double *)sval_in, ncell};
@@ -126,6 +132,8 @@ extern "C" int emu_step(int precision, int dim, const mpf_scalars *s, int nx, in
     }
     Consts<float> K = make_consts<float>(*s, nx, ny, nz, N, kmax, dim == 2 ? 1 : aniso, dim == 2 ? 1 : torque,
                                          tables);
+    if (want_pairs && !n111_pairs<float>(K.n111(), N)) return -2;
+    K.n111_pairs = want_pairs ? 1 : 0;
     float cls[3][2];
     if (use_class && !class_tables<float>(K.tab, N, cls)) return -1;
     SlotView<float> in{(float *)liq_in, (uint8_t *)cnt_in, (uint16_t *)sid_in, (float *)sval_in, ncell};

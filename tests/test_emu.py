@@ -84,6 +84,29 @@ def test_fp64_2d_bit_exact(use_class):
         assert np.array_equal(out, ref)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_paired_facet_scan_identical(precision):
+    """The n111 rows that compute_rotated_n111 builds come in opposite pairs;
+    scanning four rows then gives the same facet, so the step is identical
+    bit for bit to the eight-row scan (FP64 and FP32), and FP64 equals the
+    oracle.  A table whose pairs are broken is refused for the four-row scan."""
+    c = cases.voronoi_case(nx=10, ny=9, nz=12, n_solid=7, seed=3, qseed=4)
+    cur8 = cur4 = c["phi"]
+    for _ in range(3):
+        out8, _, _ = emu.step(cur8, c["temp"], c["tables"], c["scalars"], precision=precision)
+        out4, _, _ = emu.step(cur4, c["temp"], c["tables"], c["scalars"], precision=precision, pairs=True)
+        assert np.array_equal(out4, out8)
+        if precision == "fp64":
+            mf, nf, _ = O.apt(cur8, 50)
+            assert np.array_equal(out4, O.update(cur8, c["temp"], mf, nf, c["tables"], c["scalars"]))
+        cur8, cur4 = out8, out4
+    w, a, m, n111 = c["tables"]
+    broken = n111.copy()
+    broken[2, 6, 1] += 1e-3
+    with pytest.raises(ValueError):
+        emu.step(c["phi"], c["temp"], (w, a, m, broken), c["scalars"], pairs=True)
+
+
 def test_class_form_detection():
     """Tables from build_interaction_matrices have the class form; changing one
     off-diagonal entry, or a non-zero diagonal, removes it (the general lookup

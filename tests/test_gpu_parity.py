@@ -246,22 +246,25 @@ def test_fp64_quad_junction_bit_exact():
 
 
 def test_fp64_general_and_class_tables_bit_exact():
-    """Reference tables take the class-form kernels; a table with one entry
-    changed takes the general lookup kernels.  Both bit-identical to the oracle."""
+    """Reference tables take the class-form kernels and the four-row facet
+    scan (opposite n111 row pairs); a table with one entry changed takes the
+    general lookup kernels, and an n111 row with one value changed the
+    eight-row scan.  All bit-identical to the oracle."""
     c = cases.voronoi_case(n_solid=5)
     steps = 6
     with _solver_for(c, "fp64", temp_f32=True) as s:
-        assert s.stats()["class_tables"]
+        assert s.stats()["class_tables"] and s.stats()["n111_pairs"]
         s.step(steps)
         ref, _, _, _ = O.run(c["phi"], c["temp"], steps, c["tables"], c["scalars"], kmax=50, temp_f32=True)
         assert np.array_equal(s.phi(), ref)
     w, a, m, n111 = (t.copy() for t in c["tables"])
     w[2, 3] *= 1.25
     a[0, 4] *= 0.9
+    n111[3, 5, 2] *= 1.0 + 1e-6
     tabs = (w, a, m, n111)
     c2 = dict(c, tables=tabs)
     with _solver_for(c2, "fp64", temp_f32=True) as s:
-        assert not s.stats()["class_tables"]
+        assert not s.stats()["class_tables"] and not s.stats()["n111_pairs"]
         s.step(steps)
         ref, _, _, _ = O.run(c["phi"], c["temp"], steps, tabs, c["scalars"], kmax=50, temp_f32=True)
         assert np.array_equal(s.phi(), ref)
