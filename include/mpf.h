@@ -85,7 +85,9 @@ typedef struct mpf_stats {
                                        and the step selects their values instead of loading them */
     int kernel_steps;               /* steps timed per kernel (mpf_set_kernel_timing) */
     double kernel_ms[4];            /* summed GPU time of the step's kernels over those steps:
-                                       classification, merge, interface, many-phase */
+                                       classification (with its tile pass), merge, one-solid
+                                       interface cells, many-phase (the other interface cells are
+                                       not timed on their own) */
     unsigned long long light_cells; /* cells of the tile planes the classification pass computed,
                                        summed over steps; the other planes held liquid 1 with liquid
                                        1 around them and were left unchanged */

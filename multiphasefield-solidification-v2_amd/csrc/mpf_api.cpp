@@ -96,10 +96,10 @@ struct mpf_handle {
     std::vector<double> cls_vals;   // per instance: 6 class values
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;            // many-phase kernel, concurrent with the interface kernel
-    hipStream_t side2 = nullptr;           // interface cells with 2-4 solids, concurrent as well
+    hipStream_t side2 = nullptr;           // interface cells of several solids, concurrent as well
     hipEvent_t ev_merged = nullptr, ev_general = nullptr, ev_front = nullptr;
     int kernel_timing = 0;                 // per-kernel events (mpf_set_kernel_timing)
-    std::vector<hipEvent_t> tev;           // 6 per step of the current call
+    std::vector<hipEvent_t> tev;           // 8 per step of the current call
     double kern_ms[4] = {0, 0, 0, 0};      // k_light, k_merge, k_front, k_step_general
     int kern_steps = 0;
     Buffers b[2];
@@ -199,9 +199,8 @@ extern "C" void mpf_destroy(mpf_handle *h)
     if (h->side) hipStreamSynchronize(h->side);
     if (h->side2) hipStreamSynchronize(h->side2);
     for (hipEvent_t e : h->tev) hipEventDestroy(e);
-    if (h->ev_merged) hipEventDestroy(h->ev_merged);
-    if (h->ev_general) hipEventDestroy(h->ev_general);
-    if (h->ev_front) hipEventDestroy(h->ev_front);
+    for (hipEvent_t e : {h->ev_merged, h->ev_general, h->ev_front})
+        if (e) hipEventDestroy(e);
     if (h->side) hipStreamDestroy(h->side);
     if (h->side2) hipStreamDestroy(h->side2);
     if (h->ev0) hipEventDestroy(h->ev0);
@@ -725,7 +724,7 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
     step_grids(h, grids);
     const bool graphs = h->use_graphs && !h->kernel_timing && !c.halt_on_kmax;
     if (h->kernel_timing) {
-        const size_t need = (size_t)nsteps * 6;
+        const size_t need = (size_t)nsteps * 8;
         while (h->tev.size() < need) {
             hipEvent_t e;
             HIP_TRY(hipEventCreate(&e));
@@ -734,7 +733,7 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
     }
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
     for (int i = 0; i < nsteps; ++i) {
-        hipEvent_t *tev = h->kernel_timing ? h->tev.data() + (size_t)i * 6 : nullptr;
+        hipEvent_t *tev = h->kernel_timing ? h->tev.data() + (size_t)i * 8 : nullptr;
         hipError_t e = graphs ? graph_step(h, grids) : launch_step(h, grids, tev);
         if (e != hipSuccess) return fail(MPF_EHIP, "step launch: %s", hipGetErrorString(e));
         h->cur ^= 1;
@@ -751,12 +750,14 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
         HIP_TRY(hipStreamSynchronize(h->side2));
         if (!h->use_side) HIP_TRY(hipStreamSynchronize(h->stream));
         for (int i = 0; i < nsteps; ++i) {
-            hipEvent_t *t = h->tev.data() + (size_t)i * 6;
+            // classification (with the tile pass), merge, one-solid
+            // interface, many-phase (events of launch_step_kernels)
+            hipEvent_t *t = h->tev.data() + (size_t)i * 8;
             float k[4];
             HIP_TRY(hipEventElapsedTime(&k[0], t[0], t[1]));
             HIP_TRY(hipEventElapsedTime(&k[1], t[1], t[2]));
-            HIP_TRY(hipEventElapsedTime(&k[2], t[2], t[3]));
-            HIP_TRY(hipEventElapsedTime(&k[3], t[4], t[5]));
+            HIP_TRY(hipEventElapsedTime(&k[2], t[3], t[4]));
+            HIP_TRY(hipEventElapsedTime(&k[3], t[5], t[6]));
             for (int j = 0; j < 4; ++j) h->kern_ms[j] += k[j];
         }
         h->kern_steps += nsteps;

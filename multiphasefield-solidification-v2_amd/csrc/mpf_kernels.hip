@@ -693,7 +693,7 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
     // list and chunk.  s_off is double-buffered on the chunk parity: a wave
     // can only overwrite a buffer after every wave passed the barrier of the
     // chunk in between, by which time that buffer has been read.
-    constexpr int NQ = 1 + NFB; // pushed lists: WL_GENERAL, then the front buckets
+    constexpr int NQ = 1 + NFB; // pushed lists: WL_GENERAL, then the late interface buckets
     __shared__ int s_off[2][NQ][4];
     __shared__ int s_base[2][NQ];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -736,34 +736,33 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
                 o[w] = tot;
                 tot += v;
             }
-            const int list = threadIdx.x == 0 ? WL_GENERAL : WL_FRONT + threadIdx.x - 1;
+            const int list = threadIdx.x == 0 ? WL_GENERAL : WL_LATE + threadIdx.x - 1;
             s_base[par][threadIdx.x] = tot ? atomicAdd(wl_counter(a, list, a.step), tot) : 0;
         }
         __syncthreads();
         if (myq >= 0) {
-            const int list = myq == 0 ? WL_GENERAL : WL_FRONT + myq - 1;
+            const int list = myq == 0 ? WL_GENERAL : WL_LATE + myq - 1;
             const unsigned long long below = (1ull << lane) - 1ull;
             wl_list(a, list)[s_base[par][myq] + s_off[par][myq][wave] + __popcll(bq & below)] = (int)c;
         }
     }
 }
 
-// Pass 3: solid-liquid interface cells by bucket: the cells of bucket
-// NS - 1 have an active liquid and exactly NS active solids (both the
+// Pass 3: solid-liquid interface cells by bucket: the cells of a bucket of
+// NS solids have an active liquid and exactly NS active solids (both the
 // classification and the merge pass bucket by that count), so their update is
 // compiled for that set with no presence tests and only the registers it
-// needs.  k_front1 takes the one-solid bucket (nearly all interface cells);
-// k_front_multi the buckets of 2, 3 and 4 solids (triple lines), in block
-// ranges of one launch, concurrently with k_front1.
+// needs.  k_front1 takes the classification pass's one-solid bucket (nearly
+// all interface cells); k_front_multi, concurrently, the classification
+// pass's buckets of 2-4 solids and every bucket of the merge pass.
 static_assert(NFB == FAST_A, "interface buckets cover the register path's solid counts");
-// G lanes per cell (G >= NS; see cell_update): lane 0 of each group notes
-// the statistics and writes the cell.
 template <int DIM, int TC, int B, int NS, int G>
-__device__ __forceinline__ void front_bucket(const StepArgs<Real> &a, const uint32_t *s_coff, int blk, int nblk)
+__device__ __forceinline__ void front_bucket(const StepArgs<Real> &a, const uint32_t *s_coff, int wl, int blk,
+                                             int nblk)
 {
     const Consts<Real> &K = a.K;
-    const int n = *wl_counter(a, WL_FRONT + NS - 1, a.step);
-    const int *list = wl_list(a, WL_FRONT + NS - 1);
+    const int n = *wl_counter(a, wl, a.step);
+    const int *list = wl_list(a, wl);
     const size_t ncell = a.in.stride; // slot stride
     const bool lead = (threadIdx.x & (G - 1)) == 0;
     if (blk == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->front_cells, (unsigned long long)n);
@@ -789,43 +788,62 @@ __global__ __launch_bounds__(128) void k_front1(StepArgs<Real> a)
     if (a.halt_mode && a.stats->halt_step < a.step) return;
     __shared__ uint32_t s_coff[B ? MAX_INST : 1];
     load_cell_offsets<B>(a, s_coff);
-    front_bucket<DIM, TC, B, 1, 1>(a, s_coff, blockIdx.x, gridDim.x);
+    front_bucket<DIM, TC, B, 1, 1>(a, s_coff, WL_FRONT, blockIdx.x, gridDim.x);
 }
 
-// The grid (at least 4 blocks) is split over the buckets of 2, 3 and 4
-// solids in proportion to cells x solids, with at least one block for a
-// non-empty bucket; every block computes the same split from the counts.
-// (Lane groups per cell measured slower here: the bucket of two solids is
-// large enough to keep every lane busy with a cell of its own.)
+// The grid (at least 8 blocks) is split over the seven lists in proportion
+// to cells x solids, with at least one block for a non-empty list; every
+// block computes the same split from the counts.  (Lane groups per cell
+// measured slower here: there are enough cells to keep every lane busy with
+// a cell of its own.)
 template <int DIM, int TC, int B>
 __global__ __launch_bounds__(128) void k_front_multi(StepArgs<Real> a)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
     __shared__ uint32_t s_coff[B ? MAX_INST : 1];
     load_cell_offsets<B>(a, s_coff);
+    constexpr int NL = 2 * NFB - 1;
+    // lists and their solid counts: the classification pass's buckets 2-4,
+    // then the merge pass's buckets 1-4
+    constexpr int wls[NL] = {WL_FRONT + 1, WL_FRONT + 2, WL_FRONT + 3, WL_LATE, WL_LATE + 1, WL_LATE + 2,
+                             WL_LATE + 3};
+    constexpr int nss[NL] = {2, 3, 4, 1, 2, 3, 4};
     const int g = gridDim.x;
-    long long w[3], wt = 0;
-    int gb[3], sum = 0, big = 0;
+    long long w[NL], wt = 0;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        w[q] = (long long)(q + 2) * *wl_counter(a, WL_FRONT + q + 1, a.step);
+    for (int q = 0; q < NL; ++q) {
+        w[q] = (long long)nss[q] * *wl_counter(a, wls[q], a.step);
         wt += w[q];
     }
     if (wt == 0) return;
+    int gb[NL], sum = 0;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < NL; ++q) {
         gb[q] = w[q] > 0 ? max(1, (int)(g * w[q] / wt)) : 0;
         sum += gb[q];
-        if (w[q] > w[big]) big = q;
     }
-    gb[big] += g - sum; // stays >= 1: the largest bucket holds at least a third of g
-    const int b = blockIdx.x;
-    if (b < gb[0])
-        front_bucket<DIM, TC, B, 2, 1>(a, s_coff, b, gb[0]);
-    else if (b < gb[0] + gb[1])
-        front_bucket<DIM, TC, B, 3, 1>(a, s_coff, b - gb[0], gb[1]);
-    else
-        front_bucket<DIM, TC, B, 4, 1>(a, s_coff, b - gb[0] - gb[1], gb[2]);
+    // give the rest to (or take the excess from) the largest shares; at most
+    // NL - 1 lists were raised to one block, and g >= 8 > NL - 1
+    while (sum != g) {
+        int big = 0;
+#pragma unroll
+        for (int q = 1; q < NL; ++q)
+            if (gb[q] > gb[big]) big = q;
+        const int d = sum < g ? 1 : -1;
+        gb[big] += d;
+        sum += d;
+    }
+    int b = blockIdx.x, q = 0;
+    while (q < NL - 1 && b >= gb[q]) b -= gb[q++];
+    switch (q) {
+    case 0: front_bucket<DIM, TC, B, 2, 1>(a, s_coff, wls[0], b, gb[0]); break;
+    case 1: front_bucket<DIM, TC, B, 3, 1>(a, s_coff, wls[1], b, gb[1]); break;
+    case 2: front_bucket<DIM, TC, B, 4, 1>(a, s_coff, wls[2], b, gb[2]); break;
+    case 3: front_bucket<DIM, TC, B, 1, 1>(a, s_coff, wls[3], b, gb[3]); break;
+    case 4: front_bucket<DIM, TC, B, 2, 1>(a, s_coff, wls[4], b, gb[4]); break;
+    case 5: front_bucket<DIM, TC, B, 3, 1>(a, s_coff, wls[5], b, gb[5]); break;
+    default: front_bucket<DIM, TC, B, 4, 1>(a, s_coff, wls[6], b, gb[6]); break;
+    }
 }
 
 // Pass 4: cells with more active grains than the merge pass holds.  Cells
@@ -1392,18 +1410,20 @@ using namespace mpf::MPF_CAT(impl, MPF_SFX);
 
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
-// One step.  With side streams, the many-phase kernel and the interface
-// cells with 2-4 solids run concurrently with the one-solid interface kernel
-// (all of them only read the input state and write disjoint cells); the
-// solver stream waits for both before the step ends.  tev (optional): 6
-// timing events, recorded around the kernels: [0] before k_tiles, [1] after
-// k_light, [2] after k_merge, [3] after k_front1 (and, without side streams,
-// k_front_multi), and [4]/[5] around k_step_general.
+// One step.  With side streams, after the merge pass the many-phase kernel
+// and the interface cells of several solids run concurrently with the
+// one-solid interface kernel (every one of these kernels only reads the input
+// state and writes cells no other writes); the solver stream waits for both
+// before the step ends.  (Running k_front1 concurrently with the merge pass
+// instead measured slower: both are throughput-bound there.)  tev
+// (optional): 8 timing events: [0] before k_tiles, [1] after k_light, [2]
+// after k_merge, [3]/[4] around k_front1 and [5]/[6] around k_step_general
+// ([7] unused).
 template <int DIM, int TC, int B>
 static void launch_step_kernels(const StepArgs<Real> &a, unsigned grid, const int *grids, hipStream_t st,
                                 const StepStreams &S, hipEvent_t *tev)
 {
-    const int gm = max(grids[1], 4);
+    const int gm = max(grids[1], 8);
     if (tev) hipEventRecord(tev[0], st);
     hipLaunchKernelGGL((k_tiles<DIM, B>), dim3(nblk((size_t)a.n_tiles, 256)), dim3(256), 0, st, a);
     // 32-bit offsets when every byte offset of the light pass fits (4- or
@@ -1420,23 +1440,25 @@ static void launch_step_kernels(const StepArgs<Real> &a, unsigned grid, const in
         hipEventRecord(S.ev_merged, st);
         hipStreamWaitEvent(S.side, S.ev_merged, 0);
         hipStreamWaitEvent(S.side2, S.ev_merged, 0);
-        if (tev) hipEventRecord(tev[4], S.side);
-        hipLaunchKernelGGL((k_step_general<DIM, TC, B>), dim3(grids[2]), dim3(64), 0, S.side, a);
         if (tev) hipEventRecord(tev[5], S.side);
+        hipLaunchKernelGGL((k_step_general<DIM, TC, B>), dim3(grids[2]), dim3(64), 0, S.side, a);
+        if (tev) hipEventRecord(tev[6], S.side);
         hipEventRecord(S.ev_general, S.side);
         hipLaunchKernelGGL((k_front_multi<DIM, TC, B>), dim3(gm), dim3(128), 0, S.side2, a);
         hipEventRecord(S.ev_front, S.side2);
-        hipLaunchKernelGGL((k_front1<DIM, TC, B>), dim3(grids[1]), dim3(128), 0, st, a);
         if (tev) hipEventRecord(tev[3], st);
+        hipLaunchKernelGGL((k_front1<DIM, TC, B>), dim3(grids[1]), dim3(128), 0, st, a);
+        if (tev) hipEventRecord(tev[4], st);
         hipStreamWaitEvent(st, S.ev_general, 0);
         hipStreamWaitEvent(st, S.ev_front, 0);
     } else {
-        hipLaunchKernelGGL((k_front1<DIM, TC, B>), dim3(grids[1]), dim3(128), 0, st, a);
-        hipLaunchKernelGGL((k_front_multi<DIM, TC, B>), dim3(gm), dim3(128), 0, st, a);
         if (tev) hipEventRecord(tev[3], st);
+        hipLaunchKernelGGL((k_front1<DIM, TC, B>), dim3(grids[1]), dim3(128), 0, st, a);
         if (tev) hipEventRecord(tev[4], st);
-        hipLaunchKernelGGL((k_step_general<DIM, TC, B>), dim3(grids[2]), dim3(64), 0, st, a);
+        hipLaunchKernelGGL((k_front_multi<DIM, TC, B>), dim3(gm), dim3(128), 0, st, a);
         if (tev) hipEventRecord(tev[5], st);
+        hipLaunchKernelGGL((k_step_general<DIM, TC, B>), dim3(grids[2]), dim3(64), 0, st, a);
+        if (tev) hipEventRecord(tev[6], st);
     }
 }
 

@@ -30,9 +30,12 @@ constexpr int NO_HALT = 0x7fffffff; // halt_step value while no step has overflo
 // NFB buckets by their number of active solids (WL_FRONT + n - 1), so that a
 // wavefront of the interface pass evaluates the same number of solid-liquid
 // pairs on every lane.
+// The classification pass fills WL_FRONT.., the merge pass WL_LATE.., so
+// that the interface cells of the first can be updated while the merge pass
+// runs.
 constexpr int NFB = 4;
-enum { WL_MERGE = 0, WL_GENERAL = 1, WL_FRONT = 2 };
-constexpr int NWL = WL_FRONT + NFB;
+enum { WL_MERGE = 0, WL_GENERAL = 1, WL_FRONT = 2, WL_LATE = WL_FRONT + NFB };
+constexpr int NWL = WL_LATE + NFB;
 // Counters of a step: one per work list, then the tile list of k_light.
 constexpr int WC_TILES = NWL;
 constexpr int NWC = NWL + 1;
@@ -103,7 +106,7 @@ struct StepArgs {
     // classification pass over their cells finishes cells whose stencil holds
     // only liquid or only one grain and queues the rest (WL_MERGE); the merge
     // pass finishes cells that need no anisotropy terms and queues
-    // solid-liquid interface cells (WL_FRONT) and cells with more than the
+    // solid-liquid interface cells (WL_LATE) and cells with more than the
     // register capacity of active grains (WL_GENERAL).
     int *wl_base;    // NWL lists of ncell entries each, list i at wl_base + i*ncell
     int *wl_count;   // [NWC][2] counters, ping-pong on the step parity

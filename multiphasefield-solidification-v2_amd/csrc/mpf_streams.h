@@ -8,7 +8,7 @@ namespace mpf {
 // stream.
 struct StepStreams {
     hipStream_t side;      // many-phase kernel
-    hipStream_t side2;     // interface cells with 2-4 solids
+    hipStream_t side2;     // interface cells of several solids
     hipEvent_t ev_merged;  // merge pass done (fork)
     hipEvent_t ev_general; // joins of the side streams
     hipEvent_t ev_front;
