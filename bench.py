@@ -6,8 +6,8 @@ KMAX = 50 (config_3d.yaml physics).  One "step" is one full time step
 
 Prints ONE JSON line: MLUPS of the FP32 solver (value), the FP64 solver's
 MLUPS and its agreement with FP32 on the FP64 observables, the HBM roofline
-of the fused step kernel, and a CPU baseline (the FP64 CPU oracle in the
-reference's dense layout, timed on a bounded slab of the same workload).
+of the step, its vector-ALU issue, and a CPU baseline (the FP64 CPU oracle in
+the reference's dense layout, timed on a bounded slab of the same workload).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -97,6 +97,27 @@ def alg_bytes_per_step(ob0, ob1, bv, ncell, nz):
     (id + value) read once and written once, the per-plane temperature."""
     slots = 0.5 * (ob0["stored_slots"] + ob1["stored_slots"])
     return 2.0 * ncell * (bv + 1) + 2.0 * slots * (bv + 2) + 2.0 * nz * 8, slots / ncell
+
+
+def touched_bytes_per_step(light_cells, slots, bv, nz):
+    """Compulsory bytes of the cells a step actually computes: the cells of
+    the tile planes the classification pass computes (the other planes hold
+    liquid exactly 1 with liquid 1 around them and are neither read nor
+    written) read and written once, every stored grain slot (all of them lie
+    in those planes) read and written once, the per-plane temperature."""
+    return 2.0 * light_cells * (bv + 1) + 2.0 * slots * (bv + 2) + 2.0 * nz * 8
+
+
+def measured_valu(path):
+    """Vector-ALU instructions per step from the committed rocprofv3 PMC
+    summary (SQ_INSTS_VALU per dispatch)."""
+    if not path or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return {"valu_wave_instr_per_step": t["valu_wave_instr_per_step"],
+            "valu_time_at_full_issue_us": t["valu_time_at_full_issue_us_per_step"],
+            "model": t["model"], "source": os.path.relpath(path, ROOT)}
 
 
 def measured_traffic(path):
@@ -210,6 +231,8 @@ def main():
                     help="also run one 256 MiB device-to-device copy (reference for HBM counters)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "pmc_traffic_512x512x256_fp32.json"),
                     help="rocprofv3 PMC traffic summary to report as roofline.traffic")
+    ap.add_argument("--valu-json", default=os.path.join(ROOT, "profiles", "valu_512x512x256_fp32.json"),
+                    help="rocprofv3 PMC vector-ALU summary to report as roofline.valu")
     ap.add_argument("--graphs", type=int, default=0, help="replay steps from HIP graphs (1) or launch eagerly (0)")
     ap.add_argument("--autotune", type=int, default=4,
                     help="time launch shapes for this many steps each after the warm-up (0: default shape)")
@@ -243,7 +266,10 @@ def main():
     log(f"fp32 launch {r32['launch']} graphs={bool(args.graphs)} autotune={r32['tuned']}")
     B32, slot32 = alg_bytes_per_step(r32["ob0"], r32["ob1"], 4, ncell, NZ)
     achieved = B32 / (ms32 * 1e-3) / 1e9
+    light_per_step = r32["stats"]["light_cells"] / args.steps
+    T32 = touched_bytes_per_step(light_per_step, slot32 * ncell, 4, NZ)
     traffic = measured_traffic(args.traffic_json)
+    valu = measured_valu(args.valu_json)
 
     out = {
         "metric": METRIC, "value": round(ncell * args.steps / (r32["ms"] * 1e-3) / 1e6, 2), "unit": "MLUPS",
@@ -259,8 +285,14 @@ def main():
                      "traffic": traffic["bytes_per_step"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None,
                      "alg_bytes_per_step": round(B32), "mean_stored_slots_per_cell": round(slot32, 4),
-                     "kernel": "one step: k_light, k_merge, k_front on the solver stream, k_step_general "
-                               "on a side stream concurrent with k_front",
+                     "alg_bytes_note": "every cell's state read and written once per step (a dense sweep's "
+                                       "compulsory bytes); the step skips the liquid planes that cannot change",
+                     "touched_cells_per_step": round(light_per_step),
+                     "touched_bytes_per_step": round(T32),
+                     "touched_achieved": round(T32 / (ms32 * 1e-3) / 1e9, 1),
+                     "valu": valu,
+                     "kernel": "one step: k_tiles, k_light, k_merge, then k_front1 on the solver stream "
+                               "concurrently with k_front_multi and k_step_general on two side streams",
                      "kernel_ms_live": {k: round(v, 4) for k, v in r32["kernel_ms"].items()}},
         "wall_s_timed": round(r32["wall"], 3),
         "front_mean_cells": r32["ob1"]["front_mean"],
