@@ -284,3 +284,43 @@ def test_fp64_classification_kernels_bit_exact(shape):
         print(shape, "max |diff| =", np.abs(got - ref).max())
         assert np.array_equal(got, ref)
         _assert_apt_equal(s, ref)
+
+
+@pytest.mark.parametrize("kmax", [2, 3])
+def test_fp64_small_kmax_clamp_bit_exact(kmax):
+    """KMAX below the number of active phases: the checked APT keeps the
+    lowest ids (liquid first) and the update runs on that set.  The
+    classification pass sends an interface cell straight to its bucket only
+    when no clamp applies, so clamped cells take the merge pass.  Bit-identical
+    to the oracle, overflow flag included."""
+    c = cases.voronoi_case(n_solid=6, seed=5, qseed=6)
+    steps = 6
+    ref, _, _, st = O.run(c["phi"], c["temp"], steps, c["tables"], c["scalars"], kmax=kmax, temp_f32=True)
+    assert st[0] == 1  # the case does overflow
+    with _solver_for(c, "fp64", temp_f32=True, kmax=kmax, slot_capacity=6) as s:
+        assert s.step(steps) == steps
+        assert np.array_equal(s.phi(), ref)
+        assert s.stats()["kmax_overflow"] == 1
+
+
+def test_fp64_negative_upload_values_bit_exact():
+    """An uploaded phi may hold small negative solid values (the slot layout
+    stores every non-zero value).  A negative value at a cell makes that grain
+    inactive there although it is stored, so the classification pass does not
+    route such cells directly; they take the merge pass.  Bit-identical to the
+    oracle."""
+    c = cases.voronoi_case(n_solid=4)
+    phi = c["phi"].copy()
+    # grains absent at a cell but present at an x neighbour: store -1e-4
+    pos = phi[1:] > 0
+    nb = np.roll(pos, 1, axis=1) | np.roll(pos, -1, axis=1)
+    cand = np.argwhere(nb & ~pos & (phi[0][None] > 0.2))
+    assert len(cand) > 10
+    for g, l, m, k in cand[::3]:
+        phi[g + 1, l, m, k] = -1e-4
+    c2 = dict(c, phi=phi)
+    steps = 4
+    ref, _, _, _ = O.run(phi, c["temp"], steps, c["tables"], c["scalars"], kmax=50, temp_f32=True)
+    with _solver_for(c2, "fp64", temp_f32=True) as s:
+        assert s.step(steps) == steps
+        assert np.array_equal(s.phi(), ref)
