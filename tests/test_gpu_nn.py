@@ -36,6 +36,8 @@ def _host(t):
     ((1, 8, 6, 10), 32, 32, 27, (1, 1, 0)),
     ((2, 5, 4, 7), 64, 96, 27, (0, 0, 0)),
     ((1, 4, 4, 4), 96, 32, 27, (1, 1, 1)),
+    ((1, 3, 9, 21), 32, 64, 27, (1, 1, 0)),
+    ((2, 5, 3, 16), 128, 64, 27, (0, 1, 0)),
     ((3, 6, 5, 9), 64, 32, 1, (1, 1, 0)),
 ])
 def test_conv_forward_and_backward(dims, cin, cout, ntap, periodic):
