@@ -108,7 +108,8 @@ struct mpf_handle {
     void *tab = nullptr; // per instance: wij, aij, mij, n111 contiguous
     int *wl = nullptr;        // NWL work lists of ncell entries (mpf_state.h)
     int *wl_count = nullptr;  // [NWC][2]
-    int *tiles = nullptr;     // tile list of the classification pass (n_tiles entries)
+    int *tiles = nullptr;     // work items of the classification pass (n_tiles * LZC / 4 entries)
+    uint16_t *item_busy = nullptr; // their tiles' plane masks
     Stats *stats = nullptr;
     int cur = 0, tcur = 0;
     int step_counter = 0;
@@ -193,6 +194,7 @@ extern "C" void mpf_destroy(mpf_handle *h)
     hipFree(h->wl);
     hipFree(h->wl_count);
     hipFree(h->tiles);
+    hipFree(h->item_busy);
     hipFree(h->stats);
     hipFree(h->inst_d);
     drop_graphs(h);
@@ -310,7 +312,8 @@ static int create_handle(const mpf_config &c, const std::vector<Inst> &grids, in
     TRY_RC(dev_alloc(h, &h->tab, inst.size() * h->tab_stride * h->rsz));
     TRY_RC(dev_alloc(h, (void **)&h->wl, (size_t)NWL * ncell * sizeof(int)));
     TRY_RC(dev_alloc(h, (void **)&h->wl_count, 2 * NWC * sizeof(int)));
-    TRY_RC(dev_alloc(h, (void **)&h->tiles, (size_t)h->n_tiles * sizeof(int)));
+    TRY_RC(dev_alloc(h, (void **)&h->tiles, (size_t)h->n_tiles * (LZC / 4) * sizeof(int)));
+    TRY_RC(dev_alloc(h, (void **)&h->item_busy, (size_t)h->n_tiles * (LZC / 4) * sizeof(uint16_t)));
     TRY_RC(dev_alloc(h, (void **)&h->stats, sizeof(Stats)));
     {
         TRY_RC(dev_alloc(h, (void **)&h->inst_d, inst.size() * sizeof(Inst)));
@@ -645,7 +648,11 @@ static hipError_t launch_one_step(mpf_handle *h, const int *grids, hipEvent_t *t
     a.wl_base = h->wl;
     a.wl_count = h->wl_count;
     a.tiles = h->tiles;
+    a.item_busy = h->item_busy;
     a.n_tiles = h->n_tiles;
+    // planes per classification work item: shorter ranges on grids with few
+    // tiles, so that more blocks share the tiles with work
+    a.light_q = (c.dim == 2 || h->n_tiles >= 4096) ? LZC : (h->n_tiles >= 1024 ? LZC / 2 : LZC / 4);
     a.n_temp = h->nT;
     a.step = h->step_counter;
     a.halt_mode = c.halt_on_kmax;

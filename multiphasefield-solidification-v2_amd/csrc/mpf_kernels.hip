@@ -235,9 +235,12 @@ __device__ __forceinline__ uint32_t tile_busy(const StepArgs<Real> &a, int tx, i
     return (cm | (cm >> 1) | (cm >> 2) | nbr | (uint32_t)a.occ_out[tile_of(tx, ty, tz)]) & ((1u << np) - 1u);
 }
 
-// Pass 0: one lane per tile lists the tiles with a plane to compute for
-// k_light, counts their cells, and records that the liquid phase was active
-// on the planes left as they are.
+// Pass 0: one lane per tile lists the work items of k_light, counts their
+// cells, and records that the liquid phase was active on the planes left as
+// they are.  An item is one range of light_q planes (16, 8 or 4: small grids
+// take shorter ranges, so that more blocks share the few tiles with work)
+// holding at least one plane to compute, listed as tile * (16 / light_q) +
+// range with the tile's plane mask.
 template <int DIM, int B>
 __global__ __launch_bounds__(256) void k_tiles(StepArgs<Real> a0)
 {
@@ -258,13 +261,20 @@ __global__ __launch_bounds__(256) void k_tiles(StepArgs<Real> a0)
         cells = (unsigned long long)__popc(busy) * (unsigned)(min(LTX, a.K.nx - tx * LTX) * min(LTY, a.K.ny - ty * LTY));
     }
     const int lane = threadIdx.x & 63;
-    const unsigned long long ball = __ballot(busy != 0u);
-    if (ball != 0ull) {
+    const int q = a0.light_q, nsub = LZC / q;
+    for (int r = 0; r < nsub; ++r) {
+        const bool has = ((busy >> (r * q)) & ((1u << q) - 1u)) != 0u;
+        const unsigned long long ball = __ballot(has);
+        if (ball == 0ull) continue;
         const int leader = __ffsll((long long)ball) - 1;
         int base = 0;
         if (lane == leader) base = atomicAdd(wl_counter(a0, WC_TILES, a0.step), __popcll(ball));
         base = __shfl(base, leader, 64);
-        if (busy != 0u) a0.tiles[base + __popcll(ball & ((1ull << lane) - 1ull))] = t;
+        if (has) {
+            const int e = base + __popcll(ball & ((1ull << lane) - 1ull));
+            a0.tiles[e] = t * nsub + r;
+            a0.item_busy[e] = (uint16_t)busy;
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) cells += __shfl_xor(cells, o, 64);
@@ -372,10 +382,13 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
         i1 = (int)((long long)nlist * (x + 1) / 8);
         di = gridDim.x >> 3;
     }
+    const int nsub = LZC / a0.light_q;
     for (int li = i0; li < i1; li += di) {
         StepArgs<Real> a = a0;
         size_t cell_base;
-        const int lb = enter_tile<B>(a, a0.tiles[li], cell_base);
+        const int item = a0.tiles[li];
+        const int lb = enter_tile<B>(a, item / nsub, cell_base);
+        const int sub = item % nsub;
         const Consts<Real> &K = a.K;
         const int tx = lb % a.ntx, ty = (lb / a.ntx) % a.nty, tz = lb / (a.ntx * a.nty);
         const int lx = threadIdx.x % LTX, ly = threadIdx.x / LTX;
@@ -384,7 +397,9 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
         const int l = l0 + lx, m = m0 + ly;
         const bool valid = (lx < w) && (ly < h);
         const int k0 = tz * ZC;
-        const int k1 = (DIM == 3) ? min(k0 + ZC, K.nz) : 1;
+        // this item's planes [ks, ke) of the tile's [k0, k0 + ZC)
+        const int ks = (DIM == 3) ? k0 + sub * a0.light_q : 0;
+        const int ke = (DIM == 3) ? min(ks + a0.light_q, K.nz) : 1;
         const Off plane = (Off)K.nx * (Off)K.ny;
         if (threadIdx.x == 0) {
             s_qn = 0;
@@ -394,8 +409,14 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
         }
         if (threadIdx.x < NFB) s_bn[threadIdx.x] = 0;
         auto tile_of = [&](int ttx, int tty, int ttz) { return ((size_t)ttz * a.nty + tty) * a.ntx + ttx; };
-        uint32_t colmask;
-        const uint32_t busy = tile_busy<DIM>(a, tx, ty, tz, colmask);
+        // own column's occupancy over planes k0-1 .. k0+LZC (bit kk-k0+1)
+        const int nzc = (DIM == 3) ? (K.nz + LZC - 1) / LZC : 1;
+        uint32_t colmask = (uint32_t)a.occ_in[tile_of(tx, ty, tz)] << 1;
+        if (DIM == 3 && tz > 0) colmask |= ((uint32_t)a.occ_in[tile_of(tx, ty, tz - 1)] >> (LZC - 1)) & 1u;
+        if (DIM == 3 && tz + 1 < nzc) colmask |= ((uint32_t)a.occ_in[tile_of(tx, ty, tz + 1)] & 1u) << (LZC + 1);
+        // planes to compute (tile_busy, evaluated by k_tiles before this
+        // pass starts rewriting the output occupancy)
+        const uint32_t busy = a0.item_busy[li];
 
         // halo cells this thread loads each plane: x halo of its row, y halo
         // of its column (x periodic; y periodic in 3D, mirror in 2D)
@@ -441,31 +462,31 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
             return r;
         };
 
-        // ring[r & 3] holds plane k0 + r - 1 of the own column; hr/yr[r & 1]
-        // hold the halo of plane k0 + r
+        // ring[r & 3] holds plane ks + r - 1 of the own column; hr/yr[r & 1]
+        // hold the halo of plane ks + r
         ColCell ring[4], hr[2], yr[2];
-        ring[0] = (DIM == 3) ? own_cell(bc_mm(k0, K.nz)) : ColCell{};
-        ring[1] = own_cell(k0);
-        ring[2] = (DIM == 3) ? own_cell(bc_mp(k0, K.nz)) : ColCell{};
+        ring[0] = (DIM == 3) ? own_cell(bc_mm(ks, K.nz)) : ColCell{};
+        ring[1] = own_cell(ks);
+        ring[2] = (DIM == 3) ? own_cell(bc_mp(ks, K.nz)) : ColCell{};
         ring[3] = ColCell{};
-        hr[0] = halo_x(k0);
-        yr[0] = halo_y(k0);
+        hr[0] = halo_x(ks);
+        yr[0] = halo_y(ks);
         hr[1] = yr[1] = ColCell{};
         finish_cell(ring[0]);
         finish_cell(ring[1]);
         uint32_t occ_bits = 0;
         int nb = 0; // planes computed so far: their parity selects the LDS buffer
 
-        for (int kb = k0; kb < k1; kb += 4) {
+        for (int kb = ks; kb < ke; kb += 4) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int k = kb + j;
-                if (k >= k1) break;
+                if (k >= ke) break;
                 ColCell &cp = ring[j & 3], &cc = ring[(j + 1) & 3], &cn = ring[(j + 2) & 3];
                 ColCell &hx = hr[j & 1], &hy = yr[j & 1];
                 // loads of the next plane (the last plane reloads itself; unused)
                 if (DIM == 3) {
-                    const int kn = min(k + 1, k1 - 1);
+                    const int kn = min(k + 1, ke - 1);
                     ring[(j + 3) & 3] = own_cell(bc_mp(kn, K.nz));
                     hr[(j + 1) & 1] = halo_x(kn);
                     yr[(j + 1) & 1] = halo_y(kn);
@@ -610,7 +631,19 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
         if (threadIdx.x == 0) {
             if (s_any && *((volatile int *)&a.stats->max_active) < 1) atomicMax(&a.stats->max_active, 1);
             s_qbase = nq > 0 ? atomicAdd(wl_counter(a, WL_MERGE, a.step), nq) : 0;
-            a.occ_out[tile_of(tx, ty, tz)] = (uint16_t)s_occ;
+            if (nsub == 1) {
+                a.occ_out[tile_of(tx, ty, tz)] = (uint16_t)s_occ;
+            } else {
+                // other items write the other plane ranges of this tile's
+                // word: replace only this range's bits, in the aligned
+                // 32-bit word that holds the 16-bit entry
+                uint16_t *o = a.occ_out + tile_of(tx, ty, tz);
+                const int sh = (int)(((uintptr_t)o >> 1) & 1u) * 16;
+                unsigned int *wd = (unsigned int *)((uintptr_t)o & ~(uintptr_t)3);
+                const unsigned int rng = (((1u << a0.light_q) - 1u) << (sub * a0.light_q)) << sh;
+                atomicAnd(wd, ~rng);
+                atomicOr(wd, (s_occ << sh) & rng);
+            }
         }
         if (threadIdx.x >= 64 && threadIdx.x < 64 + NFB) {
             const int q = threadIdx.x - 64;
@@ -1469,8 +1502,8 @@ hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const 
 {
     StepArgs<Real> a = *args;
     // persistent classification grid: four blocks per CU of the 256, at most
-    // one per tile
-    const unsigned grid = (unsigned)min(a.n_tiles, 1024);
+    // one per work item
+    const unsigned grid = (unsigned)min(a.n_tiles * (LZC / a.light_q), 1024);
     if (a.n_inst > 0) {
         // a batch of 3D grids: one launch per kernel covers every instance
         if (a.tab_class)

@@ -110,7 +110,9 @@ struct StepArgs {
     // register capacity of active grains (WL_GENERAL).
     int *wl_base;    // NWL lists of ncell entries each, list i at wl_base + i*ncell
     int *wl_count;   // [NWC][2] counters, ping-pong on the step parity
-    int *tiles;      // tiles with a plane to compute (k_tiles -> k_light), n_tiles entries
+    int *tiles;      // work items of k_light (k_tiles -> k_light), n_tiles * LZC / 4 entries
+    uint16_t *item_busy; // planes to compute of each item's tile
+    int light_q;     // planes per work item: 16, 8 or 4
     size_t n_temp;   // temperature entries of all instances (planes, or cells when temp_dense)
     int step;        // step index (for the ping-pong counters and halt)
     int halt_mode;   // stop at the first KMAX overflow
