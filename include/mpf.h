@@ -194,13 +194,16 @@ int mpf_batch_download_phi_device(mpf_handle *h, int grid, void *phi, int dtype)
 /* ---- launch shape, graphs and autotuning (no counterpart in the reference,
  * which launches each kernel once per step from Python).
  * mpf_set_launch: blocks of the merge, interface and many-phase kernels (0
- * keeps the default for that kernel) and whether the many-phase kernel runs
- * on a side stream concurrent with the interface kernel.  mpf_get_launch:
- * out[4] = the three block counts in use and the side-stream flag.
+ * keeps the default for that kernel) and how the passes after the merge pass
+ * run (side_stream): 0 one after another on the solver stream, 1 the
+ * many-phase and multi-solid interface kernels on side streams concurrent
+ * with the one-solid interface kernel (default from 4096 classification
+ * tiles, 512x512x128 cells, up), 2 all three as one launch (default below).
+ * mpf_get_launch: out[4] = the three block counts in use and that mode.
  * mpf_set_graphs: replay each step from a HIP graph captured on first use
  * (one per state parity); not with halt_on_kmax, and kernel timing bypasses
  * it.  mpf_autotune: times launch shapes on the current state (trial_steps
- * per shape, coordinate descent over the side stream and the three block
+ * per shape, coordinate descent over that mode and the three block
  * counts), keeps the fastest and restores the state; out (optional, 5
  * ints): the chosen shape and its time per step in microseconds. */
 int mpf_set_launch(mpf_handle *h, int merge_blocks, int front_blocks, int general_blocks, int side_stream);

@@ -124,7 +124,7 @@ struct mpf_handle {
     // launch shape: blocks of the merge, interface and many-phase kernels
     // (0: from the grid size) and the side stream for the many-phase kernel
     int grid_req[3] = {0, 0, 0};
-    int use_side = 1;
+    int use_side = 2;     // 0: one stream in sequence, 1: side streams, 2: one interface launch
     // captured steps, one per (state buffer, temperature buffer, step parity)
     int use_graphs = 0;
     hipGraphExec_t gexec[8] = {};
@@ -222,6 +222,12 @@ static int reset_stats_dev(mpf_handle *h)
     return 0;
 }
 
+// Mode of the passes after the merge pass (mpf_set_launch): side streams on
+// large grids (measured equal to one launch in FP32 and faster in FP64 at
+// 512x512x256), one launch on small ones (the cross-stream fork and join
+// cost more than the passes there: 128^3 steps 125 -> 82 us).
+static int default_mode(const mpf_handle *h) { return h->n_tiles >= 4096 ? 1 : 2; }
+
 // Common creation of a single grid (batch = 0, one instance with the
 // configuration's grid) and of a batch of 3D grids.
 static int create_handle(const mpf_config &c, const std::vector<Inst> &grids, int batch, mpf_handle **out)
@@ -265,6 +271,7 @@ static int create_handle(const mpf_config &c, const std::vector<Inst> &grids, in
     h->batch = batch;
     h->inst = inst;
     h->n_tiles = (int)ntile;
+    h->use_side = default_mode(h);
     h->nocc = ntile;
     h->rsz = c.precision == MPF_FP64 ? 8 : 4;
     const size_t nn = (size_t)c.n_phase * c.n_phase;
@@ -668,7 +675,8 @@ static hipError_t launch_one_step(mpf_handle *h, const int *grids, hipEvent_t *t
         a.n_inst = (int)h->inst.size();
         a.tab_stride = h->tab_stride;
     }
-    StepStreams ss{h->use_side ? h->side : nullptr, h->side2, h->ev_merged, h->ev_general, h->ev_front};
+    StepStreams ss{h->use_side == 2 ? 1 : 0, h->use_side == 1 ? h->side : nullptr, h->side2, h->ev_merged,
+                   h->ev_general, h->ev_front};
     if (sizeof(R) == 8)
         return mpf_launch_step_f64((const StepArgs<double> *)&a, c.dim, grids, h->stream, &ss, tev);
     return mpf_launch_step_f32((const StepArgs<float> *)&a, c.dim, grids, h->stream, &ss, tev);
@@ -755,7 +763,7 @@ extern "C" int mpf_step(mpf_handle *h, int nsteps)
     if (h->kernel_timing) {
         HIP_TRY(hipStreamSynchronize(h->side));
         HIP_TRY(hipStreamSynchronize(h->side2));
-        if (!h->use_side) HIP_TRY(hipStreamSynchronize(h->stream));
+        if (h->use_side != 1) HIP_TRY(hipStreamSynchronize(h->stream));
         for (int i = 0; i < nsteps; ++i) {
             // classification (with the tile pass), merge, one-solid
             // interface, many-phase (events of launch_step_kernels)
@@ -919,7 +927,7 @@ extern "C" int mpf_set_launch(mpf_handle *h, int merge_blocks, int front_blocks,
     h->grid_req[0] = merge_blocks;
     h->grid_req[1] = front_blocks;
     h->grid_req[2] = general_blocks;
-    h->use_side = side_stream ? 1 : 0;
+    h->use_side = side_stream < 0 ? 0 : (side_stream > 2 ? 2 : side_stream);
     drop_graphs(h);
     return 0;
 }
@@ -1000,11 +1008,11 @@ extern "C" int mpf_autotune(mpf_handle *h, int trial_steps, int *out)
         h->grid_req[0] = h->grid_req[1] = h->grid_req[2] = 0;
         step_grids(h, g);
         h->grid_req[0] = r0[0], h->grid_req[1] = r0[1], h->grid_req[2] = r0[2];
-        cur[0] = g[0], cur[1] = g[1], cur[2] = g[2], cur[3] = 1;
+        cur[0] = g[0], cur[1] = g[1], cur[2] = g[2], cur[3] = default_mode(h);
     }
     static const int cand_m[] = {512, 1024, 2048, 4096, 8192}, cand_g[] = {64, 128, 256, 512};
     const int *cands[4] = {cand_m, cand_m, cand_g, nullptr};
-    const int ncand[4] = {5, 5, 4, 2};
+    const int ncand[4] = {5, 5, 4, 3};
     double best = 1e300;
     int pick[4] = {cur[0], cur[1], cur[2], cur[3]};
     int rc = 0;
@@ -1032,7 +1040,7 @@ extern "C" int mpf_autotune(mpf_handle *h, int trial_steps, int *out)
         const int dmn = order[oi];
         for (int v = 0; v < ncand[dmn] && !rc; ++v) {
             int shape[4] = {pick[0], pick[1], pick[2], pick[3]};
-            const int val = dmn == 3 ? 1 - v : cands[dmn][v];
+            const int val = dmn == 3 ? 2 - v : cands[dmn][v];
             if (val == pick[dmn]) continue;
             shape[dmn] = val;
             rc = trial(shape);

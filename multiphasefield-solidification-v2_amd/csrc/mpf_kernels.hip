@@ -830,18 +830,14 @@ __global__ __launch_bounds__(128) void k_front1(StepArgs<Real> a)
 // measured slower here: there are enough cells to keep every lane busy with
 // a cell of its own.)
 template <int DIM, int TC, int B>
-__global__ __launch_bounds__(128) void k_front_multi(StepArgs<Real> a)
+__device__ __forceinline__ void front_multi(const StepArgs<Real> &a, const uint32_t *s_coff, int blk, int g)
 {
-    if (a.halt_mode && a.stats->halt_step < a.step) return;
-    __shared__ uint32_t s_coff[B ? MAX_INST : 1];
-    load_cell_offsets<B>(a, s_coff);
     constexpr int NL = 2 * NFB - 1;
     // lists and their solid counts: the classification pass's buckets 2-4,
     // then the merge pass's buckets 1-4
     constexpr int wls[NL] = {WL_FRONT + 1, WL_FRONT + 2, WL_FRONT + 3, WL_LATE, WL_LATE + 1, WL_LATE + 2,
                              WL_LATE + 3};
     constexpr int nss[NL] = {2, 3, 4, 1, 2, 3, 4};
-    const int g = gridDim.x;
     long long w[NL], wt = 0;
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
@@ -866,7 +862,7 @@ __global__ __launch_bounds__(128) void k_front_multi(StepArgs<Real> a)
         gb[big] += d;
         sum += d;
     }
-    int b = blockIdx.x, q = 0;
+    int b = blk, q = 0;
     while (q < NL - 1 && b >= gb[q]) b -= gb[q++];
     switch (q) {
     case 0: front_bucket<DIM, TC, B, 2, 1>(a, s_coff, wls[0], b, gb[0]); break;
@@ -879,27 +875,32 @@ __global__ __launch_bounds__(128) void k_front_multi(StepArgs<Real> a)
     }
 }
 
+template <int DIM, int TC, int B>
+__global__ __launch_bounds__(128) void k_front_multi(StepArgs<Real> a)
+{
+    if (a.halt_mode && a.stats->halt_step < a.step) return;
+    __shared__ uint32_t s_coff[B ? MAX_INST : 1];
+    load_cell_offsets<B>(a, s_coff);
+    front_multi<DIM, TC, B>(a, s_coff, blockIdx.x, gridDim.x);
+}
+
 // Pass 4: cells with more active grains than the merge pass holds.  Cells
 // with up to MID_A active grains take unrolled register arrays, MID_A lanes
 // per cell; beyond that one lane per cell keeps the arrays in scratch memory
 // with room for KMAX phases and the loops run over the occupied entries only.
 constexpr int MID_A = 8;
 template <int DIM, int TC, int B>
-__global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
+__device__ __forceinline__ void step_general(const StepArgs<Real> &a, const uint32_t *s_coff, int blk, int nblk)
 {
-    if (a.halt_mode && a.stats->halt_step < a.step) return;
-    __shared__ uint32_t s_coff[B ? MAX_INST : 1];
-    load_cell_offsets<B>(a, s_coff);
-    const Consts<Real> &K = a.K;
     const int n = *wl_counter(a, WL_GENERAL, a.step);
     const size_t ncell = a.in.stride; // slot stride
-    if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->slow_cells, (unsigned long long)n);
+    if (blk == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->slow_cells, (unsigned long long)n);
     // MID_A lanes per cell on the register path (one per solid); lane 0 of
     // the group writes, and alone takes the scratch path
     constexpr int G = MID_A;
     const bool lead = (threadIdx.x & (G - 1)) == 0;
     const int per = blockDim.x / G;
-    for (int i = blockIdx.x * per + (int)threadIdx.x / G; i < n; i += gridDim.x * per) {
+    for (int i = blk * per + (int)threadIdx.x / G; i < n; i += nblk * per) {
         const size_t c = (size_t)wl_list(a, WL_GENERAL)[i];
         const Where W = locate<B>(a, s_coff, c);
         size_t ti = a.temp_dense ? W.c : (size_t)W.k;
@@ -922,6 +923,35 @@ __global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
         cell_update<Real, DIM, KCAP, true, TC>(W.K, L, W.l, W.m, W.k, Tn, C);
         write_cell<Real, DIM, KCAP>(W.out, ncell, W.c, C);
     }
+}
+
+template <int DIM, int TC, int B>
+__global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
+{
+    if (a.halt_mode && a.stats->halt_step < a.step) return;
+    __shared__ uint32_t s_coff[B ? MAX_INST : 1];
+    load_cell_offsets<B>(a, s_coff);
+    step_general<DIM, TC, B>(a, s_coff, blockIdx.x, gridDim.x);
+}
+
+// The three interface and many-phase passes as one launch: blocks
+// [0, gg) take the many-phase cells, [gg, gg + gm) the multi-solid interface
+// lists, the rest the one-solid interface cells.  The long serial cells get
+// the first blocks, which start first; one launch needs no cross-stream
+// events, whose fork and join cost about 15 us each on this GPU.
+template <int DIM, int TC, int B>
+__global__ __launch_bounds__(128) void k_interface(StepArgs<Real> a, int gg, int gm)
+{
+    if (a.halt_mode && a.stats->halt_step < a.step) return;
+    __shared__ uint32_t s_coff[B ? MAX_INST : 1];
+    load_cell_offsets<B>(a, s_coff);
+    const int b = blockIdx.x;
+    if (b < gg)
+        step_general<DIM, TC, B>(a, s_coff, b, gg);
+    else if (b < gg + gm)
+        front_multi<DIM, TC, B>(a, s_coff, b - gg, gm);
+    else
+        front_bucket<DIM, TC, B, 1, 1>(a, s_coff, WL_FRONT, b - gg - gm, gridDim.x - gg - gm);
 }
 
 // Checked APT of the slot state in the reference's dense mf/nf layout
@@ -1443,7 +1473,9 @@ using namespace mpf::MPF_CAT(impl, MPF_SFX);
 
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
-// One step.  With side streams, after the merge pass the many-phase kernel
+// One step.  Fused (the default): after the merge pass one launch of
+// k_interface takes the many-phase cells and all interface cells.  With side
+// streams, after the merge pass the many-phase kernel
 // and the interface cells of several solids run concurrently with the
 // one-solid interface kernel (every one of these kernels only reads the input
 // state and writes cells no other writes); the solver stream waits for both
@@ -1484,6 +1516,14 @@ static void launch_step_kernels(const StepArgs<Real> &a, unsigned grid, const in
         if (tev) hipEventRecord(tev[4], st);
         hipStreamWaitEvent(st, S.ev_general, 0);
         hipStreamWaitEvent(st, S.ev_front, 0);
+    } else if (S.fused) {
+        // general blocks in units of 128 threads
+        const int gg = max(grids[2] / 2, 1);
+        if (tev) hipEventRecord(tev[3], st);
+        hipLaunchKernelGGL((k_interface<DIM, TC, B>), dim3(gg + gm + grids[1]), dim3(128), 0, st, a, gg, gm);
+        if (tev) hipEventRecord(tev[4], st);
+        if (tev) hipEventRecord(tev[5], st);
+        if (tev) hipEventRecord(tev[6], st);
     } else {
         if (tev) hipEventRecord(tev[3], st);
         hipLaunchKernelGGL((k_front1<DIM, TC, B>), dim3(grids[1]), dim3(128), 0, st, a);

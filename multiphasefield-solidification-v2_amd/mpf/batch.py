@@ -176,7 +176,7 @@ class Batch:
         check(_lib.lib().mpf_reset_stats(self._h))
 
     def set_launch(self, merge_blocks: int = 0, front_blocks: int = 0, general_blocks: int = 0,
-                   side_stream: bool = True):
+                   side_stream: int = 2):
         from .solver import set_launch
         set_launch(self._h, merge_blocks, front_blocks, general_blocks, side_stream)
 

@@ -155,7 +155,9 @@ class Solver:
         return stats_dict(s)
 
     def set_launch(self, merge_blocks: int = 0, front_blocks: int = 0, general_blocks: int = 0,
-                   side_stream: bool = True):
+                   side_stream: int = 2):
+        """Block counts (0: default) and the mode of the passes after the
+        merge pass: 0 in sequence, 1 on side streams, 2 one launch."""
         set_launch(self._h, merge_blocks, front_blocks, general_blocks, side_stream)
 
     def launch(self) -> dict:
@@ -204,21 +206,21 @@ class Solver:
 
 def set_launch(h, merge_blocks, front_blocks, general_blocks, side_stream):
     check(_lib.lib().mpf_set_launch(h, int(merge_blocks), int(front_blocks), int(general_blocks),
-                                    int(bool(side_stream))))
+                                    int(side_stream)))
 
 
 def get_launch(h) -> dict:
     out = np.zeros(4, np.int32)
     check(_lib.lib().mpf_get_launch(h, ptr(out, C.c_int32)))
     return {"merge_blocks": int(out[0]), "front_blocks": int(out[1]), "general_blocks": int(out[2]),
-            "side_stream": bool(out[3])}
+            "side_stream": int(out[3])}
 
 
 def autotune(h, trial_steps) -> dict:
     out = np.zeros(5, np.int32)
     check(_lib.lib().mpf_autotune(h, int(trial_steps), ptr(out, C.c_int32)))
     return {"merge_blocks": int(out[0]), "front_blocks": int(out[1]), "general_blocks": int(out[2]),
-            "side_stream": bool(out[3]), "us_per_step": int(out[4])}
+            "side_stream": int(out[3]), "us_per_step": int(out[4])}
 
 
 def stats_dict(s) -> dict:

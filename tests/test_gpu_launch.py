@@ -42,10 +42,12 @@ def _oracle(c, steps, cooling=0.0):
     return O.run(c["phi"], c["temp"], steps, c["tables"], c["scalars"], kmax=50, cooling_rate=cooling)
 
 
-@pytest.mark.parametrize("shape", [(1, 1, 1, False), (3, 5, 2, True), (7, 1, 1, False), (4096, 4096, 256, True)])
+@pytest.mark.parametrize("shape", [(1, 1, 1, 0), (3, 5, 2, 1), (7, 1, 1, 2), (4096, 4096, 256, 1),
+                                   (2, 8, 4, 2), (4096, 4096, 256, 2)])
 def test_launch_shapes_bit_exact(shape):
-    """Any block count (one block walking every list included) and either
-    placement of the many-phase kernel gives the same state."""
+    """Any block count (one block walking every list included) and every
+    mode of the passes after the merge pass (in sequence, side streams, one
+    launch) gives the same state."""
     c = _many_phase_case()
     steps = 5
     ref, _, _, _ = _oracle(c, steps)
