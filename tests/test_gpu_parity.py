@@ -324,3 +324,25 @@ def test_fp64_negative_upload_values_bit_exact():
     with _solver_for(c2, "fp64", temp_f32=True) as s:
         assert s.step(steps) == steps
         assert np.array_equal(s.phi(), ref)
+
+
+def test_fp64_large_grid_equals_tiled_small_grid():
+    """Size-independent check of the large-grid code paths.  A 512x512x128
+    grid (4096 classification tiles: 16-plane work items, interface passes
+    on side streams) made of 8 x 8 copies of a 64x64x128 pattern evolves,
+    with x and y periodic, exactly like the pattern alone (64 tiles: 4-plane
+    items, one interface launch).  FP64, bit for bit."""
+    c = cases.voronoi_case(nx=64, ny=64, nz=128, n_solid=4, seed=9, qseed=10, seed_height=40)
+    steps = 12
+    with _solver_for(c, "fp64", temp_f32=True) as s:
+        assert s.launch()["side_stream"] == 2
+        s.step(steps)
+        small = s.phi()
+    big = dict(c, phi=np.tile(c["phi"], (1, 8, 8, 1)), temp=np.tile(c["temp"], (8, 8, 1)))
+    with _solver_for(big, "fp64", temp_f32=True) as s:
+        assert s.launch()["side_stream"] == 1
+        s.step(steps)
+        st = s.stats()
+        got = s.phi()
+    assert st["front_cells"] > 0 and st["merge_cells"] > 0
+    assert np.array_equal(got, np.tile(small, (1, 8, 8, 1)))
