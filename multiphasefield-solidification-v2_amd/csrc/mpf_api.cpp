@@ -100,7 +100,7 @@ struct mpf_handle {
     hipEvent_t ev_merged = nullptr, ev_general = nullptr, ev_front = nullptr;
     int kernel_timing = 0;                 // per-kernel events (mpf_set_kernel_timing)
     std::vector<hipEvent_t> tev;           // 8 per step of the current call
-    double kern_ms[4] = {0, 0, 0, 0};      // k_light, k_merge, k_front, k_step_general
+    double kern_ms[4] = {0, 0, 0, 0};      // k_tiles + k_light, k_merge, k_front1, k_step_general
     int kern_steps = 0;
     Buffers b[2];
     double *T[2] = {nullptr, nullptr};

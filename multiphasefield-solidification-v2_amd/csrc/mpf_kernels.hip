@@ -7,15 +7,20 @@
 //
 // Kernels
 //   one time step (src/gpu_kernels_3d.py:462-486, :489-527, :530-680,
-//   :683-830, :833-838) is four kernels over the slot layout:
-//   k_light       every cell: temperature, classification, and the cells whose
-//                 stencil holds only liquid or only one grain
+//   :683-830, :833-838) runs over the slot layout:
+//   k_tiles       the work items of k_light: plane ranges of the tiles that
+//                 hold a plane that can change
+//   k_light       the listed planes: classification, temperature, the cells
+//                 whose stencil holds only liquid or only one grain, and the
+//                 routing of simple interface cells to their buckets
 //   k_merge       queued cells: APT merge; finishes cells without a
 //                 solid-liquid pair (or with anisotropy off)
-//   k_front       solid-liquid interface cells with the anisotropy and torque
-//                 terms, compacted so every lane does that work
+//   k_front1      interface cells with one solid, anisotropy and torque terms
+//   k_front_multi interface cells with 2-4 solids and those the merge pass
+//                 found, each bucket compiled for its solid count
 //   k_step_general cells with more active grains than the register path
-//                 holds (scratch arrays with room for KMAX phases)
+//                 holds (one lane per solid up to 8, scratch arrays beyond)
+//   k_interface   the last three as one launch
 //   k_apt_slots   checked APT of the slot state into the reference's dense
 //                 mf/nf layout (parity export)
 //   k_pack/k_unpack dense reference layout <-> slots
@@ -283,17 +288,20 @@ __global__ __launch_bounds__(256) void k_tiles(StepArgs<Real> a0)
         atomicMax(&a0.stats->max_active, 1);
 }
 
-// Pass 1 over the listed tiles: classification and the cells whose stencil
-// holds only liquid (no stored grain at any of the 7 points) or one grain
-// with zero liquid everywhere.  Everything else is queued for k_merge.  The
-// same launch advances the temperature (per plane or per cell).
+// Pass 1 over the listed work items: classification and the cells whose
+// stencil holds only liquid (no stored grain at any of the 7 points) or one
+// grain with zero liquid everywhere.  Interface cells whose stencil stores at
+// most one grain per point go to their interface bucket; everything else is
+// queued for k_merge.  The same launch advances the temperature (per plane
+// or per cell).
 //
-// A block takes an LTX x LTY column tile at a time and marches up its LZC
-// z-planes.  The cell's own column is carried in registers across planes
+// A block takes one work item (a range of planes of an LTX x LTY column tile)
+// at a time and marches up its z-planes, skipping the planes that cannot
+// change.  The cell's own column is carried in registers across planes
 // (z-1, z, z+1); the x/y neighbours of plane z come from an LDS tile with a
-// one-cell halo.  The queued cells of a tile are collected in LDS and
-// appended to the work list with a single atomic.  The blocks of one XCD
-// take a contiguous part of the tile list (listed in occupancy order, so
+// one-cell halo.  The queued cells of an item are collected in LDS and
+// appended to each list with a single atomic.  The blocks of one XCD take a
+// contiguous part of the item list (listed in occupancy order, so
 // neighbouring tiles mostly share an L2).
 //
 // The loads of the next plane are issued before the current plane is
@@ -708,8 +716,9 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
 }
 
 // Pass 2 over the queued cells: full APT merge.  Cells that need the
-// solid-liquid anisotropy terms go to k_front, cells with more active grains
-// than the register capacity to k_step_general; the rest are finished here.
+// solid-liquid anisotropy terms go to the interface lists of this pass
+// (WL_LATE, taken by k_front_multi), cells with more active grains than the
+// register capacity to k_step_general; the rest are finished here.
 template <int DIM, int TC, int B>
 __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
 {
