@@ -587,23 +587,44 @@ struct Cell {
     Real out[NA];      // new values after clip + renormalisation
 };
 
+// The first NS solids of a cell held with room for A >= NS (the same values,
+// for an update compiled for exactly NS solids).
+template <int NS, typename Real, int DIM, int A>
+MPF_DEV Cell<Real, DIM, NS> narrow_cell(const Cell<Real, DIM, A> &C)
+{
+    constexpr int NP = Cell<Real, DIM, A>::NP;
+    Cell<Real, DIM, NS> D;
+    D.liq = C.liq;
+    D.nsol = C.nsol;
+#pragma unroll
+    for (int t = 0; t <= NS; ++t) {
+        D.gid[t] = C.gid[t];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) D.v[t][p] = C.v[t][p];
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p) D.dom1[p] = C.dom1[p];
+    return D;
+}
+
 // The update of one cell: src/gpu_kernels_3d.py:550-680 / :699-830 (3D) and
 // src/gpu_kernels.py:386-515 (2D).  Tcell is the (already cooled) temperature.
 // Cells with many active phases (A large) keep the arrays in scratch memory.
 // ANISO_TERMS = false compiles the anisotropy terms out; the caller then
 // guarantees the cell has no active solid-liquid pair or anisotropy is off.
-// EXACT = true: the caller guarantees an active liquid and exactly A active
-// solids, so every presence test folds at compile time.
+// EXACT = true: the caller guarantees exactly A active solids and an active
+// liquid (XLIQ) or an inactive one (!XLIQ), so every presence test folds at
+// compile time.
 // G > 1 (register path only): G consecutive lanes of the wavefront (G a
 // power of two, G >= A) update the same cell together; lane r of the group
 // evaluates the solid-liquid terms of solid r + 1 and the group exchanges
 // them, which shortens the serial chain of a cell with several solids.
 template <typename Real, int DIM, int A, bool ANISO_TERMS = true, int TC = 0, bool EXACT = false, int G = 1,
-          typename LiqAt>
+          bool XLIQ = true, typename LiqAt>
 MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, int k,
                                             double Tcell, Cell<Real, DIM, A> &S)
 {
-    const bool s_liq = EXACT ? true : S.liq;
+    const bool s_liq = EXACT ? XLIQ : S.liq;
     const int s_nsol = EXACT ? A : S.nsol;
     constexpr int NA = A + 1;
     constexpr int NP = Cell<Real, DIM, A>::NP;

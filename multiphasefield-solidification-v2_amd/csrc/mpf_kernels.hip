@@ -756,8 +756,20 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
             if (fits && !front) {
                 size_t ti = a.temp_dense ? W.c : (size_t)W.k;
                 SlotLiq<Real> L{W.in.liq, W.K.nx, W.K.ny};
-                cell_update<Real, DIM, FAST_A, false, TC>(W.K, L, W.l, W.m, W.k, W.T[ti], C);
-                write_cell<Real, DIM, FAST_A>(W.out, ncell, W.c, C);
+                // grain-boundary cells (no active liquid, 2-4 solids) take an
+                // update compiled for their solid count
+                if (!C.liq && C.nsol == 2) {
+                    Cell<Real, DIM, 2> D = narrow_cell<2>(C);
+                    cell_update<Real, DIM, 2, false, TC, true, 1, false>(W.K, L, W.l, W.m, W.k, W.T[ti], D);
+                    write_cell<Real, DIM, 2>(W.out, ncell, W.c, D);
+                } else if (!C.liq && C.nsol == 3) {
+                    Cell<Real, DIM, 3> D = narrow_cell<3>(C);
+                    cell_update<Real, DIM, 3, false, TC, true, 1, false>(W.K, L, W.l, W.m, W.k, W.T[ti], D);
+                    write_cell<Real, DIM, 3>(W.out, ncell, W.c, D);
+                } else {
+                    cell_update<Real, DIM, FAST_A, false, TC>(W.K, L, W.l, W.m, W.k, W.T[ti], C);
+                    write_cell<Real, DIM, FAST_A>(W.out, ncell, W.c, C);
+                }
             }
         }
         // q = 0: the many-phase list; q = 1..NFB: interface cells with q solids
