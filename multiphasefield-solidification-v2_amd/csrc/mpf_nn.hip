@@ -9,7 +9,8 @@
 // K = taps x input channels.  Input and output channel counts are multiples
 // of 32.  Each axis is padded periodically or with zeros.
 //
-//   k_conv_fwd     y = act(conv(x, w) + b), 3x3x3 (27 taps) or 1x1x1 (1 tap)
+//   k_conv_fwd     y = act(conv(x, w) + b), 3x3x3 (27 taps) or 1x1x1 (1 tap),
+//                  wave tiles of 16-64 voxels x 32-64 channels
 //   k_conv_bwd_w   dw[tap][co][ci] += sum_v dy[v][co] x[v + tap][ci]
 //   (backward data is k_conv_fwd on the flipped, transposed weights)
 //   k_pool/k_up    2x average pooling, 2x nearest upsampling and their adjoints
@@ -76,50 +77,112 @@ __device__ __forceinline__ void decompose(const Geom &g, int v, int &b, int &x, 
 }
 
 // ------------------------------------------------------------ convolution
-// Block: 4 waves; block tile 128 voxels x 32 output channels; each wave
-// 32 voxels x 32 channels as 2 x 2 MFMA tiles of 16 x 16.  MFMA operand
-// layout (16x16x32): lane l supplies A row l%16 and B column l%16, eight K
-// values each from K group l/16; it receives D rows 4*(l/16)+i, column l%16.
+// Block: 4 waves; each wave computes (16 * MT) voxels x (16 * NT) output
+// channels as MT x NT MFMA tiles of 16 x 16, so a block covers 64 * MT
+// voxels.  MFMA operand layout (16x16x32): lane l supplies A row l%16 and B
+// column l%16, eight K values each from K group l/16; it receives D rows
+// 4*(l/16)+i, column l%16.  The K loop runs over (tap, 32-channel chunk) in
+// that order; with PIPE the operands of the next k-step are loaded before the
+// MFMAs of the current one (two register sets).  Every variant accumulates
+// in the same order, so all give the same bits.
+template <int MT, int NT, bool PIPE>
 __global__ __launch_bounds__(256) void k_conv_fwd(const bf16 *__restrict__ X, const bf16 *__restrict__ W,
                                                   const float *__restrict__ bias, bf16 *__restrict__ Y,
                                                   float *__restrict__ Yf, Geom g, int relu, int nvox)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int m0 = blockIdx.x * 128 + wave * 32;
-    const int n0 = blockIdx.y * 32;
+    const int m0 = (blockIdx.x * 4 + wave) * (16 * MT);
+    const int n0 = blockIdx.y * (16 * NT);
     const int row = lane & 15, kq = lane >> 4;
-    int vb[2], vx[2], vy[2], vz[2];
-    bool vok[2];
-    for (int t = 0; t < 2; ++t) {
+    int vb[MT], vx[MT], vy[MT], vz[MT];
+    bool vok[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
         const int v = m0 + 16 * t + row;
         vok[t] = v < nvox;
         decompose(g, vok[t] ? v : 0, vb[t], vx[t], vy[t], vz[t]);
     }
-    f32x4 acc[2][2];
-    for (int t = 0; t < 2; ++t)
-        for (int u = 0; u < 2; ++u) acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int u = 0; u < NT; ++u) acc[t][u] = (f32x4){0.f, 0.f, 0.f, 0.f};
     const bf16x8 zero8 = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
     const int kchunks = g.cin >> 5;
-    for (int tap = 0; tap < g.ntap; ++tap) {
+    const size_t tapw = (size_t)g.cout * g.cin;
+    const bf16 *wrow = W + (size_t)(n0 + row) * g.cin + kq * 8;
+    int nb[MT]; // neighbour voxel of each row tile at the current tap, -1 in zero padding
+    auto set_tap = [&](int tap) {
         int dx, dy, dz;
         tap_offsets(g.ntap, tap, dx, dy, dz);
-        int nb[2];
-        for (int t = 0; t < 2; ++t) nb[t] = vok[t] ? neighbour(g, vb[t], vx[t], vy[t], vz[t], dx, dy, dz) : -1;
-        const bf16 *wt = W + ((size_t)tap * g.cout + n0 + row) * g.cin + kq * 8;
-        for (int kc = 0; kc < kchunks; ++kc) {
-            bf16x8 a[2], b[2];
-            for (int t = 0; t < 2; ++t)
-                a[t] = nb[t] >= 0 ? *(const bf16x8 *)(X + (size_t)nb[t] * g.cin + kc * 32 + kq * 8) : zero8;
-            for (int u = 0; u < 2; ++u) b[u] = *(const bf16x8 *)(wt + (size_t)16 * u * g.cin + kc * 32);
-            for (int t = 0; t < 2; ++t)
-                for (int u = 0; u < 2; ++u)
-                    acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b[u], acc[t][u], 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < MT; ++t) nb[t] = vok[t] ? neighbour(g, vb[t], vx[t], vy[t], vz[t], dx, dy, dz) : -1;
+    };
+    auto load = [&](int tap, int kc, bf16x8 *a, bf16x8 *b) {
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+            a[t] = nb[t] >= 0 ? *(const bf16x8 *)(X + (size_t)nb[t] * g.cin + kc * 32 + kq * 8) : zero8;
+        const bf16 *wt = wrow + (size_t)tap * tapw + kc * 32;
+#pragma unroll
+        for (int u = 0; u < NT; ++u) b[u] = *(const bf16x8 *)(wt + (size_t)16 * u * g.cin);
+    };
+    auto mma = [&](const bf16x8 *a, const bf16x8 *b) {
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int u = 0; u < NT; ++u)
+                acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b[u], acc[t][u], 0, 0, 0);
+    };
+    bf16x8 a0[MT], b0[NT];
+    if constexpr (!PIPE) {
+        for (int tap = 0; tap < g.ntap; ++tap) {
+            set_tap(tap);
+            for (int kc = 0; kc < kchunks; ++kc) {
+                load(tap, kc, a0, b0);
+                mma(a0, b0);
+            }
+        }
+    } else {
+        // k-steps s (even) and s + 1 alternate between the register sets
+        bf16x8 a1[MT], b1[NT];
+        const int nsteps = g.ntap * kchunks;
+        int tap = 0, kc = 0;
+        set_tap(0);
+        load(0, 0, a0, b0);
+        for (int s = 0; s < nsteps; s += 2) {
+            int tap1 = tap, kc1 = kc + 1;
+            if (kc1 == kchunks) {
+                kc1 = 0;
+                ++tap1;
+            }
+            const bool has1 = s + 1 < nsteps;
+            if (has1) {
+                if (tap1 != tap) set_tap(tap1);
+                load(tap1, kc1, a1, b1);
+            }
+            mma(a0, b0);
+            if (!has1) break;
+            int tap2 = tap1, kc2 = kc1 + 1;
+            if (kc2 == kchunks) {
+                kc2 = 0;
+                ++tap2;
+            }
+            if (s + 2 < nsteps) {
+                if (tap2 != tap1) set_tap(tap2);
+                load(tap2, kc2, a0, b0);
+            }
+            mma(a1, b1);
+            tap = tap2;
+            kc = kc2;
         }
     }
-    for (int t = 0; t < 2; ++t)
-        for (int u = 0; u < 2; ++u) {
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int u = 0; u < NT; ++u) {
             const int c = n0 + 16 * u + row;
             const float bc = bias ? bias[c] : 0.f;
+#pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int v = m0 + 16 * t + 4 * kq + i;
                 if (v >= nvox) continue;
@@ -589,6 +652,27 @@ inline Geom geom(const int *dims, int cin, int cout, int ntap, const int *period
     return g;
 }
 
+// Wave tile of the forward convolution: the largest that still gives the
+// GPU enough blocks (256 CUs), from timings of the U-Net's layers on MI355X:
+// 4x4 tiles pay at 65536 voxels x 64 channels with 64+ input channels, 4x2
+// at 2048 blocks of 32 output channels, 2x2 down to 256 blocks, 1x2 below;
+// 1x1x1 convolutions (two or three k-steps) run unpipelined.
+struct ConvTile {
+    int mt, nt;
+    bool pipe;
+};
+
+inline ConvTile conv_tile(int nvox, int cin, int cout, int ntap)
+{
+    auto blocks = [&](int mt, int nt) { return (long long)nblk(nvox, 64 * mt) * (cout / (16 * nt)); };
+    if (ntap == 1) return {2, 2, false};
+    if (cout % 64 == 0 && cin >= 64 && blocks(4, 4) >= 256) return {4, 4, true};
+    if (cout % 64 == 0 && blocks(2, 4) >= 256) return {2, 4, true};
+    if (cin >= 64 && blocks(4, 2) >= 2048) return {4, 2, true};
+    if (blocks(2, 2) >= 256) return {2, 2, true};
+    return {1, 2, true};
+}
+
 inline int conv_args_ok(const int *dims, int cin, int cout, int ntap)
 {
     if (!dims || dims[0] < 1 || dims[1] < 1 || dims[2] < 1 || dims[3] < 1) return 0;
@@ -609,8 +693,24 @@ int mpf_nn_conv_fwd(const void *x, const void *w, const float *bias, void *y, fl
     if (!x || !w || (!y && !yf) || !periodic || !conv_args_ok(dims, cin, cout, ntap)) return -1;
     const Geom g = geom(dims, cin, cout, ntap, periodic);
     const int nvox = g.B * g.X * g.Y * g.Z;
-    hipLaunchKernelGGL(k_conv_fwd, dim3(nblk(nvox, 128), cout / 32), dim3(256), 0, st, (const bf16 *)x,
-                       (const bf16 *)w, bias, (bf16 *)y, yf, g, relu, nvox);
+    const ConvTile ct = conv_tile(nvox, cin, cout, ntap);
+    const dim3 grid(nblk(nvox, 64 * ct.mt), cout / (16 * ct.nt));
+#define MPF_CONV(M, N, P)                                                                                     \
+    hipLaunchKernelGGL((k_conv_fwd<M, N, P>), grid, dim3(256), 0, st, (const bf16 *)x, (const bf16 *)w, bias, \
+                       (bf16 *)y, yf, g, relu, nvox)
+    if (!ct.pipe)
+        MPF_CONV(2, 2, false);
+    else if (ct.mt == 1)
+        MPF_CONV(1, 2, true);
+    else if (ct.mt == 2 && ct.nt == 2)
+        MPF_CONV(2, 2, true);
+    else if (ct.mt == 2)
+        MPF_CONV(2, 4, true);
+    else if (ct.nt == 2)
+        MPF_CONV(4, 2, true);
+    else
+        MPF_CONV(4, 4, true);
+#undef MPF_CONV
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -39,6 +39,11 @@ def _host(t):
     ((1, 3, 9, 21), 32, 64, 27, (1, 1, 0)),
     ((2, 5, 3, 16), 128, 64, 27, (0, 1, 0)),
     ((3, 6, 5, 9), 64, 32, 1, (1, 1, 0)),
+    # sizes at which the forward kernel takes its larger wave tiles (4x4,
+    # 2x4 with 2x2 for the backward data, 4x2 with 2x4 for the backward data)
+    ((1, 64, 64, 16), 64, 64, 27, (1, 1, 0)),
+    ((1, 32, 32, 32), 32, 64, 27, (1, 1, 0)),
+    ((1, 128, 128, 32), 64, 32, 27, (1, 1, 0)),
 ])
 def test_conv_forward_and_backward(dims, cin, cout, ntap, periodic):
     torch = _torch()
