@@ -11,7 +11,8 @@
 //
 //   k_conv_fwd     y = act(conv(x, w) + b), 3x3x3 (27 taps) or 1x1x1 (1 tap),
 //                  wave tiles of 16-64 voxels x 32-64 channels
-//   k_conv_bwd_w   dw[tap][co][ci] += sum_v dy[v][co] x[v + tap][ci]
+//   k_conv_bwd_w   dw[tap][co][ci] += sum_v dy[v][co] x[v + tap][ci] (fp32 outer
+//                  products on the matrix cores)
 //   (backward data is k_conv_fwd on the flipped, transposed weights)
 //   k_pool/k_up    2x average pooling, 2x nearest upsampling and their adjoints
 //   slot kernels   per-cell active grains of a dense phi, their features,
@@ -24,6 +25,7 @@
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 namespace {
 
@@ -194,52 +196,103 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16 *__restrict__ X, co
         }
 }
 
-// dW[tap][co][ci] += sum over a chunk of voxels of dY[v][co] * X[v + tap][ci].
-// Block: 4 waves over one tap and a 32 x 32 (co, ci) tile, each wave a
-// 16 x 16 sub-tile; M = co, N = ci, K = voxels.
+// Coordinates (b, x, y, z) of a voxel advanced by a fixed voxel count whose
+// mixed-radix digits (z fastest) are s[0..3] = (z, y, x, b).
+__device__ __forceinline__ void advance(const Geom &g, const int *s, int &b, int &x, int &y, int &z)
+{
+    z += s[0];
+    int c = z >= g.Z;
+    z -= c ? g.Z : 0;
+    y += s[1] + c;
+    c = y >= g.Y;
+    y -= c ? g.Y : 0;
+    x += s[2] + c;
+    c = x >= g.X;
+    x -= c ? g.X : 0;
+    b += s[3] + c;
+}
+
+// dW[tap][co][ci] += sum over a chunk of voxels of dY[v][co] * X[v + tap][ci],
+// as sums of outer products on the fp32 matrix cores (v_mfma_f32_32x32x2f32,
+// K = 2 voxels): lane l supplies dY[v][co0 + l%32] and X[v + tap][ci0 + l%32]
+// of voxel v = pair + l/32, so both operands are contiguous 64-byte rows of
+// the channels-last tensors; lane l receives dW rows 8*(r/4) + 4*(l/32) + r%4
+// (r < 16) of column l%32.
+// Block: 4 waves over a 32 x 32 (co, ci) tile and NZ taps that differ only in
+// z (the three z offsets of one (x, y) offset, or the one 1x1x1 tap), which
+// share the dY operand and the coordinate arithmetic.  The waves take
+// interleaved voxel pairs of the block's chunk; their sums are added in LDS,
+// then one atomic per weight.
+template <int NZ>
 __global__ __launch_bounds__(256) void k_conv_bwd_w(const bf16 *__restrict__ X, const bf16 *__restrict__ dY,
                                                     float *__restrict__ dW, Geom g, int nvox, int vchunk)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int row = lane & 15, kq = lane >> 4;
-    const int tap = blockIdx.y;
+    const int ch = lane & 31, half = lane >> 5;
     const int nct = g.cout >> 5;
-    const int c0 = (blockIdx.z % nct) * 32 + (wave & 1) * 16;
-    const int i0 = (blockIdx.z / nct) * 32 + (wave >> 1) * 16;
-    int dx, dy, dz;
-    tap_offsets(g.ntap, tap, dx, dy, dz);
-    f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const int c0 = (blockIdx.z % nct) * 32, i0 = (blockIdx.z / nct) * 32;
+    const int t0 = blockIdx.y * NZ;
+    int dx, dy, dz0;
+    tap_offsets(g.ntap, t0, dx, dy, dz0);
+    f32x16 acc[NZ];
+#pragma unroll
+    for (int q = 0; q < NZ; ++q)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[q][r] = 0.f;
     const int vbeg = blockIdx.x * vchunk;
     const int vend = min(vbeg + vchunk, nvox);
-    for (int v0 = vbeg; v0 < vend; v0 += 32) {
-        bf16x8 a, b;
-        int v = v0 + 8 * kq;
-        int vb_, vx_, vy_, vz_;
-        decompose(g, v < nvox ? v : 0, vb_, vx_, vy_, vz_);
-        for (int e = 0; e < 8; ++e, ++v) {
-            float av = 0.f, bv = 0.f;
-            if (v < vend) {
-                av = (float)dY[(size_t)v * g.cout + c0 + row];
-                const int n = neighbour(g, vb_, vx_, vy_, vz_, dx, dy, dz);
-                bv = n >= 0 ? (float)X[(size_t)n * g.cin + i0 + row] : 0.f;
-            }
-            a[e] = (bf16)av;
-            b[e] = (bf16)bv;
-            if (++vz_ == g.Z) {
-                vz_ = 0;
-                if (++vy_ == g.Y) {
-                    vy_ = 0;
-                    if (++vx_ == g.X) {
-                        vx_ = 0;
-                        ++vb_;
-                    }
-                }
-            }
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    // this lane's voxels: vbeg + 2 * wave + half + 8 * j; UNR of them per
+    // iteration, their loads issued before the MFMAs
+    constexpr int UNR = 4;
+    int v = vbeg + 2 * wave + half;
+    int b, x, y, z;
+    decompose(g, v < nvox ? v : 0, b, x, y, z);
+    int step[4]; // digits of 8 voxels
+    {
+        int r = 8;
+        step[0] = r % g.Z;
+        r /= g.Z;
+        step[1] = r % g.Y;
+        r /= g.Y;
+        step[2] = r % g.X;
+        step[3] = r / g.X;
     }
-    for (int i = 0; i < 4; ++i)
-        atomicAdd(&dW[((size_t)tap * g.cout + c0 + 4 * kq + i) * g.cin + i0 + row], acc[i]);
+    for (int u = vbeg + 2 * wave; u < vend; u += 8 * UNR) {
+        float a[UNR], bv[UNR][NZ];
+#pragma unroll
+        for (int j = 0; j < UNR; ++j, v += 8) {
+            // element offsets fit in 32 bits (checked by conv_args_ok)
+            bool ok = v < vend;
+            a[j] = ok ? (float)dY[(uint32_t)(v * g.cout + c0 + ch)] : 0.f;
+            // the (x, y) part of the neighbours, shared by the z offsets
+            const int xx = wrap_axis(x + dx, g.X, g.px, ok);
+            const int yy = wrap_axis(y + dy, g.Y, g.py, ok);
+            const int row = ((b * g.X + xx) * g.Y + yy) * g.Z;
+#pragma unroll
+            for (int q = 0; q < NZ; ++q) {
+                bool okz = ok;
+                const int zz = wrap_axis(z + dz0 + q, g.Z, g.pz, okz);
+                bv[j][q] = okz ? (float)X[(uint32_t)((row + zz) * g.cin + i0 + ch)] : 0.f;
+            }
+            advance(g, step, b, x, y, z);
+        }
+#pragma unroll
+        for (int j = 0; j < UNR; ++j)
+#pragma unroll
+            for (int q = 0; q < NZ; ++q) acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[j], bv[j][q], acc[q], 0, 0, 0);
+    }
+    __shared__ float s_red[4][32 * 32];
+#pragma unroll
+    for (int q = 0; q < NZ; ++q) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) s_red[wave][(8 * (r / 4) + 4 * half + (r % 4)) * 32 + ch] = acc[q][r];
+        __syncthreads();
+        for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+            const float sum = s_red[0][e] + s_red[1][e] + s_red[2][e] + s_red[3][e];
+            atomicAdd(&dW[((size_t)(t0 + q) * g.cout + c0 + (e >> 5)) * g.cin + i0 + (e & 31)], sum);
+        }
+        __syncthreads();
+    }
 }
 
 // fp32 master weights [ntap][co][ci] -> bf16, as they are (flip 0) or flipped
@@ -721,9 +774,18 @@ int mpf_nn_conv_bwd_w(const void *x, const void *dy, float *dw, const int *dims,
     if (!x || !dy || !dw || !periodic || !conv_args_ok(dims, cin, cout, ntap)) return -1;
     const Geom g = geom(dims, cin, cout, ntap, periodic);
     const int nvox = g.B * g.X * g.Y * g.Z;
-    const int vchunk = 2048;
-    hipLaunchKernelGGL(k_conv_bwd_w, dim3(nblk(nvox, vchunk), ntap, (cout / 32) * (cin / 32)), dim3(256), 0, st,
-                       (const bf16 *)x, (const bf16 *)dy, dw, g, nvox, vchunk);
+    // voxel chunk per block: about 2048 blocks in all, chunks of 256 to 8192
+    const int ngrp = (ntap == 27 ? 9 : 1) * (cout / 32) * (cin / 32);
+    long long vc = (long long)nvox * ngrp / 2048;
+    vc = vc < 256 ? 256 : (vc > 8192 ? 8192 : vc);
+    const int vchunk = (int)(vc / 256 * 256);
+    const dim3 grid(nblk(nvox, vchunk), ntap == 27 ? 9 : 1, (cout / 32) * (cin / 32));
+    if (ntap == 27)
+        hipLaunchKernelGGL((k_conv_bwd_w<3>), grid, dim3(256), 0, st, (const bf16 *)x, (const bf16 *)dy, dw, g, nvox,
+                           vchunk);
+    else
+        hipLaunchKernelGGL((k_conv_bwd_w<1>), grid, dim3(256), 0, st, (const bf16 *)x, (const bf16 *)dy, dw, g, nvox,
+                           vchunk);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
