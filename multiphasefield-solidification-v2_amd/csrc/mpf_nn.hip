@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <initializer_list>
+
 #include "mpf_nn.h"
 
 typedef __bf16 bf16;
@@ -320,16 +322,16 @@ __global__ void k_relu_bwd(bf16 *dY, const bf16 *Y, size_t n)
     if (i < n && !((float)Y[i] > 0.f)) dY[i] = (bf16)0.f;
 }
 
-// db[c] += sum_v dY[v][c]; one block per 256-voxel chunk per 64 channels.
-__global__ void k_bias_grad(const bf16 *dY, float *db, int nvox, int C)
+// db[c] += sum_v dY[v][c]; one block per chunk of voxels per 64 channels.
+__global__ void k_bias_grad(const bf16 *dY, float *db, int nvox, int C, int chunk)
 {
     const int c = blockIdx.y * 64 + (threadIdx.x & 63);
     const int part = threadIdx.x >> 6;
     __shared__ float red[4][64];
     float s = 0.f;
     if (c < C) {
-        const int vbeg = blockIdx.x * 256;
-        for (int v = vbeg + part; v < min(vbeg + 256, nvox); v += 4) s += (float)dY[(size_t)v * C + c];
+        const int vbeg = blockIdx.x * chunk;
+        for (int v = vbeg + part; v < min(vbeg + chunk, nvox); v += 4) s += (float)dY[(size_t)v * C + c];
     }
     red[part][threadIdx.x & 63] = s;
     __syncthreads();
@@ -420,6 +422,121 @@ __global__ void k_copy_ch(const bf16 *src, int sC, int soff, bf16 *dst, int dC, 
     const float s = (float)src[v * sC + soff + j];
     bf16 *d = dst + v * dC + doff + j;
     *d = acc ? (bf16)((float)*d + s) : (bf16)s;
+}
+
+
+// ------------------------------------------------------------ channel kernels, 8 channels per thread
+// Vector forms of the channel kernels below for the common case (channel
+// counts and offsets multiples of 8, 16-byte aligned tensors, fewer than 2^31
+// vectors): one thread moves 8 bf16 channels (16 bytes) and index arithmetic
+// is 32-bit.  Per channel they do the same operations in the same order as
+// the scalar forms, so the results are identical.
+__device__ __forceinline__ bf16x8 ld8(const bf16 *p) { return *(const bf16x8 *)p; }
+__device__ __forceinline__ void st8(bf16 *p, bf16x8 v) { *(bf16x8 *)p = v; }
+
+__global__ void k_copy_ch8(const bf16 *src, int sC, int soff, bf16 *dst, int dC, int doff, int nch8, int n8,
+                           int acc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n8) return;
+    const int j = (i % nch8) * 8;
+    const unsigned v = (unsigned)(i / nch8);
+    const bf16x8 a = ld8(src + (size_t)v * sC + soff + j);
+    bf16 *d = dst + (size_t)v * dC + doff + j;
+    if (!acc) {
+        st8(d, a);
+        return;
+    }
+    bf16x8 o = ld8(d);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) o[e] = (bf16)((float)o[e] + (float)a[e]);
+    st8(d, o);
+}
+
+__global__ void k_slot_pool8(const bf16 *e, const int32_t *sid, bf16 *h, int nv, int K, int C)
+{
+    const int G = C / 8;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv * G) return;
+    const int v = i / G;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < K; ++k)
+        if (sid[(size_t)k * nv + v] >= 0) {
+            const bf16x8 a = ld8(e + ((size_t)k * nv * G + i) * 8);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) s[c] += (float)a[c];
+        }
+    bf16x8 o;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) o[c] = (bf16)s[c];
+    st8(h + (size_t)i * 8, o);
+}
+
+__global__ void k_slot_pool_bwd8(const bf16 *dh, const int32_t *sid, bf16 *de, int nv, int K, int C, int acc)
+{
+    const int G = C / 8;
+    const int nvg = nv * G;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nvg * K) return;
+    const int k = i / nvg, r = i % nvg;
+    const int v = r / G;
+    const bool on = sid[(size_t)k * nv + v] >= 0;
+    const bf16x8 d = ld8(dh + (size_t)r * 8);
+    bf16 *p = de + (size_t)i * 8;
+    bf16x8 o = acc ? ld8(p) : bf16x8{};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const float dv = on ? (float)d[c] : 0.f;
+        o[c] = acc ? (bf16)((float)o[c] + dv) : (bf16)dv;
+    }
+    st8(p, o);
+}
+
+__global__ void k_head_in8(const bf16 *F, int CF, const bf16 *e, int CE, bf16 *hin, int nv, int K)
+{
+    const int G = (CF + CE) / 8, GF = CF / 8;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= K * nv * G) return;
+    const int g = i % G;
+    const int sv = i / G;
+    const int v = sv % nv;
+    st8(hin + (size_t)i * 8, g < GF ? ld8(F + (size_t)v * CF + g * 8) : ld8(e + (size_t)sv * CE + (g - GF) * 8));
+}
+
+__global__ void k_head_in_bwd8(const bf16 *dhin, bf16 *dF, int CF, bf16 *de, int CE, int nv, int K)
+{
+    const int C = CF + CE, G = C / 8, GF = CF / 8;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv * G) return;
+    const int g = i % G;
+    const int v = i / G;
+    if (g < GF) {
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int k = 0; k < K; ++k) {
+            const bf16x8 a = ld8(dhin + ((size_t)k * nv + v) * C + g * 8);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) s[c] += (float)a[c];
+        }
+        bf16x8 o;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) o[c] = (bf16)s[c];
+        st8(dF + (size_t)v * CF + g * 8, o);
+    } else {
+        for (int k = 0; k < K; ++k)
+            st8(de + ((size_t)k * nv + v) * CE + (g - GF) * 8, ld8(dhin + ((size_t)k * nv + v) * C + g * 8));
+    }
+}
+
+__global__ void k_relu_bwd8(bf16 *dY, const bf16 *Y, int n8)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n8) return;
+    const bf16x8 y = ld8(Y + (size_t)i * 8);
+    bf16x8 d = ld8(dY + (size_t)i * 8);
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        if (!((float)y[c] > 0.f)) d[c] = (bf16)0.f;
+    st8(dY + (size_t)i * 8, d);
 }
 
 // Adam on fp32 parameters (bias-corrected step sizes passed in).
@@ -689,6 +806,17 @@ __global__ void k_fill_f32(float *x, float v, size_t n)
 
 inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
+// The 8-channel kernels apply: every pointer 16-byte aligned, every channel
+// count and offset a multiple of 8, and fewer than 2^31 vectors.
+inline bool vec8(std::initializer_list<const void *> ptrs, std::initializer_list<long long> counts, size_t nvec)
+{
+    for (const void *p : ptrs)
+        if ((uintptr_t)p & 15) return false;
+    for (long long c : counts)
+        if (c % 8) return false;
+    return nvec < (size_t)0x7fffffff;
+}
+
 inline Geom geom(const int *dims, int cin, int cout, int ntap, const int *periodic)
 {
     Geom g;
@@ -802,7 +930,11 @@ int mpf_nn_relu_bwd(void *dy, const void *y, size_t n, void *stream)
 {
     const hipStream_t st = (hipStream_t)stream;
     if (!dy || !y) return -1;
-    if (n) hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(n, 256)), dim3(256), 0, st, (bf16 *)dy, (const bf16 *)y, n);
+    if (n && vec8({dy, y}, {(long long)(n % 8)}, n / 8))
+        hipLaunchKernelGGL(k_relu_bwd8, dim3(nblk(n / 8, 256)), dim3(256), 0, st, (bf16 *)dy, (const bf16 *)y,
+                           (int)(n / 8));
+    else if (n)
+        hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(n, 256)), dim3(256), 0, st, (bf16 *)dy, (const bf16 *)y, n);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -810,8 +942,9 @@ int mpf_nn_bias_grad(const void *dy, float *db, int nvox, int c, void *stream)
 {
     const hipStream_t st = (hipStream_t)stream;
     if (!dy || !db || nvox < 1 || c < 1) return -1;
-    hipLaunchKernelGGL(k_bias_grad, dim3(nblk(nvox, 256), (c + 63) / 64), dim3(256), 0, st, (const bf16 *)dy, db,
-                       nvox, c);
+    const int chunk = 256;
+    hipLaunchKernelGGL(k_bias_grad, dim3(nblk(nvox, chunk), (c + 63) / 64), dim3(256), 0, st, (const bf16 *)dy, db,
+                       nvox, c, chunk);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -849,8 +982,12 @@ int mpf_nn_copy_channels(const void *src, int src_c, int src_off, void *dst, int
     const hipStream_t st = (hipStream_t)stream;
     if (!src || !dst || nch < 1 || nvox < 1 || src_off + nch > src_c || dst_off + nch > dst_c) return -1;
     const size_t n = (size_t)nvox * nch;
-    hipLaunchKernelGGL(k_copy_ch, dim3(nblk(n, 256)), dim3(256), 0, st, (const bf16 *)src, src_c, src_off,
-                       (bf16 *)dst, dst_c, dst_off, nch, nvox, accumulate);
+    if (vec8({src, dst}, {src_c, src_off, dst_c, dst_off, nch}, n / 8))
+        hipLaunchKernelGGL(k_copy_ch8, dim3(nblk(n / 8, 256)), dim3(256), 0, st, (const bf16 *)src, src_c, src_off,
+                           (bf16 *)dst, dst_c, dst_off, nch / 8, (int)(n / 8), accumulate);
+    else
+        hipLaunchKernelGGL(k_copy_ch, dim3(nblk(n, 256)), dim3(256), 0, st, (const bf16 *)src, src_c, src_off,
+                           (bf16 *)dst, dst_c, dst_off, nch, nvox, accumulate);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -883,9 +1020,16 @@ int mpf_nn_slot_pool(const void *e, const int32_t *sid, void *h, int nv, int k, 
 {
     const hipStream_t st = (hipStream_t)stream;
     if (!e || !sid || !h || nv < 1 || k < 1 || c < 1) return -1;
-    if (backward)   // e is dh, h is de
+    const bool v8 = vec8({e, h}, {c}, (size_t)nv * c * k / 8);
+    if (backward && v8)   // e is dh, h is de
+        hipLaunchKernelGGL(k_slot_pool_bwd8, dim3(nblk((size_t)nv * c * k / 8, 256)), dim3(256), 0, st,
+                           (const bf16 *)e, sid, (bf16 *)h, nv, k, c, accumulate);
+    else if (backward)
         hipLaunchKernelGGL(k_slot_pool_bwd, dim3(nblk((size_t)nv * c * k, 256)), dim3(256), 0, st,
                            (const bf16 *)e, sid, (bf16 *)h, nv, k, c, accumulate);
+    else if (v8)
+        hipLaunchKernelGGL(k_slot_pool8, dim3(nblk((size_t)nv * c / 8, 256)), dim3(256), 0, st, (const bf16 *)e,
+                           sid, (bf16 *)h, nv, k, c);
     else
         hipLaunchKernelGGL(k_slot_pool, dim3(nblk((size_t)nv * c, 256)), dim3(256), 0, st, (const bf16 *)e, sid,
                            (bf16 *)h, nv, k, c);
@@ -897,9 +1041,16 @@ int mpf_nn_head_in(const void *f, int cf, const void *e, int ce, void *hin, int 
 {
     const hipStream_t st = (hipStream_t)stream;
     if (!f || !e || !hin || nv < 1 || k < 1) return -1;
-    if (backward)   // hin is dhin, f is dF, e is de
+    const bool v8 = vec8({f, e, hin}, {cf, ce}, (size_t)k * nv * (cf + ce) / 8);
+    if (backward && v8)   // hin is dhin, f is dF, e is de
+        hipLaunchKernelGGL(k_head_in_bwd8, dim3(nblk((size_t)nv * (cf + ce) / 8, 256)), dim3(256), 0, st,
+                           (const bf16 *)hin, (bf16 *)f, cf, (bf16 *)e, ce, nv, k);
+    else if (backward)
         hipLaunchKernelGGL(k_head_in_bwd, dim3(nblk((size_t)nv * (cf + ce), 256)), dim3(256), 0, st,
                            (const bf16 *)hin, (bf16 *)f, cf, (bf16 *)e, ce, nv, k);
+    else if (v8)
+        hipLaunchKernelGGL(k_head_in8, dim3(nblk((size_t)k * nv * (cf + ce) / 8, 256)), dim3(256), 0, st,
+                           (const bf16 *)f, cf, (const bf16 *)e, ce, (bf16 *)hin, nv, k);
     else
         hipLaunchKernelGGL(k_head_in, dim3(nblk((size_t)k * nv * (cf + ce), 256)), dim3(256), 0, st,
                            (const bf16 *)f, cf, (const bf16 *)e, ce, (bf16 *)hin, nv, k);
