@@ -742,21 +742,14 @@ __global__ void k_slot_apply(float *phi, int N, int nx, int ny, int nz, int z0, 
     }
 }
 
-// max |a - b| over the band of two dense phi, as float bits (non-negative).
-__global__ void k_band_maxdiff(const float *a, const float *b, int N, int nx, int ny, int nz, int z0, int Zb,
-                               unsigned *out)
+// Band reductions over a dense phi[N][nx][ny][nz]: element i of the band
+// is column i / Zb (the flat (g, l, m) index) at height z0 + i % Zb, so
+// consecutive threads read consecutive heights.  Each thread keeps a running
+// maximum over a grid-stride loop (32-bit indices: the host checks the size);
+// one block reduction and one atomic per block.
+__device__ __forceinline__ void block_max_to(float d, unsigned *out)
 {
-    const size_t nv = (size_t)nx * ny * Zb;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ float red[256];
-    float d = 0.f;
-    if (i < nv * N) {
-        const int g = (int)(i / nv);
-        const int v = (int)(i % nv);
-        const int kz = v % Zb, m = (v / Zb) % ny, l = v / (Zb * ny);
-        const size_t c = (size_t)g * nx * ny * nz + ((size_t)l * ny + m) * nz + z0 + kz;
-        d = fabsf(a[c] - b[c]);
-    }
     red[threadIdx.x] = d;
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
@@ -766,29 +759,28 @@ __global__ void k_band_maxdiff(const float *a, const float *b, int N, int nx, in
     if (threadIdx.x == 0) atomicMax(out, __float_as_uint(red[0]));
 }
 
+// max |a - b| over the band of two dense phi, as float bits (non-negative).
+__global__ void k_band_maxdiff(const float *a, const float *b, unsigned n, int nz, int z0, int Zb, unsigned *out)
+{
+    float d = 0.f;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const size_t q = (size_t)(i / Zb) * nz + z0 + i % Zb;
+        d = fmaxf(d, fabsf(a[q] - b[q]));
+    }
+    block_max_to(d, out);
+}
+
 // max over the band of |(c - b) - (b - a) * inv_h|: the exact one-step change
 // c - b at a proposal b against the proposal's mean rate from a over h steps.
-__global__ void k_band_residual(const float *a, const float *b, const float *c, int N, int nx, int ny, int nz,
-                                int z0, int Zb, float inv_h, unsigned *out)
+__global__ void k_band_residual(const float *a, const float *b, const float *c, unsigned n, int nz, int z0, int Zb,
+                                float inv_h, unsigned *out)
 {
-    const size_t nv = (size_t)nx * ny * Zb;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    __shared__ float red[256];
     float d = 0.f;
-    if (i < nv * N) {
-        const int g = (int)(i / nv);
-        const int v = (int)(i % nv);
-        const int kz = v % Zb, m = (v / Zb) % ny, l = v / (Zb * ny);
-        const size_t q = (size_t)g * nx * ny * nz + ((size_t)l * ny + m) * nz + z0 + kz;
-        d = fabsf((c[q] - b[q]) - (b[q] - a[q]) * inv_h);
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const size_t q = (size_t)(i / Zb) * nz + z0 + i % Zb;
+        d = fmaxf(d, fabsf((c[q] - b[q]) - (b[q] - a[q]) * inv_h));
     }
-    red[threadIdx.x] = d;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + h]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicMax(out, __float_as_uint(red[0]));
+    block_max_to(d, out);
 }
 
 // float <-> bf16 conversions of plain arrays.
@@ -1095,7 +1087,9 @@ int mpf_nn_band_maxdiff(const float *a, const float *b, int n_phase, int nx, int
     const hipStream_t st = (hipStream_t)stream;
     if (!a || !b || !out || z0 < 0 || zb < 1 || z0 + zb > nz) return -1;
     const size_t n = (size_t)n_phase * nx * ny * zb;
-    hipLaunchKernelGGL(k_band_maxdiff, dim3(nblk(n, 256)), dim3(256), 0, st, a, b, n_phase, nx, ny, nz, z0, zb, out);
+    if (n >= 0xffffffffull - 256ull * 2048) return -1;
+    const unsigned nb = nblk(n, 256) < 2048 ? nblk(n, 256) : 2048;
+    hipLaunchKernelGGL(k_band_maxdiff, dim3(nb), dim3(256), 0, st, a, b, (unsigned)n, nz, z0, zb, out);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -1105,8 +1099,9 @@ int mpf_nn_band_residual(const float *a, const float *b, const float *c, int n_p
     const hipStream_t st = (hipStream_t)stream;
     if (!a || !b || !c || !out || z0 < 0 || zb < 1 || z0 + zb > nz) return -1;
     const size_t n = (size_t)n_phase * nx * ny * zb;
-    hipLaunchKernelGGL(k_band_residual, dim3(nblk(n, 256)), dim3(256), 0, st, a, b, c, n_phase, nx, ny, nz, z0, zb,
-                       inv_h, out);
+    if (n >= 0xffffffffull - 256ull * 2048) return -1;
+    const unsigned nb = nblk(n, 256) < 2048 ? nblk(n, 256) : 2048;
+    hipLaunchKernelGGL(k_band_residual, dim3(nb), dim3(256), 0, st, a, b, c, (unsigned)n, nz, z0, zb, inv_h, out);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -109,6 +109,74 @@ def test_pool_upsample_copy_kernels():
     hb = _host(big)
     np.testing.assert_array_equal(hb[:, 16:48], NO.bf16(2 * x.reshape(-1, C)))
     assert not hb[:, :16].any() and not hb[:, 48:].any()
+    # an offset that is not a multiple of 8 channels takes the one-channel kernel
+    odd = torch.zeros((nv, 40), dtype=torch.bfloat16, device="cuda")
+    _nnlib.call("mpf_nn_copy_channels", xd.data_ptr(), C, 0, odd.data_ptr(), 40, 4, C, nv, 0, _stream())
+    ho = _host(odd)
+    np.testing.assert_array_equal(ho[:, 4:36], x.reshape(-1, C))
+    assert not ho[:, :4].any() and not ho[:, 36:].any()
+
+
+@pytest.mark.parametrize("C", [32, 12])
+def test_slot_pool_and_head_input_kernels(C):
+    """Sum pooling over valid slots, the head's input and their adjoints
+    against numpy, for channel counts that take the 8-channel kernels (32)
+    and the one-channel kernels (12)."""
+    torch = _torch()
+    from mpf import _nnlib
+    from mpf.surrogate import _stream
+    rng = np.random.default_rng(11)
+    K, nv = 3, 200
+    sid = rng.integers(-1, 5, size=(K, nv)).astype(np.int32)
+    e = NO.bf16(rng.standard_normal((K, nv, C)))
+    ed, sd = _dev(e, torch.bfloat16), _dev(sid, torch.int32)
+    h = torch.empty((nv, C), dtype=torch.bfloat16, device="cuda")
+    _nnlib.call("mpf_nn_slot_pool", ed.data_ptr(), sd.data_ptr(), h.data_ptr(), nv, K, C, 0, 0, _stream())
+    ref = NO.bf16(((sid >= 0)[..., None] * e).sum(axis=0))
+    np.testing.assert_allclose(_host(h), ref, rtol=2 ** -7, atol=1e-6)
+    dh = NO.bf16(rng.standard_normal((nv, C)))
+    de = _dev(np.ones((K, nv, C)), torch.bfloat16)
+    _nnlib.call("mpf_nn_slot_pool", _dev(dh, torch.bfloat16).data_ptr(), sd.data_ptr(), de.data_ptr(), nv, K, C, 1,
+                1, _stream())
+    np.testing.assert_allclose(_host(de), NO.bf16(1.0 + (sid >= 0)[..., None] * dh[None]), rtol=2 ** -7, atol=1e-6)
+    F = NO.bf16(rng.standard_normal((nv, C)))
+    Fd = _dev(F, torch.bfloat16)
+    hin = torch.empty((K, nv, 2 * C), dtype=torch.bfloat16, device="cuda")
+    _nnlib.call("mpf_nn_head_in", Fd.data_ptr(), C, ed.data_ptr(), C, hin.data_ptr(), nv, K, 0, _stream())
+    np.testing.assert_array_equal(_host(hin), np.concatenate([np.broadcast_to(F, (K, nv, C)), e], axis=2))
+    dhin = NO.bf16(rng.standard_normal((K, nv, 2 * C)))
+    dF = torch.empty((nv, C), dtype=torch.bfloat16, device="cuda")
+    de2 = torch.empty((K, nv, C), dtype=torch.bfloat16, device="cuda")
+    _nnlib.call("mpf_nn_head_in", dF.data_ptr(), C, de2.data_ptr(), C, _dev(dhin, torch.bfloat16).data_ptr(), nv, K,
+                1, _stream())
+    np.testing.assert_allclose(_host(dF), NO.bf16(dhin[..., :C].sum(axis=0)), rtol=2 ** -7, atol=1e-6)
+    np.testing.assert_array_equal(_host(de2), dhin[..., C:])
+
+
+def test_band_reductions():
+    """Band maxima of the assisted stepper's acceptance test against numpy."""
+    torch = _torch()
+    from mpf import _nnlib
+    from mpf.surrogate import _p, _stream
+    rng = np.random.default_rng(12)
+    N, nx, ny, nz, z0, zb = 3, 8, 6, 20, 5, 9
+    a, b, c = (rng.standard_normal((N, nx, ny, nz)).astype(np.float32) for _ in range(3))
+    band = (slice(None), slice(None), slice(None), slice(z0, z0 + zb))
+    # outside the band the arrays differ far more, which must not count
+    b[..., :z0] += 100.0
+    ad, bd, cd = (_dev(t, torch.float32) for t in (a, b, c))
+
+    def reduce(name, *args):   # the maximum comes back as float bits in an int32
+        out = torch.zeros(1, dtype=torch.int32, device="cuda")
+        _nnlib.call(name, *args, _p(out), _stream())
+        return np.frombuffer(np.int32(out.item()).tobytes(), np.float32)[0]
+
+    got = reduce("mpf_nn_band_maxdiff", _p(ad), _p(bd), N, nx, ny, nz, z0, zb)
+    assert got == np.abs(a[band] - b[band]).max()
+    inv_h = np.float32(1.0 / 16)
+    got = reduce("mpf_nn_band_residual", _p(ad), _p(bd), _p(cd), N, nx, ny, nz, z0, zb, float(inv_h))
+    ref = np.abs((c[band] - b[band]) - (b[band] - a[band]) * inv_h).max()
+    assert abs(got - ref) <= 1e-6 * abs(ref)
 
 
 def test_slot_features_match_oracle():
