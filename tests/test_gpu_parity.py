@@ -15,7 +15,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-FP64_TOL = 1e-10   # the north-star bound; the checks below also demand bit-equality
+FP64_TOL = 1e-10   # the FP64 phi bound; the checks below also demand bit-equality
 FP32_TOL = 1e-4
 
 
