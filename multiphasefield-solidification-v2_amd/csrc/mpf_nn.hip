@@ -31,6 +31,42 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 namespace {
 
+__device__ __forceinline__ bf16x8 ld8(const bf16 *p) { return *(const bf16x8 *)p; }
+__device__ __forceinline__ void st8(bf16 *p, bf16x8 v) { *(bf16x8 *)p = v; }
+
+// V consecutive channels (1, or 8 in one 16-byte access) as floats.
+template <int V>
+struct Chans {
+    float v[V];
+};
+
+template <int V>
+__device__ __forceinline__ Chans<V> ld_ch(const bf16 *p)
+{
+    Chans<V> r;
+    if constexpr (V == 8) {
+        const bf16x8 a = ld8(p);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) r.v[e] = (float)a[e];
+    } else {
+        r.v[0] = (float)*p;
+    }
+    return r;
+}
+
+template <int V>
+__device__ __forceinline__ void st_ch(bf16 *p, const Chans<V> &x)
+{
+    if constexpr (V == 8) {
+        bf16x8 o;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) o[e] = (bf16)x.v[e];
+        st8(p, o);
+    } else {
+        *p = (bf16)x.v[0];
+    }
+}
+
 struct Geom {
     int B, X, Y, Z;    // voxel grid
     int cin, cout;     // channels
@@ -339,76 +375,98 @@ __global__ void k_bias_grad(const bf16 *dY, float *db, int nvox, int C, int chun
                                                   red[3][threadIdx.x]);
 }
 
-// 2x average pooling [B][X][Y][Z][C] -> [B][X/2][Y/2][Z/2][C] and its adjoint.
+// 2x average pooling [B][X][Y][Z][C] -> [B][X/2][Y/2][Z/2][C] and its adjoint,
+// 2x nearest upsampling [B][x][y][z][C] -> [B][2x][2y][2z][C] and its
+// adjoint.  A thread takes V channels of one voxel (V = 8: one 16-byte
+// access); I is the index type (32-bit when the host finds it wide enough).
+// Per channel the operations are the same for every V.
+template <int V, typename I>
 __global__ void k_pool_fwd(const bf16 *X, bf16 *Y, int B, int nx, int ny, int nz, int C)
 {
-    const int hx = nx / 2, hy = ny / 2, hz = nz / 2;
-    const size_t n = (size_t)B * hx * hy * hz * C;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int hx = nx / 2, hy = ny / 2, hz = nz / 2, CV = C / V;
+    const I n = (I)B * hx * hy * hz * CV;
+    const I i = (I)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int c = (int)(i % C);
-    size_t r = i / C;
+    const int c = (int)(i % CV) * V;
+    I r = i / CV;
     const int z = (int)(r % hz); r /= hz;
     const int y = (int)(r % hy); r /= hy;
     const int x = (int)(r % hx);
     const int b = (int)(r / hx);
-    float s = 0.f;
+    Chans<V> s;
+#pragma unroll
+    for (int e = 0; e < V; ++e) s.v[e] = 0.f;
     for (int a = 0; a < 8; ++a) {
         const int xx = 2 * x + (a >> 2), yy = 2 * y + ((a >> 1) & 1), zz = 2 * z + (a & 1);
-        s += (float)X[((((size_t)b * nx + xx) * ny + yy) * nz + zz) * C + c];
+        const Chans<V> t = ld_ch<V>(X + ((((size_t)b * nx + xx) * ny + yy) * nz + zz) * C + c);
+#pragma unroll
+        for (int e = 0; e < V; ++e) s.v[e] += t.v[e];
     }
-    Y[i] = (bf16)(s * 0.125f);
+#pragma unroll
+    for (int e = 0; e < V; ++e) s.v[e] *= 0.125f;
+    st_ch<V>(Y + (size_t)i * V, s);
 }
 
+template <int V, typename I>
 __global__ void k_pool_bwd(const bf16 *dY, bf16 *dX, int B, int nx, int ny, int nz, int C)
 {
-    const size_t n = (size_t)B * nx * ny * nz * C;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int CV = C / V;
+    const I n = (I)B * nx * ny * nz * CV;
+    const I i = (I)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int c = (int)(i % C);
-    size_t r = i / C;
+    const int c = (int)(i % CV) * V;
+    I r = i / CV;
     const int z = (int)(r % nz); r /= nz;
     const int y = (int)(r % ny); r /= ny;
     const int x = (int)(r % nx);
     const int b = (int)(r / nx);
     const int hx = nx / 2, hy = ny / 2, hz = nz / 2;
-    dX[i] = (bf16)(0.125f * (float)dY[((((size_t)b * hx + x / 2) * hy + y / 2) * hz + z / 2) * C + c]);
+    Chans<V> t = ld_ch<V>(dY + ((((size_t)b * hx + x / 2) * hy + y / 2) * hz + z / 2) * C + c);
+#pragma unroll
+    for (int e = 0; e < V; ++e) t.v[e] = 0.125f * t.v[e];
+    st_ch<V>(dX + (size_t)i * V, t);
 }
 
-// 2x nearest upsampling [B][x][y][z][C] -> [B][2x][2y][2z][C] and its adjoint.
+template <int V, typename I>
 __global__ void k_up_fwd(const bf16 *X, bf16 *Y, int B, int nx, int ny, int nz, int C)
 {
-    const int fx = 2 * nx, fy = 2 * ny, fz = 2 * nz;
-    const size_t n = (size_t)B * fx * fy * fz * C;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int fx = 2 * nx, fy = 2 * ny, fz = 2 * nz, CV = C / V;
+    const I n = (I)B * fx * fy * fz * CV;
+    const I i = (I)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int c = (int)(i % C);
-    size_t r = i / C;
+    const int c = (int)(i % CV) * V;
+    I r = i / CV;
     const int z = (int)(r % fz); r /= fz;
     const int y = (int)(r % fy); r /= fy;
     const int x = (int)(r % fx);
     const int b = (int)(r / fx);
-    Y[i] = X[((((size_t)b * nx + x / 2) * ny + y / 2) * nz + z / 2) * C + c];
+    st_ch<V>(Y + (size_t)i * V, ld_ch<V>(X + ((((size_t)b * nx + x / 2) * ny + y / 2) * nz + z / 2) * C + c));
 }
 
+template <int V, typename I>
 __global__ void k_up_bwd(const bf16 *dY, bf16 *dX, int B, int nx, int ny, int nz, int C)
 {
-    const size_t n = (size_t)B * nx * ny * nz * C;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int CV = C / V;
+    const I n = (I)B * nx * ny * nz * CV;
+    const I i = (I)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int c = (int)(i % C);
-    size_t r = i / C;
+    const int c = (int)(i % CV) * V;
+    I r = i / CV;
     const int z = (int)(r % nz); r /= nz;
     const int y = (int)(r % ny); r /= ny;
     const int x = (int)(r % nx);
     const int b = (int)(r / nx);
     const int fx = 2 * nx, fy = 2 * ny, fz = 2 * nz;
-    float s = 0.f;
+    Chans<V> s;
+#pragma unroll
+    for (int e = 0; e < V; ++e) s.v[e] = 0.f;
     for (int a = 0; a < 8; ++a) {
         const int xx = 2 * x + (a >> 2), yy = 2 * y + ((a >> 1) & 1), zz = 2 * z + (a & 1);
-        s += (float)dY[((((size_t)b * fx + xx) * fy + yy) * fz + zz) * C + c];
+        const Chans<V> t = ld_ch<V>(dY + ((((size_t)b * fx + xx) * fy + yy) * fz + zz) * C + c);
+#pragma unroll
+        for (int e = 0; e < V; ++e) s.v[e] += t.v[e];
     }
-    dX[i] = (bf16)s;
+    st_ch<V>(dX + (size_t)i * V, s);
 }
 
 // dst[v][doff + j] (=, or += with acc) src[v][soff + j] for j < nch.
@@ -431,8 +489,6 @@ __global__ void k_copy_ch(const bf16 *src, int sC, int soff, bf16 *dst, int dC, 
 // vectors): one thread moves 8 bf16 channels (16 bytes) and index arithmetic
 // is 32-bit.  Per channel they do the same operations in the same order as
 // the scalar forms, so the results are identical.
-__device__ __forceinline__ bf16x8 ld8(const bf16 *p) { return *(const bf16x8 *)p; }
-__device__ __forceinline__ void st8(bf16 *p, bf16x8 v) { *(bf16x8 *)p = v; }
 
 __global__ void k_copy_ch8(const bf16 *src, int sC, int soff, bf16 *dst, int dC, int doff, int nch8, int n8,
                            int acc)
@@ -945,12 +1001,18 @@ int mpf_nn_pool(const void *x, void *y, const int *dims, int c, int backward, vo
     const hipStream_t st = (hipStream_t)stream;
     if (!x || !y || !dims || c < 1 || (dims[1] & 1) || (dims[2] & 1) || (dims[3] & 1)) return -1;
     const size_t n = (size_t)dims[0] * dims[1] * dims[2] * dims[3] * c / (backward ? 1 : 8);
-    if (backward)
-        hipLaunchKernelGGL(k_pool_bwd, dim3(nblk(n, 256)), dim3(256), 0, st, (const bf16 *)x, (bf16 *)y, dims[0],
-                           dims[1], dims[2], dims[3], c);
+    const bool v8 = vec8({x, y}, {c}, n / 8);
+#define MPF_POOL(K, V, I)                                                                                       \
+    hipLaunchKernelGGL((K<V, I>), dim3(nblk(n / V, 256)), dim3(256), 0, st, (const bf16 *)x, (bf16 *)y, dims[0], \
+                       dims[1], dims[2], dims[3], c)
+    if (backward && v8)
+        MPF_POOL(k_pool_bwd, 8, uint32_t);
+    else if (backward)
+        MPF_POOL(k_pool_bwd, 1, size_t);
+    else if (v8)
+        MPF_POOL(k_pool_fwd, 8, uint32_t);
     else
-        hipLaunchKernelGGL(k_pool_fwd, dim3(nblk(n, 256)), dim3(256), 0, st, (const bf16 *)x, (bf16 *)y, dims[0],
-                           dims[1], dims[2], dims[3], c);
+        MPF_POOL(k_pool_fwd, 1, size_t);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -959,12 +1021,16 @@ int mpf_nn_upsample(const void *x, void *y, const int *dims, int c, int backward
     const hipStream_t st = (hipStream_t)stream;
     if (!x || !y || !dims || c < 1) return -1;
     const size_t n = (size_t)dims[0] * dims[1] * dims[2] * dims[3] * c * (backward ? 1 : 8);
-    if (backward)
-        hipLaunchKernelGGL(k_up_bwd, dim3(nblk(n, 256)), dim3(256), 0, st, (const bf16 *)x, (bf16 *)y, dims[0],
-                           dims[1], dims[2], dims[3], c);
+    const bool v8 = vec8({x, y}, {c}, n / 8);
+    if (backward && v8)
+        MPF_POOL(k_up_bwd, 8, uint32_t);
+    else if (backward)
+        MPF_POOL(k_up_bwd, 1, size_t);
+    else if (v8)
+        MPF_POOL(k_up_fwd, 8, uint32_t);
     else
-        hipLaunchKernelGGL(k_up_fwd, dim3(nblk(n, 256)), dim3(256), 0, st, (const bf16 *)x, (bf16 *)y, dims[0],
-                           dims[1], dims[2], dims[3], c);
+        MPF_POOL(k_up_fwd, 1, size_t);
+#undef MPF_POOL
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -78,12 +78,14 @@ def test_conv_forward_and_backward(dims, cin, cout, ntap, periodic):
     assert np.abs(dxh - dxr).max() <= np.abs(dxr).max() * 2 ** -7 + 1e-6
 
 
-def test_pool_upsample_copy_kernels():
+@pytest.mark.parametrize("C", [32, 12])
+def test_pool_upsample_copy_kernels(C):
+    """C = 32 takes the 8-channel kernels, C = 12 the one-channel kernels."""
     torch = _torch()
     from mpf import _nnlib
     from mpf.surrogate import _stream, _p
     rng = np.random.default_rng(8)
-    dims, C = (2, 4, 6, 8), 32
+    dims = (2, 4, 6, 8)
     x = NO.bf16(rng.standard_normal(dims + (C,)))
     xd = _dev(x.reshape(-1, C), torch.bfloat16)
     n_c = 2 * 2 * 3 * 4
@@ -107,14 +109,14 @@ def test_pool_upsample_copy_kernels():
     _nnlib.call("mpf_nn_copy_channels", xd.data_ptr(), C, 0, big.data_ptr(), 64, 16, C, nv, 0, _stream())
     _nnlib.call("mpf_nn_copy_channels", xd.data_ptr(), C, 0, big.data_ptr(), 64, 16, C, nv, 1, _stream())
     hb = _host(big)
-    np.testing.assert_array_equal(hb[:, 16:48], NO.bf16(2 * x.reshape(-1, C)))
-    assert not hb[:, :16].any() and not hb[:, 48:].any()
+    np.testing.assert_array_equal(hb[:, 16:16 + C], NO.bf16(2 * x.reshape(-1, C)))
+    assert not hb[:, :16].any() and not hb[:, 16 + C:].any()
     # an offset that is not a multiple of 8 channels takes the one-channel kernel
     odd = torch.zeros((nv, 40), dtype=torch.bfloat16, device="cuda")
     _nnlib.call("mpf_nn_copy_channels", xd.data_ptr(), C, 0, odd.data_ptr(), 40, 4, C, nv, 0, _stream())
     ho = _host(odd)
-    np.testing.assert_array_equal(ho[:, 4:36], x.reshape(-1, C))
-    assert not ho[:, :4].any() and not ho[:, 36:].any()
+    np.testing.assert_array_equal(ho[:, 4:4 + C], x.reshape(-1, C))
+    assert not ho[:, :4].any() and not ho[:, 4 + C:].any()
 
 
 @pytest.mark.parametrize("C", [32, 12])
