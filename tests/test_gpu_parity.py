@@ -270,10 +270,13 @@ def test_fp64_general_and_class_tables_bit_exact():
         assert np.array_equal(s.phi(), ref)
 
 
-@pytest.mark.parametrize("shape", [(10, 8, 14), (12, 17, 14), (68, 18, 20)])
+@pytest.mark.parametrize("shape", [(10, 8, 14), (12, 17, 14), (68, 18, 20),
+                                   (1, 9, 14), (9, 1, 14), (1, 1, 12), (10, 8, 3)])
 def test_fp64_classification_kernels_bit_exact(shape):
     """Grids whose classification tiles (64 x 8 cells) are partial in x and y,
-    including the 3D path with 32-bit offsets on odd sizes."""
+    including the 3D path with 32-bit offsets on odd sizes, and degenerate
+    grids: one cell across x, y or both (the periodic neighbour is the cell
+    itself) and three planes, all inside the seed layer."""
     nx, ny, nz = shape
     c = cases.voronoi_case(nx=nx, ny=ny, nz=nz, n_solid=5, seed=5, qseed=6, seed_height=5)
     steps = 6
