@@ -191,6 +191,36 @@ def surrogate_block():
     }
 
 
+def pcie_inclusive(steps=2000):
+    """The 128^3 16-grain case driven the way a caller holding the
+    reference's dense host arrays drives it: phi [17][128][128][128] float32
+    uploaded (host-to-device copy and conversion to the slot layout), `steps`
+    FP32 steps, phi downloaded again.  The cell-update rate with the
+    transfers included is reported beside the headline value, never as it."""
+    from mpf import surrogate_run as SR
+    make, _, _, _ = SR.c2_case()
+    n = 128
+    with make() as s:
+        s.step(16)
+        phi = s.phi(np.float32)
+        s.upload_phi(phi)
+        s.step(1)
+        t0 = time.perf_counter()
+        s.upload_phi(phi)
+        t1 = time.perf_counter()
+        s.step(steps)
+        t2 = time.perf_counter()
+        s.phi(np.float32)
+        t3 = time.perf_counter()
+    cells = float(n ** 3) * steps
+    return {"workload": f"128^3, 16 grains, FP32, {steps} steps between one dense upload and one dense download",
+            "dense_phi_bytes": int(phi.nbytes), "upload_s": round(t1 - t0, 4), "steps_s": round(t2 - t1, 4),
+            "download_s": round(t3 - t2, 4), "upload_GBps": round(phi.nbytes / (t1 - t0) / 1e9, 2),
+            "download_GBps": round(phi.nbytes / (t3 - t2) / 1e9, 2),
+            "mlups_steps_only": round(cells / (t2 - t1) / 1e6, 1),
+            "mlups_with_transfers": round(cells / (t3 - t0) / 1e6, 1)}
+
+
 def cpu_baseline(W, steps=6):
     """FP64 CPU oracle (reference dense layout, OpenMP over cells) on a
     256 x 256 x 48 slab holding the seed layer and the front."""
@@ -319,6 +349,8 @@ def main():
                 "grain_fraction_max_rel_diff": float(np.max(np.abs(f32 - f64) / np.maximum(f64, 1e-30))),
                 "solid_fraction_diff": abs(r32["ob1"]["solid_fraction"] - r64["ob1"]["solid_fraction"]),
             }
+    out["pcie_inclusive"] = pcie_inclusive()
+    log(f"pcie inclusive: {out['pcie_inclusive']}")
     if args.sweep_steps > 0:
         out["batched_sweep"] = batched_sweep(args.sweep_steps, bool(args.sweep_graphs))
     if args.surrogate:
