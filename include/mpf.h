@@ -5,7 +5,9 @@
  *
  * 1. The resident solver (handle API).  The library owns the device state in
  *    its own compact layout (dense liquid + per-cell grain slots) and advances
- *    it with one fused HIP kernel per step.  It replaces the reference's host
+ *    it with four to six HIP launches per step (a tile pass, a classification
+ *    pass that finishes most cells, a merge pass over queued cells, and the
+ *    interface passes).  It replaces the reference's host
  *    time loop with its three launches per step
  *      run_randommode_3d.py:267-287   (temp -> APT -> update -> swap)
  *      tests/verification_3d.py:830-905 (same, with the checked APT and a
@@ -32,7 +34,7 @@
 extern "C" {
 #endif
 
-#define MPF_ABI_VERSION 2
+#define MPF_ABI_VERSION 3
 
 enum { MPF_FP32 = 0, MPF_FP64 = 1 };
 enum {
@@ -93,6 +95,9 @@ typedef struct mpf_stats {
                                        1 around them and were left unchanged */
     int n111_pairs;                 /* 1: every grain's n111 rows come in opposite pairs (row 7-t = -row t,
                                        src/orientation_utils.py:182-212) and the facet search scans four */
+    unsigned long long gb_cells;    /* two-grain boundary cells (no liquid in the stencil) that the
+                                       classification pass finished without the merge pass, summed
+                                       over steps */
 } mpf_stats;
 
 /* ---- resident solver ---------------------------------------------------- */
@@ -222,6 +227,15 @@ int mpf_set_kernel_timing(mpf_handle *h, int on);
  * out[0..6] = max|sum phi - 1|, min phi, max phi, NaN count, solid sum, cells,
  * stored grain slots. */
 int mpf_observables(mpf_handle *h, double *grain_volume, int32_t *front, double *out);
+
+/* Difference of the current states of two handles on the same grid and phase
+ * count (any precisions), reduced on the device in FP64 with grain slots
+ * matched by id: out[0] = max over cells and phases of |phi_a - phi_b|,
+ * out[1] = their sum, out[2] = cells whose sets of non-zero grains differ,
+ * out[3] = cells with a difference above tol.  States the FP32 solver's error
+ * against the FP64 one (run_randommode_3d.py:267-300 advances both) without
+ * copying the dense phi to the host.  No counterpart in the reference. */
+int mpf_compare_phi(mpf_handle *a, mpf_handle *b, double tol, double *out);
 
 /* Bytes of device memory held by the handle. */
 size_t mpf_device_bytes(mpf_handle *h);

@@ -307,11 +307,14 @@ static int create_handle(const mpf_config &c, const std::vector<Inst> &grids, in
         delete h;
         return fail(MPF_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
+    // at least two slot arrays: the classification pass reads slot 1 of
+    // every cell it visits (and uses it only where the count is 2)
+    const size_t nslot = h->scap < 2 ? 2 : (size_t)h->scap;
     for (int i = 0; i < 2; ++i) {
         TRY_RC(dev_alloc(h, &h->b[i].liq, ncell * h->rsz));
         TRY_RC(dev_alloc(h, (void **)&h->b[i].cnt, ncell));
-        TRY_RC(dev_alloc(h, (void **)&h->b[i].sid, ncell * h->scap * sizeof(uint16_t)));
-        TRY_RC(dev_alloc(h, &h->b[i].sval, ncell * h->scap * h->rsz));
+        TRY_RC(dev_alloc(h, (void **)&h->b[i].sid, ncell * nslot * sizeof(uint16_t)));
+        TRY_RC(dev_alloc(h, &h->b[i].sval, ncell * nslot * h->rsz));
         TRY_RC(dev_alloc(h, (void **)&h->b[i].occ, h->nocc * sizeof(uint16_t)));
     }
     h->nT = c.temp_dense ? ncell : nplane;
@@ -904,6 +907,7 @@ extern "C" int mpf_get_stats(mpf_handle *h, mpf_stats *st)
     st->merge_cells = s.merge_cells;
     st->front_cells = s.front_cells;
     st->light_cells = s.light_cells;
+    st->gb_cells = s.gb_cells;
     st->n111_pairs = h->n111_pairs;
     st->class_tables = h->tab_class;
     for (int j = 0; j < 4; ++j) st->kernel_ms[j] = h->kern_ms[j];
@@ -1192,6 +1196,42 @@ extern "C" int mpf_observables(mpf_handle *h, double *grain_volume, int32_t *fro
     if (!h) return fail(MPF_EINVAL, "mpf_observables: null handle");
     if (h->batch) return fail(MPF_EINVAL, "mpf_observables: use mpf_batch_observables for a batch");
     return observables_inst(h, 0, grain_volume, front, out);
+}
+
+extern "C" hipError_t mpf_launch_compare(const void *const *pa, int fa, size_t sa, const void *const *pb, int fb,
+                                         size_t sb, size_t ncell, double tol, unsigned long long *out,
+                                         hipStream_t st);
+
+extern "C" int mpf_compare_phi(mpf_handle *a, mpf_handle *b, double tol, double *out)
+{
+    if (!a || !b || !out) return fail(MPF_EINVAL, "mpf_compare_phi: null argument");
+    if (a->batch || b->batch) return fail(MPF_EINVAL, "mpf_compare_phi: single grids only");
+    const mpf_config &ca = a->cfg, &cb = b->cfg;
+    if (ca.nx != cb.nx || ca.ny != cb.ny || ca.nz != cb.nz || ca.dim != cb.dim || ca.n_phase != cb.n_phase)
+        return fail(MPF_EINVAL, "mpf_compare_phi: the two handles hold different grids or phase counts");
+    const Buffers &A = a->b[a->cur], &B = b->b[b->cur];
+    const void *pa[4] = {A.liq, A.cnt, A.sid, A.sval};
+    const void *pb[4] = {B.liq, B.cnt, B.sid, B.sval};
+    unsigned long long *d = nullptr;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipStreamSynchronize(a->stream));
+    HIP_TRY(hipMalloc((void **)&d, 4 * sizeof(unsigned long long)));
+    unsigned long long r[4] = {0, 0, 0, 0};
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(r), a->stream);
+    if (e == hipSuccess)
+        e = mpf_launch_compare(pa, a->rsz == 8, a->ncell, pb, b->rsz == 8, b->ncell, a->ncell, tol, d, a->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(a->stream);
+    if (e == hipSuccess) e = hipMemcpy(r, d, sizeof(r), hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_compare_phi: %s", hipGetErrorString(e));
+    double mx, sm;
+    memcpy(&mx, &r[0], 8);
+    memcpy(&sm, &r[1], 8);
+    out[0] = mx;
+    out[1] = sm;
+    out[2] = (double)r[2];
+    out[3] = (double)r[3];
+    return 0;
 }
 
 extern "C" int mpf_batch_observables(mpf_handle *h, int ii, double *grain_volume, int32_t *front, double *out)

@@ -144,6 +144,53 @@ MPF_DEV bool apt_active(const Real *vals)
     return any;
 }
 
+// Two-grain boundary cell from the first two slots of its stencil points.
+// The cell qualifies when no stencil point holds liquid (so the liquid is
+// inactive), every point stores one or two grains with positive values, and
+// exactly two grains a < b occur over the stencil: a point storing two holds
+// (a, b), a point storing one holds a or b.  Both grains are then active (each
+// is positive at the cell or at a face neighbour) and nothing else is, so the
+// APT merge of the full slot lists (gather) would give the active set {a, b}
+// with these values.  cnt/id0/v0 describe slot 0 and the count, id1/v1 slot 1
+// (read only where the count is 2).
+template <typename Real, int DIM>
+MPF_DEV bool gb2_cell(const uint32_t *cnt, const uint32_t *id0, const Real *v0, const uint32_t *id1,
+                      const Real *v1, const Real *pl, Cell<Real, DIM, 2> &D)
+{
+    constexpr int NP = Cell<Real, DIM, 2>::NP;
+    bool ok = true;
+    uint32_t ga = NO_ID, gb = 0;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const bool two = cnt[p] == 2u;
+        ok = ok && pl[p] == Real(0.0) && (cnt[p] == 1u || two) && v0[p] > Real(0.0);
+        ga = id0[p] < ga ? id0[p] : ga;
+        const uint32_t hi = two ? id1[p] : id0[p];
+        gb = hi > gb ? hi : gb;
+    }
+    if (!ok || ga >= gb) return false;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const bool two = cnt[p] == 2u;
+        ok = ok && (two ? (id0[p] == ga && id1[p] == gb && v1[p] > Real(0.0)) : (id0[p] == ga || id0[p] == gb));
+    }
+    if (!ok) return false;
+    D.liq = false;
+    D.nsol = 2;
+    D.gid[0] = 0;
+    D.gid[1] = (int)ga;
+    D.gid[2] = (int)gb;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const bool two = cnt[p] == 2u;
+        D.v[0][p] = pl[p];
+        D.v[1][p] = id0[p] == ga ? v0[p] : Real(0.0);
+        D.v[2][p] = two ? v1[p] : (id0[p] == gb ? v0[p] : Real(0.0));
+        D.dom1[p] = Real(0.0);
+    }
+    return true;
+}
+
 // Gather the active set of one cell (APT with the checked KMAX clamp).
 // Returns the total number of active phases (before the clamp) in *ntot and
 // whether the kept solids fit in A (*fits).

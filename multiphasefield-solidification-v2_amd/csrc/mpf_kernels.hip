@@ -37,6 +37,13 @@
 #ifndef MPF_REAL
 #error "MPF_REAL must be float or double"
 #endif
+// MPF_PART selects the part of the file one compilation emits, so that the
+// parts build in parallel: 0 the 3D step, 1 the batched 3D step, 2 the 2D
+// step, 3 every other kernel and the step dispatch.  Unset: everything.
+#ifndef MPF_PART
+#define MPF_PART -1
+#endif
+#define MPF_HAS(p) (MPF_PART < 0 || MPF_PART == (p))
 #define MPF_CAT2(a, b) a##b
 #define MPF_CAT(a, b) MPF_CAT2(a, b)
 #define MPF_NAME(n) MPF_CAT(n, MPF_SFX)
@@ -88,6 +95,27 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
 {
     if ((nb & 7) != 0) return b;
     return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+// Chunks [c0, c1) with stride dc of a list of nch chunks for block blk of
+// nblk: with a multiple of 8 blocks (blocks are dealt round-robin over the 8
+// XCDs, so blk % 8 is the block's XCD when blk is its grid index) the blocks
+// of XCD x take the x-th eighth of the list in turn, so that neighbouring
+// list entries -- neighbouring cells, since the lists are filled tile by
+// tile -- are processed together on one XCD and share its L2.
+struct ChunkRange {
+    int c0, c1, dc;
+};
+__device__ __forceinline__ ChunkRange xcd_chunks(int nch, int blk, int nblk)
+{
+    ChunkRange r{blk, nch, nblk};
+    if ((nblk & 7) == 0) {
+        const int x = blk & 7;
+        r.c0 = (int)((long long)nch * x / 8) + (blk >> 3);
+        r.c1 = (int)((long long)nch * (x + 1) / 8);
+        r.dc = nblk >> 3;
+    }
+    return r;
 }
 
 __device__ __forceinline__ int *wl_counter(const StepArgs<Real> &a, int list, int step)
@@ -740,8 +768,9 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
     __shared__ int s_base[2][NQ];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int par = 0;
-    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x, par ^= 1) {
-        int i = base + threadIdx.x;
+    const ChunkRange cr = xcd_chunks((n + (int)blockDim.x - 1) / (int)blockDim.x, blockIdx.x, gridDim.x);
+    for (int ch = cr.c0; ch < cr.c1; ch += cr.dc, par ^= 1) {
+        int i = ch * (int)blockDim.x + threadIdx.x;
         bool valid = i < n;
         const size_t c = valid ? (size_t)wl_list(a, WL_MERGE)[i] : 0;
         const Where W = locate<B>(a, s_coff, c);
@@ -810,7 +839,8 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
 // all interface cells); k_front_multi, concurrently, the classification
 // pass's buckets of 2-4 solids and every bucket of the merge pass.
 static_assert(NFB == FAST_A, "interface buckets cover the register path's solid counts");
-template <int DIM, int TC, int B, int NS, int G>
+// XCD: blk is the block's grid index (chunks by XCD, xcd_chunks)
+template <int DIM, int TC, int B, int NS, int G, bool XCD = false>
 __device__ __forceinline__ void front_bucket(const StepArgs<Real> &a, const uint32_t *s_coff, int wl, int blk,
                                              int nblk)
 {
@@ -821,7 +851,10 @@ __device__ __forceinline__ void front_bucket(const StepArgs<Real> &a, const uint
     const bool lead = (threadIdx.x & (G - 1)) == 0;
     if (blk == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->front_cells, (unsigned long long)n);
     const int per = blockDim.x / G;
-    for (int i = blk * per + (int)threadIdx.x / G; i < n; i += nblk * per) {
+    const ChunkRange cr = XCD ? xcd_chunks((n + per - 1) / per, blk, nblk) : ChunkRange{blk, (n + per - 1) / per, nblk};
+    for (int ch = cr.c0; ch < cr.c1; ch += cr.dc) {
+        const int i = ch * per + (int)threadIdx.x / G;
+        if (i >= n) continue;
         const size_t c = (size_t)list[i];
         const Where W = locate<B>(a, s_coff, c);
         Cell<Real, DIM, NS> C;
@@ -842,7 +875,7 @@ __global__ __launch_bounds__(128) void k_front1(StepArgs<Real> a)
     if (a.halt_mode && a.stats->halt_step < a.step) return;
     __shared__ uint32_t s_coff[B ? MAX_INST : 1];
     load_cell_offsets<B>(a, s_coff);
-    front_bucket<DIM, TC, B, 1, 1>(a, s_coff, WL_FRONT, blockIdx.x, gridDim.x);
+    front_bucket<DIM, TC, B, 1, 1, true>(a, s_coff, WL_FRONT, blockIdx.x, gridDim.x);
 }
 
 // The grid (at least 8 blocks) is split over the seven lists in proportion
@@ -977,6 +1010,7 @@ __global__ __launch_bounds__(128) void k_interface(StepArgs<Real> a, int gg, int
 
 // Checked APT of the slot state in the reference's dense mf/nf layout
 // (kernel_update_nfmf_3d_checked, src/gpu_kernels_3d.py:489-527).
+#if MPF_HAS(3)
 template <int DIM>
 __global__ void k_apt_slots(Consts<Real> K, SlotView<Real> S, int32_t *mf, int32_t *nf, int32_t *status,
                             int depth)
@@ -1485,6 +1519,8 @@ __global__ void k_phase_map(SlotView<Real> S, int nx, int ny, int nz, int axis, 
     out[p] = g;
 }
 
+#endif // MPF_HAS(3)
+
 } // namespace impl_*
 } // namespace mpf
 
@@ -1558,6 +1594,51 @@ static void launch_step_kernels(const StepArgs<Real> &a, unsigned grid, const in
 
 extern "C" {
 
+// The step of one kind of grid: 3D, a batch of 3D grids, 2D.
+hipError_t MPF_NAME(mpf_step_3d)(const StepArgs<Real> &a, unsigned grid, const int *grids, hipStream_t st,
+                                 const StepStreams *ss, hipEvent_t *tev);
+hipError_t MPF_NAME(mpf_step_batch)(const StepArgs<Real> &a, unsigned grid, const int *grids, hipStream_t st,
+                                    const StepStreams *ss, hipEvent_t *tev);
+hipError_t MPF_NAME(mpf_step_2d)(const StepArgs<Real> &a, unsigned grid, const int *grids, hipStream_t st,
+                                 const StepStreams *ss, hipEvent_t *tev);
+
+#if MPF_HAS(0)
+hipError_t MPF_NAME(mpf_step_3d)(const StepArgs<Real> &a, unsigned grid, const int *grids, hipStream_t st,
+                                 const StepStreams *ss, hipEvent_t *tev)
+{
+    if (a.tab_class)
+        launch_step_kernels<3, 1, 0>(a, grid, grids, st, *ss, tev);
+    else
+        launch_step_kernels<3, 0, 0>(a, grid, grids, st, *ss, tev);
+    return hipGetLastError();
+}
+#endif
+
+#if MPF_HAS(1)
+hipError_t MPF_NAME(mpf_step_batch)(const StepArgs<Real> &a, unsigned grid, const int *grids, hipStream_t st,
+                                    const StepStreams *ss, hipEvent_t *tev)
+{
+    if (a.tab_class)
+        launch_step_kernels<3, 1, 1>(a, grid, grids, st, *ss, tev);
+    else
+        launch_step_kernels<3, 0, 1>(a, grid, grids, st, *ss, tev);
+    return hipGetLastError();
+}
+#endif
+
+#if MPF_HAS(2)
+hipError_t MPF_NAME(mpf_step_2d)(const StepArgs<Real> &a, unsigned grid, const int *grids, hipStream_t st,
+                                 const StepStreams *ss, hipEvent_t *tev)
+{
+    if (a.tab_class)
+        launch_step_kernels<2, 1, 0>(a, grid, grids, st, *ss, tev);
+    else
+        launch_step_kernels<2, 0, 0>(a, grid, grids, st, *ss, tev);
+    return hipGetLastError();
+}
+#endif
+
+#if MPF_HAS(3)
 hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const int *grids, hipStream_t st,
                                      const StepStreams *ss, hipEvent_t *tev)
 {
@@ -1565,28 +1646,12 @@ hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const 
     // persistent classification grid: four blocks per CU of the 256, at most
     // one per work item
     const unsigned grid = (unsigned)min(a.n_tiles * (LZC / a.light_q), 1024);
-    if (a.n_inst > 0) {
-        // a batch of 3D grids: one launch per kernel covers every instance
-        if (a.tab_class)
-            launch_step_kernels<3, 1, 1>(a, grid, grids, st, *ss, tev);
-        else
-            launch_step_kernels<3, 0, 1>(a, grid, grids, st, *ss, tev);
-        return hipGetLastError();
-    }
+    if (a.n_inst > 0) // a batch of 3D grids: one launch per kernel covers every instance
+        return MPF_NAME(mpf_step_batch)(a, grid, grids, st, ss, tev);
     a.ntx = (a.K.nx + LTX - 1) / LTX;
     a.nty = (a.K.ny + LTY - 1) / LTY;
-    if (dim == 3) {
-        if (a.tab_class)
-            launch_step_kernels<3, 1, 0>(a, grid, grids, st, *ss, tev);
-        else
-            launch_step_kernels<3, 0, 0>(a, grid, grids, st, *ss, tev);
-    } else {
-        if (a.tab_class)
-            launch_step_kernels<2, 1, 0>(a, grid, grids, st, *ss, tev);
-        else
-            launch_step_kernels<2, 0, 0>(a, grid, grids, st, *ss, tev);
-    }
-    return hipGetLastError();
+    if (dim == 3) return MPF_NAME(mpf_step_3d)(a, grid, grids, st, ss, tev);
+    return MPF_NAME(mpf_step_2d)(a, grid, grids, st, ss, tev);
 }
 
 hipError_t MPF_NAME(mpf_launch_apt_slots)(const Consts<Real> *K, SlotView<Real> S, int dim, int32_t *mf,
@@ -1711,5 +1776,7 @@ hipError_t MPF_NAME(mpf_launch_observe)(SlotView<Real> S, int nx, int ny, int nz
     hipLaunchKernelGGL(k_front_height, dim3(nblk((size_t)nx * ny, 256)), dim3(256), 0, st, S.liq, nx, ny, nz, front);
     return hipGetLastError();
 }
+
+#endif // MPF_HAS(3)
 
 } // extern "C"

@@ -58,14 +58,20 @@ def build(verbose: bool = False) -> str:
     # and division instead of the correctly rounded sequences.
     variants = (("double", "_f64", ["-ffp-contract=off"]),
                 ("float", "_f32", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]))
+    # each precision in four parts (3D step, batched 3D step, 2D step, the
+    # rest; MPF_PART in mpf_kernels.hip) so that the compilations run in parallel
     for real, sfx, flags in variants:
-        o = os.path.join(BUILD, f"mpf_kernels{sfx}.o")
-        steps.append(([cc, *_common(), f"-DMPF_REAL={real}", f"-DMPF_SFX={sfx}", *flags, "-c", kern, "-o", o], o))
+        for part in range(4):
+            o = os.path.join(BUILD, f"mpf_kernels{sfx}_p{part}.o")
+            steps.append(([cc, *_common(), f"-DMPF_REAL={real}", f"-DMPF_SFX={sfx}", f"-DMPF_PART={part}", *flags,
+                           "-c", kern, "-o", o], o))
+    cmpo = os.path.join(BUILD, "mpf_compare.o")
+    steps.append(([cc, *_common(), "-ffp-contract=off", "-c", os.path.join(CSRC, "mpf_compare.hip"), "-o", cmpo], cmpo))
     nn = os.path.join(BUILD, "mpf_nn.o")
     steps.append(([cc, *_common(), "-ffp-contract=fast", "-c", os.path.join(CSRC, "mpf_nn.hip"), "-o", nn], nn))
     api = os.path.join(BUILD, "mpf_api.o")
     steps.append(([cc, *_common(), "-x", "hip", "-c", os.path.join(CSRC, "mpf_api.cpp"), "-o", api], api))
-    with ThreadPoolExecutor(max_workers=len(steps)) as ex:
+    with ThreadPoolExecutor(max_workers=min(len(steps), os.cpu_count() or 8)) as ex:
         for f in [ex.submit(_run, cmd, verbose) for cmd, _ in steps]:
             f.result()
     objs = [o for _, o in steps]

@@ -23,6 +23,7 @@ EXPORTS = (
     "mpf_create", "mpf_destroy", "mpf_set_tables", "mpf_upload_phi", "mpf_init_columns",
     "mpf_upload_temp", "mpf_download_temp", "mpf_step", "mpf_download_phi", "mpf_download_apt",
     "mpf_get_stats", "mpf_reset_stats", "mpf_set_kernel_timing", "mpf_observables", "mpf_device_bytes",
+    "mpf_compare_phi",
     "mpf_dense_apt", "mpf_dense_update", "mpf_dense_temp",
     "mpf_malloc", "mpf_free", "mpf_memcpy_htod", "mpf_memcpy_dtoh", "mpf_memcpy_dtod", "mpf_memset",
     "mpf_device_synchronize", "mpf_last_error", "mpf_abi_version",
@@ -67,7 +68,7 @@ class Stats(C.Structure):
         ("halt_step", C.c_int), ("steps_done", C.c_int), ("last_step_ms", C.c_double),
         ("merge_cells", C.c_ulonglong), ("front_cells", C.c_ulonglong),
         ("class_tables", C.c_int), ("kernel_steps", C.c_int), ("kernel_ms", C.c_double * 4),
-        ("light_cells", C.c_ulonglong), ("n111_pairs", C.c_int),
+        ("light_cells", C.c_ulonglong), ("n111_pairs", C.c_int), ("gb_cells", C.c_ulonglong),
     ]
 
 
@@ -117,6 +118,7 @@ def lib():
         "mpf_reset_stats": (C.c_int, [H]),
         "mpf_set_kernel_timing": (C.c_int, [H, C.c_int]),
         "mpf_observables": (C.c_int, [H, dp, ip, dp]),
+        "mpf_compare_phi": (C.c_int, [H, H, C.c_double, dp]),
         "mpf_device_bytes": (C.c_size_t, [H]),
         "mpf_dense_apt": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_int, C.c_int, C.c_int, vp]),

@@ -87,10 +87,40 @@ def _grid(phi: DeviceArray):
     raise ValueError("phi must be [N, nx, ny, nz] or [N, nx, ny]")
 
 
+def _check_shapes(name, phi, n_phase, mf, nf, status=None, phi_new=None, temp=None, tables=None):
+    """The entry points see only pointers and sizes, so every array the
+    kernels index is checked against the grid and phase count here, before a
+    launch: a mismatch would otherwise read or write out of bounds."""
+    n_phase = int(n_phase)
+    grid = phi.shape[1:]
+    if n_phase < 1 or phi.shape[0] < n_phase:
+        raise ValueError(f"{name}: phi holds {phi.shape[0]} phases, number_of_grain is {n_phase}")
+    if phi_new is not None and phi_new.shape != phi.shape:
+        raise ValueError(f"{name}: phi_new {phi_new.shape} does not match phi {phi.shape}")
+    if mf.ndim != phi.ndim or tuple(mf.shape[1:]) != grid or mf.shape[0] < 1 or mf.dtype != np.int32:
+        raise ValueError(f"{name}: mf must be int32 [depth, {', '.join(map(str, grid))}], got {mf.dtype} {mf.shape}")
+    if tuple(nf.shape) != grid or nf.dtype != np.int32:
+        raise ValueError(f"{name}: nf must be int32 {grid}, got {nf.dtype} {nf.shape}")
+    if status is not None and (status.size < 2 or status.dtype != np.int32):
+        raise ValueError(f"{name}: status must be int32 with 2 entries, got {status.dtype} {status.shape}")
+    if temp is not None and temp.size != int(np.prod(grid)):
+        raise ValueError(f"{name}: temp {temp.shape} does not match the grid {grid}")
+    if tables is not None:
+        wij, aij, mij, n111 = tables
+        for tn, t in (("wij", wij), ("aij", aij), ("mij", mij)):
+            if tuple(t.shape) != (n_phase, n_phase):
+                raise ValueError(f"{name}: {tn} must be [{n_phase}, {n_phase}], got {t.shape}")
+        if tuple(n111.shape) != (n_phase, 8, 3):
+            raise ValueError(f"{name}: n111 must be [{n_phase}, 8, 3], got {n111.shape}")
+
+
 def apt(phi: DeviceArray, mf: DeviceArray, nf: DeviceArray, status: DeviceArray | None, n_phase: int,
         checked: bool = True, kmax: int = 50):
     """kernel_update_nfmf_3d(_checked) / kernel_update_nfmf."""
     dim, nx, ny, nz = _grid(phi)
+    _check_shapes("apt", phi, n_phase, mf, nf, status=status if checked else None)
+    if checked and status is None:
+        raise ValueError("apt: the checked APT needs a status array")
     check(_lib.lib().mpf_dense_apt(phi.ptr, _dt(phi), mf.ptr, nf.ptr, status.ptr if status else None,
                                    int(n_phase), nx, ny, nz, dim, mf.shape[0], int(kmax),
                                    int(bool(checked)), None))
@@ -104,6 +134,7 @@ def update(phi: DeviceArray, phi_new: DeviceArray, temp: DeviceArray, mf: Device
     for t in (wij, aij, mij, n111, phi_new):
         if t.dtype != phi.dtype:
             raise TypeError("phi, phi_new and the tables must share one dtype")
+    _check_shapes("update", phi, n_phase, mf, nf, phi_new=phi_new, temp=temp, tables=(wij, aij, mij, n111))
     s = _lib.scalars_struct(scalars)
     check(_lib.lib().mpf_dense_update(phi.ptr, phi_new.ptr, _dt(phi), temp.ptr, _dt(temp), mf.ptr, nf.ptr,
                                       wij.ptr, aij.ptr, mij.ptr, n111.ptr, C.byref(s), int(n_phase),

@@ -154,6 +154,16 @@ class Solver:
         check(_lib.lib().mpf_get_stats(self._h, C.byref(s)))
         return stats_dict(s)
 
+    def compare(self, other: "Solver", tol: float = 1e-4) -> dict:
+        """Difference between this solver's phi and another's on the same grid
+        (any precisions), reduced on the device in FP64: max and sum of
+        |phi_a - phi_b| over cells and phases, cells whose sets of non-zero
+        grains differ, cells above tol."""
+        out = np.zeros(4, np.float64)
+        check(_lib.lib().mpf_compare_phi(self._h, other._h, float(tol), ptr(out)))
+        return {"phi_max_abs": float(out[0]), "phi_sum_abs": float(out[1]),
+                "grain_set_diff_cells": int(out[2]), "cells_above_tol": int(out[3]), "tol": float(tol)}
+
     def set_launch(self, merge_blocks: int = 0, front_blocks: int = 0, general_blocks: int = 0,
                    side_stream: int = 2):
         """Block counts (0: default) and the mode of the passes after the
@@ -230,6 +240,7 @@ def stats_dict(s) -> dict:
             "last_step_ms": s.last_step_ms, "merge_cells": s.merge_cells,
             "front_cells": s.front_cells, "class_tables": bool(s.class_tables),
             "kernel_steps": s.kernel_steps, "light_cells": s.light_cells, "n111_pairs": bool(s.n111_pairs),
+            "gb_cells": s.gb_cells,
             "kernel_ms": dict(zip(("light", "merge", "front", "general"), list(s.kernel_ms)))}
 
 

@@ -12,10 +12,14 @@
  * Layout: phi[g][l][m][k] (3D) / phi[g][l][m] (2D), C order, last index
  * fastest; mf[t][l][m][k], nf[l][m][k]; temp dense [l][m][k].
  *
- * FP64 contract ("pure FP64"): every value is an IEEE double and every
- * operation is evaluated in the order the reference writes it; the reference's
- * float32 casts (facet/A12/A13 returns, lap_sl/b_sl local arrays, phi_new
- * stores) are not applied.  With use_libm=1 the transcendentals are the C
+ * FP64 contract ("pure FP64", faithful = 0): every value is an IEEE double
+ * and every operation is evaluated in the order the reference writes it; the
+ * reference's float32 casts (facet/A12/A13 returns, lap_sl/b_sl local
+ * arrays, phi_new stores) are not applied.  With faithful = 1 (3D update) the
+ * values are rounded to float32 at exactly those points, as the reference's
+ * float32 arrays and float32-typed returns do (src/gpu_kernels_3d.py:257-261,
+ * 296-300, 311, 337-340, 383, 553-554, 661, 671, 678); the arithmetic in
+ * between stays FP64, as Numba types it.  With use_libm=1 the transcendentals are the C
  * library's (what Python's math module calls); with use_libm=0 they are the
  * shared routines of include/mpf_math.h, which the GPU FP64 solver also uses.
  * Build with -ffp-contract=off (oracle/Makefile).
@@ -32,7 +36,11 @@ typedef struct {
     double dx, dt, T_melt, Sf, eps0_sl, w0_sl;
     double a0, delta_a, mu_a, p_round, g2_floor, ksi, theta_c_rad;
     int use_libm;
+    int faithful; /* 1: float32 rounding at the reference's truncation points (3D) */
 } orc_params;
+
+/* Round to float32 in faithful mode. */
+static double f32p(const orc_params *p, double v) { return p->faithful ? (double)(float)v : v; }
 
 typedef struct {
     const orc_params *p;
@@ -189,6 +197,7 @@ static void facet3(const ctx_t *c, double phix, double phiy, double phiz, int s,
     }
     double inv_g = 1.0 / sqrt(q);
     double gx = phix * inv_g, gy = phiy * inv_g, gz = phiz * inv_g;
+    const orc_params *p = c->p;
     double best_abs = 0.0, best_signed = 0.0, bnx = 1.0, bny = 0.0, bnz = 0.0;
     for (int t = 0; t < 8; ++t) {
         double ux = N111(c, s, t, 0), uy = N111(c, s, t, 1), uz = N111(c, s, t, 2);
@@ -201,7 +210,8 @@ static void facet3(const ctx_t *c, double phix, double phiy, double phiz, int s,
     if (best_abs > 1.0) best_abs = 1.0;
     if (best_signed > 1.0) best_signed = 1.0;
     if (best_signed < -1.0) best_signed = -1.0;
-    o[0] = best_signed; o[1] = best_abs; o[2] = bnx; o[3] = bny; o[4] = bnz;
+    o[0] = f32p(p, best_signed); o[1] = f32p(p, best_abs);
+    o[2] = f32p(p, bnx); o[3] = f32p(p, bny); o[4] = f32p(p, bnz);
 }
 
 /* da_dphixyz_A12, src/gpu_kernels_3d.py:304-341.  out: a, da/dphix, da/dphiy, da/dphiz */
@@ -228,10 +238,10 @@ static void A12_3(const ctx_t *c, int l, int m, int k, int s, double *o)
     double C = sqrt(cs * cs + p->p_round * p->p_round);
     double S = sqrt(mpf_pymax(1.0 - cs * cs, 0.0) + p->p_round * p->p_round);
     double coef = (cs / C) - o_tan(p, p->a0) * (cs / S);
-    o[0] = a_val;
-    o[1] = p->mu_a * p->delta_a * coef * dcx;
-    o[2] = p->mu_a * p->delta_a * coef * dcy;
-    o[3] = p->mu_a * p->delta_a * coef * dcz;
+    o[0] = f32p(p, a_val);
+    o[1] = f32p(p, p->mu_a * p->delta_a * coef * dcx);
+    o[2] = f32p(p, p->mu_a * p->delta_a * coef * dcy);
+    o[3] = f32p(p, p->mu_a * p->delta_a * coef * dcz);
 }
 
 /* A13_divergence_3d, src/gpu_kernels_3d.py:344-383 */
@@ -243,11 +253,11 @@ static void A13_3(const ctx_t *c, int l, int m, int k, int s, double *o)
     int kp = idx_zp(k, p->nz), km = idx_zm(k, p->nz);
     double a[4], b[4];
     A12_3(c, lp, m, k, s, a); A12_3(c, lm, m, k, s, b);
-    o[0] = (a[1] - b[1]) * (0.5 / p->dx);
+    o[0] = f32p(p, (a[1] - b[1]) * (0.5 / p->dx));
     A12_3(c, l, mp, k, s, a); A12_3(c, l, mm, k, s, b);
-    o[1] = (a[2] - b[2]) * (0.5 / p->dx);
+    o[1] = f32p(p, (a[2] - b[2]) * (0.5 / p->dx));
     A12_3(c, l, m, kp, s, a); A12_3(c, l, m, km, s, b);
-    o[2] = (a[3] - b[3]) * (0.5 / p->dx);
+    o[2] = f32p(p, (a[3] - b[3]) * (0.5 / p->dx));
 }
 
 /* torque_A11_3d, src/gpu_kernels_3d.py:386-441 */
@@ -366,10 +376,10 @@ static void cached_terms3(const ctx_t *c, int l, int m, int k, int gid, int anis
         double lap_val = aniso_term1_3(c, l, m, k, gid);
         if (torque) lap_val += torque3(c, l, m, k, gid);
         double bc = best_cos3(c, gxl, gyl, gzl, gid);
-        *b_sl = calc_b_from_cos(p, bc);
-        *lap_sl = lap_val;
+        *b_sl = f32p(p, calc_b_from_cos(p, bc));
+        *lap_sl = f32p(p, lap_val);
     } else {
-        *lap_sl = (p->eps0_sl * p->eps0_sl) * lap_liq;
+        *lap_sl = f32p(p, (p->eps0_sl * p->eps0_sl) * lap_liq);
         *b_sl = 1.0;
     }
 }
@@ -479,7 +489,7 @@ void orc_update3(const double *phi, double *phi_new, const double *temp,
                             mij_eff = mij[(size_t)i * N + j] * b_sl[t2];
                         dpi -= 2.0 * mij_eff / (double)n_act * (ppp - term_force);
                     }
-                    phi_new[(size_t)i * ncell + cell] = PHI3(c, i, l, m, k) + p->dt * dpi;
+                    phi_new[(size_t)i * ncell + cell] = f32p(p, PHI3(c, i, l, m, k) + p->dt * dpi);
                 }
 
                 double s = 0.0;
@@ -493,7 +503,10 @@ void orc_update3(const double *phi, double *phi_new, const double *temp,
                 }
                 if (s > 1e-20) {
                     double invs = 1.0 / s;
-                    for (int t = 0; t < n_act; ++t) phi_new[(size_t)ids[t] * ncell + cell] *= invs;
+                    for (int t = 0; t < n_act; ++t) {
+                        size_t o = (size_t)ids[t] * ncell + cell;
+                        phi_new[o] = f32p(p, phi_new[o] * invs);
+                    }
                 } else {
                     phi_new[cell] = 1.0;
                 }

@@ -27,7 +27,7 @@ class OrcParams(C.Structure):
         ("a0", C.c_double), ("delta_a", C.c_double), ("mu_a", C.c_double),
         ("p_round", C.c_double), ("g2_floor", C.c_double), ("ksi", C.c_double),
         ("theta_c_rad", C.c_double),
-        ("use_libm", C.c_int),
+        ("use_libm", C.c_int), ("faithful", C.c_int),
     ]
 
 
@@ -72,7 +72,7 @@ PARAM_KEYS = ("dx", "dt", "T_melt", "Sf", "eps0_sl", "w0_sl", "a0", "delta_a", "
               "p_round", "g2_floor", "ksi", "theta_c_rad")
 
 
-def make_params(shape, N: int, scalars: dict, use_libm: bool = False) -> OrcParams:
+def make_params(shape, N: int, scalars: dict, use_libm: bool = False, faithful: bool = False) -> OrcParams:
     p = OrcParams()
     p.nx = int(shape[0])
     p.ny = int(shape[1])
@@ -81,6 +81,7 @@ def make_params(shape, N: int, scalars: dict, use_libm: bool = False) -> OrcPara
     for k in PARAM_KEYS:
         setattr(p, k, float(scalars[k]))
     p.use_libm = 1 if use_libm else 0
+    p.faithful = 1 if faithful else 0
     return p
 
 
@@ -124,13 +125,20 @@ def update(phi, temp, mf, nf, tables, scalars, aniso=True, torque=True, use_libm
 
 
 def run(phi, temp, nsteps, tables, scalars, kmax=50, aniso=True, torque=True,
-        cooling_rate=0.0, temp_f32=False, halt_on_kmax=False, use_libm=False):
-    """Verification-style loop on the CPU.  Returns (phi, temp, steps_done, status)."""
+        cooling_rate=0.0, temp_f32=False, halt_on_kmax=False, use_libm=False, faithful=False):
+    """Verification-style loop on the CPU.  Returns (phi, temp, steps_done, status).
+    faithful (3D): float32 rounding at the reference's truncation points,
+    including the float32 phi stores; the initial phi is rounded to float32
+    as the reference's to_device of a float32 array would hold it."""
     phi = _f64(phi).copy()
     N = phi.shape[0]
     grid = phi.shape[1:]
+    if faithful:
+        if len(grid) != 3:
+            raise ValueError("the faithful mode covers the 3D update")
+        phi = phi.astype(np.float32).astype(np.float64)
     temp = _f64(np.broadcast_to(temp, grid)).copy()
-    p = make_params(grid, N, scalars, use_libm)
+    p = make_params(grid, N, scalars, use_libm, faithful)
     wij, aij, mij, n111 = (_f64(t) for t in tables)
     st = np.zeros(2, np.int32)
     done = lib().orc_run(len(grid), phi, temp, int(nsteps), int(kmax), wij, aij, mij, n111,

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_errors_without_gpu():
     L = _lib.lib()
-    assert L.mpf_abi_version() == 2
+    assert L.mpf_abi_version() == 3
     # argument validation happens before any device call
     import ctypes as C
     cfg = _lib.Config()

@@ -42,3 +42,47 @@ def test_host_arrays_and_bad_launches_are_rejected():
         _check_cover("k", (2, 2), (4, 4, 4), (5, 4, 4))
     _check_cover("k", (2, 1, 1), (4, 4, 4), (5, 4, 4))
     assert blocks_for((65, 64, 1), (8, 8, 4)) == (9, 8, 1)
+
+
+def _fake(shape, dtype):
+    """A DeviceArray header without device memory (the checks below must
+    reject the call before any pointer is used)."""
+    from mpf.dense import DeviceArray
+    a = object.__new__(DeviceArray)
+    a.shape = tuple(shape)
+    a.dtype = np.dtype(dtype)
+    a.size = int(np.prod(a.shape))
+    a.nbytes = a.size * a.dtype.itemsize
+    a._ptr = None
+    return a
+
+
+def test_dense_dropins_reject_mismatched_shapes_before_launch():
+    from mpf import dense
+    N, g = 5, (6, 4, 3)
+    phi, new = _fake((N,) + g, np.float64), _fake((N,) + g, np.float64)
+    mf, nf, st = _fake((50,) + g, np.int32), _fake(g, np.int32), _fake((2,), np.int32)
+    T = _fake(g, np.float64)
+    tabs = [_fake((N, N), np.float64) for _ in range(3)] + [_fake((N, 8, 3), np.float64)]
+    sc = {k: 1.0 for k in ("dx", "dt", "T_melt", "Sf", "eps0_sl", "w0_sl", "a0", "delta_a", "mu_a",
+                           "p_round", "g2_floor", "ksi", "theta_c_rad")}
+    bad_apt = [
+        dict(phi=phi, mf=mf, nf=nf, status=st, n_phase=N + 1),                       # more phases than phi
+        dict(phi=phi, mf=_fake((50, 6, 4, 2), np.int32), nf=nf, status=st, n_phase=N),  # mf grid
+        dict(phi=phi, mf=_fake((50,) + g, np.int64), nf=nf, status=st, n_phase=N),       # mf dtype
+        dict(phi=phi, mf=mf, nf=_fake((6, 4), np.int32), status=st, n_phase=N),          # nf grid
+        dict(phi=phi, mf=mf, nf=nf, status=_fake((1,), np.int32), n_phase=N),            # status size
+        dict(phi=phi, mf=mf, nf=nf, status=None, n_phase=N),                             # checked, no status
+    ]
+    for kw in bad_apt:
+        with pytest.raises(ValueError):
+            dense.apt(**kw)
+    good = dict(phi=phi, phi_new=new, temp=T, mf=mf, nf=nf, wij=tabs[0], aij=tabs[1], mij=tabs[2],
+                n111=tabs[3], scalars=sc, n_phase=N)
+    for key, val in (("phi_new", _fake((N, 6, 4, 2), np.float64)), ("temp", _fake((6, 4), np.float64)),
+                     ("wij", _fake((N + 1, N + 1), np.float64)), ("mij", _fake((N, N - 1), np.float64)),
+                     ("n111", _fake((N, 4, 3), np.float64)), ("nf", _fake((6, 4, 2), np.int32))):
+        kw = dict(good)
+        kw[key] = val
+        with pytest.raises(ValueError):
+            dense.update(**kw)

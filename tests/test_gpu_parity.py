@@ -90,6 +90,24 @@ def test_fp64_2d_disk_bit_exact():
         _assert_apt_equal(s, ref)
 
 
+def test_fp64_2d_disk_linear_cooling_bit_exact():
+    """The 2D single-grain case with linear cooling (T -= rate every step),
+    200 steps, FP64 bit-exact."""
+    c = cases.disk2d_case()
+    steps, cool = 200, 5.0e-4
+    ref, T_ref, done, st = O.run(c["phi"], c["temp"], steps, c["tables"], c["scalars"], kmax=50,
+                                 cooling_rate=cool)
+    with _solver_for(c, "fp64", temp_dense=True, cooling_rate=cool) as s:
+        assert s.step(steps) == steps
+        got = s.phi()
+        assert np.array_equal(got, ref)
+        assert np.array_equal(s.temp(), T_ref)
+        _assert_apt_equal(s, ref)
+    # the cooling changed the run: more solid than without it
+    still, _, _, _ = O.run(c["phi"], c["temp"], steps, c["tables"], c["scalars"], kmax=50)
+    assert ref[1].sum() > still[1].sum()
+
+
 def test_fp32_within_tolerance():
     c = cases.voronoi_case()
     steps = 12
