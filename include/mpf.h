@@ -95,9 +95,6 @@ typedef struct mpf_stats {
                                        1 around them and were left unchanged */
     int n111_pairs;                 /* 1: every grain's n111 rows come in opposite pairs (row 7-t = -row t,
                                        src/orientation_utils.py:182-212) and the facet search scans four */
-    unsigned long long gb_cells;    /* two-grain boundary cells (no liquid in the stencil) that the
-                                       classification pass finished without the merge pass, summed
-                                       over steps */
 } mpf_stats;
 
 /* ---- resident solver ---------------------------------------------------- */

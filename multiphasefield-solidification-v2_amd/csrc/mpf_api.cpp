@@ -907,7 +907,6 @@ extern "C" int mpf_get_stats(mpf_handle *h, mpf_stats *st)
     st->merge_cells = s.merge_cells;
     st->front_cells = s.front_cells;
     st->light_cells = s.light_cells;
-    st->gb_cells = s.gb_cells;
     st->n111_pairs = h->n111_pairs;
     st->class_tables = h->tab_class;
     for (int j = 0; j < 4; ++j) st->kernel_ms[j] = h->kern_ms[j];

@@ -97,27 +97,6 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
-// Chunks [c0, c1) with stride dc of a list of nch chunks for block blk of
-// nblk: with a multiple of 8 blocks (blocks are dealt round-robin over the 8
-// XCDs, so blk % 8 is the block's XCD when blk is its grid index) the blocks
-// of XCD x take the x-th eighth of the list in turn, so that neighbouring
-// list entries -- neighbouring cells, since the lists are filled tile by
-// tile -- are processed together on one XCD and share its L2.
-struct ChunkRange {
-    int c0, c1, dc;
-};
-__device__ __forceinline__ ChunkRange xcd_chunks(int nch, int blk, int nblk)
-{
-    ChunkRange r{blk, nch, nblk};
-    if ((nblk & 7) == 0) {
-        const int x = blk & 7;
-        r.c0 = (int)((long long)nch * x / 8) + (blk >> 3);
-        r.c1 = (int)((long long)nch * (x + 1) / 8);
-        r.dc = nblk >> 3;
-    }
-    return r;
-}
-
 __device__ __forceinline__ int *wl_counter(const StepArgs<Real> &a, int list, int step)
 {
     return a.wl_count + list * 2 + (step & 1);
@@ -768,9 +747,8 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
     __shared__ int s_base[2][NQ];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int par = 0;
-    const ChunkRange cr = xcd_chunks((n + (int)blockDim.x - 1) / (int)blockDim.x, blockIdx.x, gridDim.x);
-    for (int ch = cr.c0; ch < cr.c1; ch += cr.dc, par ^= 1) {
-        int i = ch * (int)blockDim.x + threadIdx.x;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x, par ^= 1) {
+        int i = base + threadIdx.x;
         bool valid = i < n;
         const size_t c = valid ? (size_t)wl_list(a, WL_MERGE)[i] : 0;
         const Where W = locate<B>(a, s_coff, c);
@@ -839,8 +817,7 @@ __global__ __launch_bounds__(256) void k_merge(StepArgs<Real> a)
 // all interface cells); k_front_multi, concurrently, the classification
 // pass's buckets of 2-4 solids and every bucket of the merge pass.
 static_assert(NFB == FAST_A, "interface buckets cover the register path's solid counts");
-// XCD: blk is the block's grid index (chunks by XCD, xcd_chunks)
-template <int DIM, int TC, int B, int NS, int G, bool XCD = false>
+template <int DIM, int TC, int B, int NS, int G>
 __device__ __forceinline__ void front_bucket(const StepArgs<Real> &a, const uint32_t *s_coff, int wl, int blk,
                                              int nblk)
 {
@@ -851,10 +828,7 @@ __device__ __forceinline__ void front_bucket(const StepArgs<Real> &a, const uint
     const bool lead = (threadIdx.x & (G - 1)) == 0;
     if (blk == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&a.stats->front_cells, (unsigned long long)n);
     const int per = blockDim.x / G;
-    const ChunkRange cr = XCD ? xcd_chunks((n + per - 1) / per, blk, nblk) : ChunkRange{blk, (n + per - 1) / per, nblk};
-    for (int ch = cr.c0; ch < cr.c1; ch += cr.dc) {
-        const int i = ch * per + (int)threadIdx.x / G;
-        if (i >= n) continue;
+    for (int i = blk * per + (int)threadIdx.x / G; i < n; i += nblk * per) {
         const size_t c = (size_t)list[i];
         const Where W = locate<B>(a, s_coff, c);
         Cell<Real, DIM, NS> C;
@@ -875,7 +849,7 @@ __global__ __launch_bounds__(128) void k_front1(StepArgs<Real> a)
     if (a.halt_mode && a.stats->halt_step < a.step) return;
     __shared__ uint32_t s_coff[B ? MAX_INST : 1];
     load_cell_offsets<B>(a, s_coff);
-    front_bucket<DIM, TC, B, 1, 1, true>(a, s_coff, WL_FRONT, blockIdx.x, gridDim.x);
+    front_bucket<DIM, TC, B, 1, 1>(a, s_coff, WL_FRONT, blockIdx.x, gridDim.x);
 }
 
 // The grid (at least 8 blocks) is split over the seven lists in proportion
@@ -1033,10 +1007,11 @@ __global__ void k_apt_slots(Consts<Real> K, SlotView<Real> S, int32_t *mf, int32
         n = 1;
         if (n <= K.kmax && n <= depth) mf[(size_t)(n - 1) * ncell + r] = 0;
     }
-    M.init(S, S.stride);
+    const SlotAcc<Real> acc{S, S.stride};
+    M.init(acc);
     Real vals[NP];
     uint32_t g;
-    while ((g = M.next(S, S.stride, vals)) != NO_ID) {
+    while ((g = M.next(acc, vals)) != NO_ID) {
         if (!apt_active<NP>(vals)) continue;
         n += 1;
         if (n <= K.kmax && n <= depth) mf[(size_t)(n - 1) * ncell + r] = (int32_t)g;

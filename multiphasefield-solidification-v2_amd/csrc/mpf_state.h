@@ -74,7 +74,6 @@ struct Stats {
     unsigned long long merge_cells;    // cells queued for the merge pass (summed over steps)
     unsigned long long front_cells;    // solid-liquid interface cells (summed over steps)
     unsigned long long light_cells;    // cells of the tile planes the classification pass computed
-    unsigned long long gb_cells;       // two-grain boundary cells the classification pass finished
 };
 
 // One instance of a batch: its grid and where its cells, classification

@@ -68,7 +68,7 @@ class Stats(C.Structure):
         ("halt_step", C.c_int), ("steps_done", C.c_int), ("last_step_ms", C.c_double),
         ("merge_cells", C.c_ulonglong), ("front_cells", C.c_ulonglong),
         ("class_tables", C.c_int), ("kernel_steps", C.c_int), ("kernel_ms", C.c_double * 4),
-        ("light_cells", C.c_ulonglong), ("n111_pairs", C.c_int), ("gb_cells", C.c_ulonglong),
+        ("light_cells", C.c_ulonglong), ("n111_pairs", C.c_int)
     ]
 
 

@@ -240,7 +240,6 @@ def stats_dict(s) -> dict:
             "last_step_ms": s.last_step_ms, "merge_cells": s.merge_cells,
             "front_cells": s.front_cells, "class_tables": bool(s.class_tables),
             "kernel_steps": s.kernel_steps, "light_cells": s.light_cells, "n111_pairs": bool(s.n111_pairs),
-            "gb_cells": s.gb_cells,
             "kernel_ms": dict(zip(("light", "merge", "front", "general"), list(s.kernel_ms)))}
 
 

@@ -86,7 +86,7 @@ def step(phi, T_cooled, tables, scalars, kmax=50, aniso=True, torque=True, preci
                                           tuple(range(dim - 1, -1, -1)))).ravel()
     lo, co = np.zeros_like(liq), np.zeros_like(cnt)
     so, vo = np.zeros_like(sid), np.zeros_like(sval)
-    paths = np.zeros(5, np.int32)
+    paths = np.zeros(4, np.int32)
     path_of = np.zeros(liq.size, np.int8)
     s = L.scalars_struct(scalars)
     vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731

@@ -75,27 +75,6 @@ static int step(const Consts<R> &K, SlotView<R> in, SlotView<R> out, const doubl
                         if (path_of) path_of[c] = 0;
                         continue;
                     }
-                    // two-grain boundary cells: the classification pass's
-                    // update from the first two slots (gb2_cell)
-                    {
-                        uint32_t pcn[NP], pi0[NP], pi1[NP];
-                        R pv1[NP];
-                        for (int p = 0; p < NP; ++p) {
-                            pcn[p] = in.cnt[pc[p]];
-                            pi0[p] = pcn[p] ? in.sid[pc[p]] : NO_ID;
-                            pi1[p] = pcn[p] > 1 ? in.sid[ncell + pc[p]] : NO_ID;
-                            pv1[p] = pcn[p] > 1 ? in.sval[ncell + pc[p]] : R(0);
-                        }
-                        Cell<R, DIM, 2> D;
-                        if (K.kmax >= 2 && gb2_cell<R, DIM>(pcn, pi0, psv, pi1, pv1, pl, D)) {
-                            SlotLiq<R> L{in.liq, K.nx, K.ny};
-                            cell_update<R, DIM, 2, false, TC, true, 1, false>(K, L, l, m, k, T[c], D);
-                            write_cell<R, DIM, 2>(out, ncell, c, D);
-                            paths[4]++;
-                            if (path_of) path_of[c] = 4;
-                            continue;
-                        }
-                    }
                     if (run_cell<R, DIM, 4, TC>(K, in, out, ncell, l, m, k, T[c])) {
                         paths[1]++;
                         if (path_of) path_of[c] = 1;

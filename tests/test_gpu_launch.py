@@ -149,3 +149,4 @@ def test_batch_graph_replay():
             T = np.broadcast_to(c["temp"][0, 0], shapes[i]).copy()
             ref, _, _, _ = O.run(pad(c["phi"]), T, 5, c0["tables"], c0["scalars"], kmax=50)
             assert np.array_equal(g.phi(i), ref)
+
