@@ -693,7 +693,7 @@ static void step_grids(mpf_handle *h, int grids[3])
         size_t b = (h->ncell + threads_per_block - 1) / threads_per_block;
         return (int)(b < (size_t)cap ? (b ? b : 1) : cap);
     };
-    grids[0] = h->grid_req[0] > 0 ? h->grid_req[0] : blocks(256, 2048);
+    grids[0] = h->grid_req[0] > 0 ? h->grid_req[0] : blocks(256, 8192);
     grids[1] = h->grid_req[1] > 0 ? h->grid_req[1] : blocks(128, 2048);
     grids[2] = h->grid_req[2] > 0 ? h->grid_req[2] : blocks(64, 256);
 }

@@ -31,6 +31,10 @@
 
 #include "mpf_math.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 typedef struct {
     int nx, ny, nz, N;
     double dx, dt, T_melt, Sf, eps0_sl, w0_sl;
@@ -940,6 +944,16 @@ int orc_run(int dim, double *phi, double *temp, int nsteps, int kmax,
     if (cur != phi) memcpy(phi, cur, nphi * sizeof(double));
     free(buf); free(mf); free(nf);
     return done;
+}
+
+/* Threads the OpenMP loops of the oracle use. */
+int orc_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 /* The shared transcendentals, exported so the tests can compare them with

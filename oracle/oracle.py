@@ -61,6 +61,7 @@ def lib():
         L.orc_run.argtypes = [C.c_int, dp, dp, C.c_int, C.c_int, dp, dp, dp, dp, P,
                               C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, ip]
         L.orc_run.restype = C.c_int
+        L.orc_threads.restype = C.c_int
         for fn in ("orc_mpf_tan", "orc_mpf_acos", "orc_mpf_tanh"):
             getattr(L, fn).argtypes = [C.c_double]
             getattr(L, fn).restype = C.c_double
@@ -147,6 +148,11 @@ def run(phi, temp, nsteps, tables, scalars, kmax=50, aniso=True, torque=True,
     if done < 0:
         raise MemoryError("oracle run could not allocate its buffers")
     return phi, temp, done, st
+
+
+def threads() -> int:
+    """OpenMP threads the oracle's loops use."""
+    return int(lib().orc_threads())
 
 
 def probe3(phi, n111, scalars, l, m, k, s, use_libm=False):
