@@ -255,6 +255,20 @@ int mpf_dense_update(const void *phi, void *phi_new, int dtype, const void *temp
                      int nz, int dim, int depth, int kmax, int enable_anisotropy, int enable_torque,
                      void *stream);
 
+/* mpf_dense_update without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  The same update and arguments as mpf_dense_update;
+ * the tables are gathered into a file-scope device array of the library (one
+ * per GPU, held until the process ends, 1.6 MB: n_phase <= 256, larger values
+ * are rejected with MPF_EINVAL).  Nothing is allocated or freed and the call
+ * does not wait for the kernel.  Calls that overlap on one GPU race on that
+ * array. */
+int mpf_dense_update_noalloc(const void *phi, void *phi_new, int dtype, const void *temp, int temp_dtype,
+                             const int32_t *mf, const int32_t *nf, const void *wij, const void *aij,
+                             const void *mij, const void *n111, const mpf_scalars *s, int n_phase, int nx,
+                             int ny, int nz, int dim, int depth, int kmax, int enable_anisotropy,
+                             int enable_torque, void *stream);
+
 /* kernel_update_temp_3d (src/gpu_kernels_3d.py:833-838) / kernel_update_temp
  * (src/gpu_kernels.py:520-525): temp[i] -= cooling_rate over n values. */
 int mpf_dense_temp(void *temp, int temp_dtype, double cooling_rate, size_t n, void *stream);

@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1346,22 +1347,16 @@ extern "C" int mpf_dense_apt(const void *phi, int dtype, int32_t *mf, int32_t *n
     return 0;
 }
 
-extern "C" int mpf_dense_update(const void *phi, void *phi_new, int dtype, const void *temp, int temp_dtype,
-                                const int32_t *mf, const int32_t *nf, const void *wij, const void *aij,
-                                const void *mij, const void *n111, const mpf_scalars *s, int N, int nx, int ny,
-                                int nz, int dim, int depth, int kmax, int aniso, int torque, void *stream)
+// The update on the dense layout with the four tables gathered into `tab`
+// (tab_bytes available), the contiguous block the device code indexes.
+// sync: wait for the kernel before returning.
+static int dense_update_into(const char *who, const void *phi, void *phi_new, int dtype, const void *temp,
+                             int temp_dtype, const int32_t *mf, const int32_t *nf, const void *wij,
+                             const void *aij, const void *mij, const void *n111, const mpf_scalars *s, int N,
+                             int nx, int ny, int nz, int dim, int depth, int kmax, int aniso, int torque,
+                             void *stream, void *tab, bool sync)
 {
-    if (!phi || !phi_new || !temp || !mf || !nf || !wij || !aij || !mij || !n111 || !s)
-        return fail(MPF_EINVAL, "mpf_dense_update: null argument");
-    if (dim != 2 && dim != 3) return fail(MPF_EINVAL, "dim must be 2 or 3");
-    if (kmax < 1 || kmax > KCAP) return fail(MPF_EINVAL, "kmax must be in [1, %d]", KCAP);
-    if (depth < 1) return fail(MPF_EINVAL, "depth must be positive");
-    // The drop-in kernel takes the four tables as separate arrays; gather
-    // them into the contiguous block the device code indexes.
-    size_t esz = dtype == MPF_FP64 ? 8 : 4, nn = (size_t)N * N;
-    size_t tb = (3 * nn + (size_t)N * 24) * esz;
-    void *tab = nullptr;
-    HIP_TRY(hipMalloc(&tab, tb));
+    const size_t esz = dtype == MPF_FP64 ? 8 : 4, nn = (size_t)N * N;
     hipStream_t st = as_stream(stream);
     hipError_t e = hipMemcpyAsync(tab, wij, nn * esz, hipMemcpyDeviceToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync((char *)tab + nn * esz, aij, nn * esz, hipMemcpyDeviceToDevice, st);
@@ -1383,10 +1378,90 @@ extern "C" int mpf_dense_update(const void *phi, void *phi_new, int dtype, const
                                             depth, st);
         }
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipFree(tab);
-    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_dense_update: %s", hipGetErrorString(e));
+    if (e == hipSuccess && sync) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(MPF_EHIP, "%s: %s", who, hipGetErrorString(e));
     return 0;
+}
+
+static int dense_update_args(const char *who, const void *phi, const void *phi_new, const void *temp,
+                             const int32_t *mf, const int32_t *nf, const void *wij, const void *aij,
+                             const void *mij, const void *n111, const mpf_scalars *s, int dim, int depth, int kmax)
+{
+    if (!phi || !phi_new || !temp || !mf || !nf || !wij || !aij || !mij || !n111 || !s)
+        return fail(MPF_EINVAL, "%s: null argument", who);
+    if (dim != 2 && dim != 3) return fail(MPF_EINVAL, "dim must be 2 or 3");
+    if (kmax < 1 || kmax > KCAP) return fail(MPF_EINVAL, "kmax must be in [1, %d]", KCAP);
+    if (depth < 1) return fail(MPF_EINVAL, "depth must be positive");
+    return 0;
+}
+
+extern "C" int mpf_dense_update(const void *phi, void *phi_new, int dtype, const void *temp, int temp_dtype,
+                                const int32_t *mf, const int32_t *nf, const void *wij, const void *aij,
+                                const void *mij, const void *n111, const mpf_scalars *s, int N, int nx, int ny,
+                                int nz, int dim, int depth, int kmax, int aniso, int torque, void *stream)
+{
+    if (int rc = dense_update_args("mpf_dense_update", phi, phi_new, temp, mf, nf, wij, aij, mij, n111, s, dim,
+                                   depth, kmax))
+        return rc;
+    size_t esz = dtype == MPF_FP64 ? 8 : 4, nn = (size_t)N * N;
+    size_t tb = (3 * nn + (size_t)N * 24) * esz;
+    void *tab = nullptr;
+    HIP_TRY(hipMalloc(&tab, tb));
+    int rc = dense_update_into("mpf_dense_update", phi, phi_new, dtype, temp, temp_dtype, mf, nf, wij, aij, mij,
+                               n111, s, N, nx, ny, nz, dim, depth, kmax, aniso, torque, stream, tab, true);
+    hipFree(tab);
+    return rc;
+}
+
+// ---- the update without allocation ------------------------------------------
+// The gathered tables of mpf_dense_update_noalloc live in this file-scope
+// device array, one copy per GPU for the life of the process.  Host code
+// never passes its name: the address comes from hipGetSymbolAddress, taken
+// once per device (with that device current) on the first call there.
+constexpr int NOALLOC_MAX_PHASES = 256;
+constexpr size_t NOALLOC_SCRATCH_BYTES =
+    (3 * (size_t)NOALLOC_MAX_PHASES * NOALLOC_MAX_PHASES + (size_t)NOALLOC_MAX_PHASES * 24) * 8;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+static int scratch_on_current_device(void **out)
+{
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(MPF_EINVAL, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MPF_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+extern "C" int mpf_dense_update_noalloc(const void *phi, void *phi_new, int dtype, const void *temp, int temp_dtype,
+                                        const int32_t *mf, const int32_t *nf, const void *wij, const void *aij,
+                                        const void *mij, const void *n111, const mpf_scalars *s, int N, int nx,
+                                        int ny, int nz, int dim, int depth, int kmax, int aniso, int torque,
+                                        void *stream)
+{
+    if (int rc = dense_update_args("mpf_dense_update_noalloc", phi, phi_new, temp, mf, nf, wij, aij, mij, n111, s,
+                                   dim, depth, kmax))
+        return rc;
+    if (N < 1) return fail(MPF_EINVAL, "mpf_dense_update_noalloc: n_phase must be positive");
+    if (dtype != MPF_FP32 && dtype != MPF_FP64) return fail(MPF_EINVAL, "mpf_dense_update_noalloc: bad dtype");
+    if (N > NOALLOC_MAX_PHASES)
+        return fail(MPF_EINVAL, "mpf_dense_update_noalloc: n_phase %d exceeds the %d phases its tables hold", N,
+                    NOALLOC_MAX_PHASES);
+    void *tab = nullptr;
+    if (int rc = scratch_on_current_device(&tab)) return rc;
+    return dense_update_into("mpf_dense_update_noalloc", phi, phi_new, dtype, temp, temp_dtype, mf, nf, wij, aij,
+                             mij, n111, s, N, nx, ny, nz, dim, depth, kmax, aniso, torque, stream, tab, false);
 }
 
 extern "C" int mpf_dense_temp(void *temp, int temp_dtype, double cooling, size_t n, void *stream)

@@ -24,7 +24,7 @@ EXPORTS = (
     "mpf_upload_temp", "mpf_download_temp", "mpf_step", "mpf_download_phi", "mpf_download_apt",
     "mpf_get_stats", "mpf_reset_stats", "mpf_set_kernel_timing", "mpf_observables", "mpf_device_bytes",
     "mpf_compare_phi",
-    "mpf_dense_apt", "mpf_dense_update", "mpf_dense_temp",
+    "mpf_dense_apt", "mpf_dense_update", "mpf_dense_update_noalloc", "mpf_dense_temp",
     "mpf_malloc", "mpf_free", "mpf_memcpy_htod", "mpf_memcpy_dtoh", "mpf_memcpy_dtod", "mpf_memset",
     "mpf_device_synchronize", "mpf_last_error", "mpf_abi_version",
     "mpf_batch_create", "mpf_batch_size", "mpf_batch_set_tables", "mpf_batch_upload_phi",
@@ -125,6 +125,9 @@ def lib():
         "mpf_dense_update": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp, vp, vp, vp, vp, vp,
                                        C.POINTER(Scalars), C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
+        "mpf_dense_update_noalloc": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp, vp, vp, vp, vp, vp,
+                                               C.POINTER(Scalars), C.c_int, C.c_int, C.c_int, C.c_int,
+                                               C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
         "mpf_dense_temp": (C.c_int, [vp, C.c_int, C.c_double, C.c_size_t, vp]),
         "mpf_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
         "mpf_free": (C.c_int, [vp]),
