@@ -67,20 +67,6 @@ struct DenseLiq {
     }
 };
 
-// Readers of the slot lists of a stencil point.  SlotAcc reads the slot
-// layout in device (or host) memory, points given as cell indices; the tile
-// merge pass reads a copy staged in LDS through an accessor of the same shape.
-template <typename Real>
-struct SlotAcc {
-    using Index = size_t;
-    const SlotView<Real> &S;
-    size_t ncell;
-    MPF_DEV int cnt(Index i) const { return S.cnt[i]; }
-    MPF_DEV Real liq(Index i) const { return S.liq[i]; }
-    MPF_DEV uint32_t sid(Index i, int s) const { return S.sid[(size_t)s * ncell + i]; }
-    MPF_DEV Real sval(Index i, int s) const { return S.sval[(size_t)s * ncell + i]; }
-};
-
 // k-way merge of the sorted slot lists of the stencil points.  Yields the
 // union of grain ids present at any of the points in ascending order with the
 // value at every point (0 where absent).
@@ -89,36 +75,36 @@ struct SlotAcc {
 // in one round trip after the counts, and a refill (lists of 3 or more
 // slots) is needed only two merge steps after it is issued, so the merge does
 // not wait on memory once per step.
-template <typename Real, int DIM, typename Acc = SlotAcc<Real>>
+template <typename Real, int DIM>
 struct Merge {
     static constexpr int NP = (DIM == 3) ? 7 : 5;
-    typename Acc::Index pc[NP];
+    size_t pc[NP];
     int cnt[NP], pos[NP];   // pos: slot index of the head
     uint32_t hid[NP], nid[NP];
     Real hv[NP], nv[NP];
 
-    MPF_DEV void init(const Acc &A)
+    MPF_DEV void init(const SlotView<Real> &S, size_t ncell)
     {
 #pragma unroll
-        for (int p = 0; p < NP; ++p) cnt[p] = A.cnt(pc[p]);
+        for (int p = 0; p < NP; ++p) cnt[p] = S.cnt[pc[p]];
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
             pos[p] = 0;
             hid[p] = nid[p] = NO_ID;
             hv[p] = nv[p] = Real(0.0);
             if (cnt[p] > 0) {
-                hid[p] = A.sid(pc[p], 0);
-                hv[p] = A.sval(pc[p], 0);
+                hid[p] = S.sid[pc[p]];
+                hv[p] = S.sval[pc[p]];
             }
             if (cnt[p] > 1) {
-                nid[p] = A.sid(pc[p], 1);
-                nv[p] = A.sval(pc[p], 1);
+                nid[p] = S.sid[ncell + pc[p]];
+                nv[p] = S.sval[ncell + pc[p]];
             }
         }
     }
 
     // Next candidate; returns NO_ID when all lists are exhausted.
-    MPF_DEV uint32_t next(const Acc &A, Real *vals)
+    MPF_DEV uint32_t next(const SlotView<Real> &S, size_t ncell, Real *vals)
     {
         uint32_t g = hid[0];
 #pragma unroll
@@ -133,8 +119,9 @@ struct Merge {
                 hv[p] = nv[p];
                 nid[p] = NO_ID;
                 if (pos[p] + 1 < cnt[p]) {
-                    nid[p] = A.sid(pc[p], pos[p] + 1);
-                    nv[p] = A.sval(pc[p], pos[p] + 1);
+                    size_t o = (size_t)(pos[p] + 1) * ncell + pc[p];
+                    nid[p] = S.sid[o];
+                    nv[p] = S.sval[o];
                 }
             } else {
                 vals[p] = Real(0.0);
@@ -160,20 +147,18 @@ MPF_DEV bool apt_active(const Real *vals)
 // Gather the active set of one cell (APT with the checked KMAX clamp).
 // Returns the total number of active phases (before the clamp) in *ntot and
 // whether the kept solids fit in A (*fits).
-// The stencil points are given in pc (indices of the accessor).
-template <typename Real, int DIM, int A, typename Acc>
-MPF_DEV void gather_acc(const Consts<Real> &K, const Acc &S, const typename Acc::Index *pc,
-                        Cell<Real, DIM, A> &C, int *ntot, bool *fits)
+template <typename Real, int DIM, int A>
+MPF_DEV void gather(const Consts<Real> &K, const SlotView<Real> &S, size_t ncell,
+                                       int l, int m, int k, Cell<Real, DIM, A> &C, int *ntot, bool *fits)
 {
     constexpr int NP = Cell<Real, DIM, A>::NP;
     constexpr bool SMALL = (A <= 8);
-    Merge<Real, DIM, Acc> M;
-#pragma unroll
-    for (int p = 0; p < NP; ++p) M.pc[p] = pc[p];
+    Merge<Real, DIM> M;
+    stencil_cells<Real, DIM>(K, l, m, k, M.pc);
     // liquid (dense)
     Real lv[NP];
 #pragma unroll
-    for (int p = 0; p < NP; ++p) lv[p] = S.liq(M.pc[p]);
+    for (int p = 0; p < NP; ++p) lv[p] = S.liq[M.pc[p]];
     C.liq = apt_active<NP>(lv);
     C.gid[0] = 0;
 #pragma unroll
@@ -181,7 +166,7 @@ MPF_DEV void gather_acc(const Consts<Real> &K, const Acc &S, const typename Acc:
         C.v[0][p] = lv[p];
         C.dom1[p] = Real(0.0);
     }
-    M.init(S);
+    M.init(S, ncell);
     int keep = K.kmax - (C.liq ? 1 : 0); // solids kept under the KMAX clamp
     int nact = 0;                        // active solids found
     C.nsol = 0;
@@ -193,7 +178,7 @@ MPF_DEV void gather_acc(const Consts<Real> &K, const Acc &S, const typename Acc:
             if (nact < keep) {
                 uint32_t g;
                 bool act = false;
-                while ((g = M.next(S, vals)) != NO_ID) {
+                while ((g = M.next(S, ncell, vals)) != NO_ID) {
                     if constexpr (DIM == 2) {
                         if (g == 1) {
 #pragma unroll
@@ -217,7 +202,7 @@ MPF_DEV void gather_acc(const Consts<Real> &K, const Acc &S, const typename Acc:
     }
     // count (and, for the general path, store) the remaining active solids
     uint32_t g;
-    while ((g = M.next(S, vals)) != NO_ID) {
+    while ((g = M.next(S, ncell, vals)) != NO_ID) {
         if constexpr (DIM == 2) {
             if (g == 1) {
 #pragma unroll
@@ -237,17 +222,6 @@ MPF_DEV void gather_acc(const Consts<Real> &K, const Acc &S, const typename Acc:
     *ntot = nact + (C.liq ? 1 : 0);
     int kept = nact < keep ? nact : keep;
     *fits = (kept <= A);
-}
-
-// Gather the active set of cell (l, m, k) from the slot layout in memory.
-template <typename Real, int DIM, int A>
-MPF_DEV void gather(const Consts<Real> &K, const SlotView<Real> &S, size_t ncell,
-                                       int l, int m, int k, Cell<Real, DIM, A> &C, int *ntot, bool *fits)
-{
-    size_t pc[Cell<Real, DIM, A>::NP];
-    stencil_cells<Real, DIM>(K, l, m, k, pc);
-    SlotAcc<Real> acc{S, ncell};
-    gather_acc<Real, DIM, A>(K, acc, pc, C, ntot, fits);
 }
 
 template <typename Real, int DIM, int A>
