@@ -96,8 +96,17 @@ def run_trajectory(W, precision, mid, warmup, steps, total, kernel_steps=0, grap
     s = new_solver(W, precision)
     segs = []
     done = 0
+    tuned0 = None
+    if autotune and mid > 0:
+        # the first steps at the default launch shape, then a shape tuned on
+        # the early state for the first half of the run
+        r = segment(s, min(5, mid))
+        r["from"] = 0
+        segs.append(r)
+        done = r["steps"]
+        tuned0 = s.autotune(autotune)
     while done < mid:
-        n = min(seg, mid - done)
+        n = min(seg - done % seg, mid - done)
         r = segment(s, n)
         r["from"] = done
         segs.append(r)
@@ -105,8 +114,9 @@ def run_trajectory(W, precision, mid, warmup, steps, total, kernel_steps=0, grap
         ob = s.observables(dx=W["p"]["dx"])
         r["front_mean"] = ob["front_mean"]
         r["stored_slots"] = ob["stored_slots"]
-        log(f"{precision}: step {done}: {r['ms'] / n:.3f} ms/step, front {ob['front_mean']:.1f}, "
-            f"merge {r['merge_cells'] // n}, light {r['light_cells'] // n}")
+        if done % seg == 0 or done == mid:
+            log(f"{precision}: step {done}: {r['ms'] / n:.3f} ms/step, front {ob['front_mean']:.1f}, "
+                f"merge {r['merge_cells'] // n}, light {r['light_cells'] // n}")
     if warmup:
         r = segment(s, warmup)
         r["from"] = done
@@ -149,7 +159,8 @@ def run_trajectory(W, precision, mid, warmup, steps, total, kernel_steps=0, grap
             r["stored_slots"] = ob["stored_slots"]
             log(f"{precision}: step {done}: {r['ms'] / n:.3f} ms/step, front {ob['front_mean']:.1f}")
     return {"solver": s, "segments": segs, "timed": timed, "ob0": ob0, "ob1": ob1, "kernel_ms": kern,
-            "tuned": tuned, "launch": s.launch(), "bytes": s.device_bytes, "steps_done": done}
+            "tuned": tuned, "tuned_early": tuned0, "launch": s.launch(), "bytes": s.device_bytes,
+            "steps_done": done}
 
 
 def trajectory_summary(run, ncell):
@@ -472,6 +483,7 @@ def main():
     ap.add_argument("--surrogate", type=int, default=1,
                     help="train the surrogate and measure assisted stepping at 128^3 (0 skips it)")
     ap.add_argument("--c2", type=int, default=1, help="FP32 vs FP64 on the 128^3 16-grain case (0 skips it)")
+    ap.add_argument("--pcie", type=int, default=1, help="the host-buffer (PCIe-inclusive) rate at 128^3 (0 skips it)")
     ap.add_argument("--summarize-trace", default=None,
                     help="print per-kernel averages over the last --steps dispatches of a rocprofv3 CSV and exit")
     args = ap.parse_args()
@@ -566,8 +578,9 @@ def main():
     if args.c2:
         out["c2_fp32_vs_fp64"] = c2_precision()
         log(f"c2: {out['c2_fp32_vs_fp64']}")
-    out["pcie_inclusive"] = pcie_inclusive()
-    log(f"pcie inclusive: {out['pcie_inclusive']}")
+    if args.pcie:
+        out["pcie_inclusive"] = pcie_inclusive()
+        log(f"pcie inclusive: {out['pcie_inclusive']}")
     if args.sweep_steps > 0:
         out["batched_sweep"] = batched_sweep(args.sweep_steps, bool(args.sweep_graphs))
     if args.surrogate:
