@@ -22,7 +22,7 @@ def results(tmp_path_factory):
 
 def test_verification_cases_pass(results):
     tmp_path, res = results
-    failed = [(r.name, r.failure_reason) for r in res if not r.passed]
+    failed = [(r.name, r.failure) for r in res if not r.passed]
     assert not failed, failed
     by = {r.name: r for r in res}
     # reference: status [1, 52], first overflowing cell [0, 1, 1]
