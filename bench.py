@@ -490,6 +490,8 @@ def main():
     if args.summarize_trace:
         print(json.dumps(summarize_trace(args.summarize_trace, args.steps), indent=1))
         return
+    if args.surrogate:
+        import mpf.surrogate  # noqa: F401 -- torch's HIP runtime has to be loaded before the solver's
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the solver has no "
               "multi-GPU path in this build.", file=sys.stderr)

@@ -38,8 +38,9 @@ using namespace mpf;
                                                        const void *, int, const int32_t *,               \
                                                        const int32_t *, int, hipStream_t);               \
     extern "C" hipError_t mpf_launch_dense_temp##SFX(void *, int, double, size_t, hipStream_t);          \
-    extern "C" hipError_t mpf_launch_observe##SFX(SlotView<R>, int, int, int, int, double *, double *,   \
-                                                  int32_t *, hipStream_t);                                \
+    extern "C" hipError_t mpf_launch_observe##SFX(SlotView<R>, const Inst *, const long long *, int,     \
+                                                  long long, int, double *, double *, int32_t *,          \
+                                                  hipStream_t);                                           \
     extern "C" hipError_t mpf_launch_probe##SFX(SlotView<R>, const Inst *, int, const int32_t *,         \
                                                 const int32_t *, const double *, const double *, double,  \
                                                 double, double *, hipStream_t);                           \
@@ -129,6 +130,9 @@ struct mpf_handle {
     // captured steps, one per (state buffer, temperature buffer, step parity)
     int use_graphs = 0;
     hipGraphExec_t gexec[8] = {};
+    // observables' buffer: results and instance offsets, kept between calls
+    void *obs = nullptr;
+    size_t obs_cap = 0;
 };
 
 static void drop_graphs(mpf_handle *h)
@@ -198,6 +202,7 @@ extern "C" void mpf_destroy(mpf_handle *h)
     hipFree(h->item_busy);
     hipFree(h->stats);
     hipFree(h->inst_d);
+    hipFree(h->obs);
     drop_graphs(h);
     if (h->side) hipStreamSynchronize(h->side);
     if (h->side2) hipStreamSynchronize(h->side2);
@@ -1118,8 +1123,9 @@ extern "C" int mpf_reset_stats(mpf_handle *h)
 
 // Observables of instances [i0, i0 + n): per instance N phase sums, the
 // topmost solid cell of every column (fronts one instance after the other)
-// and 7 scalars.  One device buffer, the reductions of all instances
-// launched back to back, one synchronisation.
+// and 7 scalars.  One launch over all the instances' planes and columns, in
+// a device buffer the handle keeps between calls (grown when a call needs
+// more), one copy back.
 static int observe_many(mpf_handle *h, int i0, int n, double *grain_volume, int32_t *front, double *out)
 {
     if (!grain_volume || !out) return fail(MPF_EINVAL, "observables: null argument");
@@ -1127,44 +1133,60 @@ static int observe_many(mpf_handle *h, int i0, int n, double *grain_volume, int3
     const int N = c.n_phase;
     if (((size_t)N + 256 * 6) * sizeof(double) > 64 * 1024)
         return fail(MPF_EINVAL, "observables support at most %d phases", 64 * 1024 / 8 - 1536);
-    size_t planes = 0, cols = 0;
+    // pre: first plane block, first column block and first column per instance
+    std::vector<long long> pre(3 * ((size_t)n + 1));
+    long long planes = 0, colblk = 0, cols = 0;
     for (int q = 0; q < n; ++q) {
-        planes += h->inst[i0 + q].nz;
-        cols += (size_t)h->inst[i0 + q].nx * h->inst[i0 + q].ny;
+        const Inst &g = h->inst[i0 + q];
+        pre[q] = planes;
+        pre[n + 1 + q] = colblk;
+        pre[2 * (size_t)n + 2 + q] = cols;
+        planes += g.nz;
+        colblk += ((long long)g.nx * g.ny + 255) / 256;
+        cols += (long long)g.nx * g.ny;
     }
-    const size_t bytes = (size_t)n * N * 8 + planes * 6 * 8 + cols * 4;
-    void *d = nullptr;
+    pre[n] = planes;
+    pre[2 * (size_t)n + 1] = colblk;
+    pre[3 * (size_t)n + 2] = cols;
+    if (planes + colblk > 0x7fffffffLL) return fail(MPF_EINVAL, "observables: too many planes and columns");
+    const size_t gv_bytes = (size_t)n * N * 8, part_bytes = (size_t)planes * 6 * 8, fr_bytes = (size_t)cols * 4;
+    const size_t res_bytes = (gv_bytes + part_bytes + fr_bytes + 7) & ~(size_t)7;
+    const size_t bytes = res_bytes + pre.size() * sizeof(long long);
     HIP_TRY(hipStreamSynchronize(h->stream));
-    HIP_TRY(hipMalloc(&d, bytes));
-    double *gv = (double *)d, *part = gv + (size_t)n * N;
-    int32_t *fr = (int32_t *)(part + planes * 6);
-    hipError_t e = hipMemsetAsync(d, 0, bytes, h->stream);
-    size_t po = 0, fo = 0;
-    for (int q = 0; q < n && e == hipSuccess; ++q) {
-        const int ii = i0 + q;
-        const Inst &g = h->inst[ii];
-        double *gq = gv + (size_t)q * N, *pq = part + po * 6;
-        int32_t *fq = fr + fo;
-        if (h->rsz == 8)
-            e = mpf_launch_observe_f64(iview<double>(h, h->cur, ii), g.nx, g.ny, g.nz, N, gq, pq, fq, h->stream);
-        else
-            e = mpf_launch_observe_f32(iview<float>(h, h->cur, ii), g.nx, g.ny, g.nz, N, gq, pq, fq, h->stream);
-        po += g.nz;
-        fo += (size_t)g.nx * g.ny;
+    if (bytes > h->obs_cap) {
+        hipFree(h->obs);
+        h->obs = nullptr;
+        h->obs_cap = 0;
+        HIP_TRY(hipMalloc(&h->obs, bytes));
+        h->obs_cap = bytes;
     }
-    std::vector<double> hp(planes * 6);
+    char *base = (char *)h->obs;
+    double *gv = (double *)base, *part = (double *)(base + gv_bytes);
+    int32_t *fr = (int32_t *)(base + gv_bytes + part_bytes);
+    long long *pre_d = (long long *)(base + res_bytes);
+    hipError_t e = hipMemcpyAsync(pre_d, pre.data(), pre.size() * sizeof(long long), hipMemcpyHostToDevice,
+                                  h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(gv, 0, gv_bytes, h->stream);
+    if (e == hipSuccess) {
+        if (h->rsz == 8)
+            e = mpf_launch_observe_f64(view<double>(h, h->cur), h->inst_d + i0, pre_d, n, planes + colblk, N, gv,
+                                       part, fr, h->stream);
+        else
+            e = mpf_launch_observe_f32(view<float>(h, h->cur), h->inst_d + i0, pre_d, n, planes + colblk, N, gv,
+                                       part, fr, h->stream);
+    }
+    std::vector<char> res(gv_bytes + part_bytes + fr_bytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(res.data(), base, res.size(), hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    if (e == hipSuccess) e = hipMemcpy(grain_volume, gv, (size_t)n * N * 8, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && front) e = hipMemcpy(front, fr, cols * 4, hipMemcpyDeviceToHost);
-    hipFree(d);
     if (e != hipSuccess) return fail(MPF_EHIP, "observables: %s", hipGetErrorString(e));
-    po = 0;
+    memcpy(grain_volume, res.data(), gv_bytes);
+    if (front) memcpy(front, res.data() + gv_bytes + part_bytes, fr_bytes);
+    const double *hp = (const double *)(res.data() + gv_bytes);
     for (int q = 0; q < n; ++q) {
         const Inst &g = h->inst[i0 + q];
         double emax = 0.0, pmin = 1e300, pmax = -1e300, nanc = 0.0, solid = 0.0, slots = 0.0;
         for (int k = 0; k < g.nz; ++k) {
-            const double *p = hp.data() + (po + k) * 6;
+            const double *p = hp + (pre[q] + k) * 6;
             emax = p[0] > emax ? p[0] : emax;
             pmin = p[1] < pmin ? p[1] : pmin;
             pmax = p[2] > pmax ? p[2] : pmax;
@@ -1172,7 +1194,6 @@ static int observe_many(mpf_handle *h, int i0, int n, double *grain_volume, int3
             solid += p[4];
             slots += p[5];
         }
-        po += g.nz;
         double *o = out + (size_t)q * 7;
         o[0] = emax;
         o[1] = pmin;

@@ -1277,33 +1277,73 @@ __global__ void k_dense_temp(void *temp, int temp_f64, double cooling, size_t n)
         ((float *)temp)[i] = ((float *)temp)[i] - (float)cooling;
 }
 
-// FP64 observables (tests/verification_3d.py:409-416, :612-654).  One block
-// per z-plane: per-grain sums with LDS atomics, scalar partials by a
-// deterministic block reduction into part[k][6]: max|sum-1|, min, max, NaN
-// count, sum of (1 - phi_L), stored grain slots.  front[l][m] gets the topmost k with
-// 1 - phi_L > 0.5 from a separate pass.
-__global__ __launch_bounds__(256) void k_observe(SlotView<Real> S, int nx, int ny, int nz, int N,
-                                                 double *gvol, double *part)
+// FP64 observables of a run of instances (tests/verification_3d.py:409-416,
+// :612-654), one launch.  pre[0..n] are the instances' first plane blocks and
+// pre[n+1..2n+1] their first column blocks (counted after all plane blocks),
+// pre[2n+2..3n+2] their first columns in front[].
+//
+// A plane block (one z-plane of one instance): per-grain sums with LDS
+// atomics, added to gvol[q][g]; scalar partials by a deterministic block
+// reduction into part[block][6]: max|sum-1|, min, max, NaN count, sum of
+// (1 - phi_L), stored grain slots.  A column block: front[l][m] of 256
+// columns, the topmost k with 1 - phi_L > 0.5 (-1 when none).
+MPF_DEV int obs_instance(const long long *pre, int n, long long b)
 {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_observe(SlotView<Real> S, const Inst *inst, const long long *pre, int n,
+                                                 int N, double *gvol, double *part, int32_t *front)
+{
+    const long long b = blockIdx.x, planes = pre[n];
+    if (b >= planes) {
+        const long long cb = b - planes;
+        const int q = obs_instance(pre + n + 1, n, cb);
+        const Inst g = inst[q];
+        const long long col = (cb - pre[n + 1 + q]) * 256 + threadIdx.x;
+        if (col >= (long long)g.nx * g.ny) return;
+        const int l = (int)(col % g.nx), m = (int)(col / g.nx);
+        const Real *liq = S.liq + g.cell_off;
+        int f = -1;
+        for (int k = g.nz - 1; k >= 0; --k) {
+            if (1.0 - (double)liq[cidx(l, m, k, g.nx, g.ny)] > 0.5) {
+                f = k;
+                break;
+            }
+        }
+        front[pre[2 * n + 2 + q] + (long long)l * g.ny + m] = f;
+        return;
+    }
     extern __shared__ double sh[];
     double *vol = sh;                  // [N]
     double *red = sh + N;              // [256*6]
-    const int k = blockIdx.x;
-    for (int g = threadIdx.x; g < N; g += blockDim.x) vol[g] = 0.0;
+    const int q = obs_instance(pre, n, b);
+    const Inst g = inst[q];
+    const int nxy = g.nx * g.ny;
+    const size_t c0 = g.cell_off + (size_t)(b - pre[q]) * nxy;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) vol[i] = 0.0;
     __syncthreads();
     double e_max = 0.0, p_min = 1e300, p_max = -1e300, nan_c = 0.0, solid = 0.0, slots = 0.0;
     bool any_absent = false;
-    for (int i = threadIdx.x; i < nx * ny; i += blockDim.x) {
-        size_t c = (size_t)k * nx * ny + i;
+    for (int i = threadIdx.x; i < nxy; i += blockDim.x) {
+        const size_t c = c0 + i;
         double lq = (double)S.liq[c];
         double sum = lq;
-        int n = S.cnt[c];
-        slots += (double)n;
+        int ns = S.cnt[c];
+        slots += (double)ns;
         if (lq != lq) nan_c += 1.0;
         p_min = lq < p_min ? lq : p_min;
         p_max = lq > p_max ? lq : p_max;
         atomicAdd(&vol[0], lq);
-        for (int s = 0; s < n; ++s) {
+        for (int s = 0; s < ns; ++s) {
             size_t o = (size_t)s * S.stride + c;
             double v = (double)S.sval[o];
             sum += v;
@@ -1312,7 +1352,7 @@ __global__ __launch_bounds__(256) void k_observe(SlotView<Real> S, int nx, int n
             p_max = v > p_max ? v : p_max;
             atomicAdd(&vol[S.sid[o]], v);
         }
-        if (n + 1 < N) any_absent = true;
+        if (ns + 1 < N) any_absent = true;
         double e = fabs(sum - 1.0);
         e_max = e > e_max ? e : e_max;
         solid += 1.0 - lq;
@@ -1337,24 +1377,9 @@ __global__ __launch_bounds__(256) void k_observe(SlotView<Real> S, int nx, int n
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int q = 0; q < 6; ++q) part[(size_t)k * 6 + q] = red[q];
-    for (int g = threadIdx.x; g < N; g += blockDim.x) atomicAdd(&gvol[g], vol[g]);
-}
-
-__global__ void k_front_height(const Real *liq, int nx, int ny, int nz, int32_t *front)
-{
-    size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= (size_t)nx * ny) return;
-    int l = (int)(col % nx), m = (int)(col / nx);
-    int f = -1;
-    for (int k = nz - 1; k >= 0; --k) {
-        double s = 1.0 - (double)liq[cidx(l, m, k, nx, ny)];
-        if (s > 0.5) {
-            f = k;
-            break;
-        }
-    }
-    front[(size_t)l * ny + m] = f;
+        for (int i = 0; i < 6; ++i) part[(size_t)b * 6 + i] = red[i];
+    double *gv = gvol + (size_t)q * N;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) atomicAdd(&gv[i], vol[i]);
 }
 
 // ------------------------------------------------------------ ray probes
@@ -1745,12 +1770,12 @@ hipError_t MPF_NAME(mpf_launch_phase_map)(SlotView<Real> S, int nx, int ny, int 
     return hipGetLastError();
 }
 
-hipError_t MPF_NAME(mpf_launch_observe)(SlotView<Real> S, int nx, int ny, int nz, int N, double *gvol,
-                                        double *part, int32_t *front, hipStream_t st)
+hipError_t MPF_NAME(mpf_launch_observe)(SlotView<Real> S, const Inst *inst, const long long *pre, int n,
+                                        long long blocks, int N, double *gvol, double *part, int32_t *front,
+                                        hipStream_t st)
 {
     size_t shm = ((size_t)N + 256 * 6) * sizeof(double);
-    hipLaunchKernelGGL(k_observe, dim3(nz), dim3(256), shm, st, S, nx, ny, nz, N, gvol, part);
-    hipLaunchKernelGGL(k_front_height, dim3(nblk((size_t)nx * ny, 256)), dim3(256), 0, st, S.liq, nx, ny, nz, front);
+    hipLaunchKernelGGL(k_observe, dim3((unsigned)blocks), dim3(256), shm, st, S, inst, pre, n, N, gvol, part, front);
     return hipGetLastError();
 }
 

@@ -75,23 +75,15 @@ class Stats(C.Structure):
 _lib = None
 
 
-def _torch_runtime_first():
-    """PyTorch-ROCm carries its own HIP runtime.  A process that initialises
-    the system HIP runtime (this library's) first cannot initialise torch's
-    afterwards (torch then reports no GPU), while the other order serves
-    both; so when torch is installed, load and initialise it first."""
-    try:
-        import torch
-    except ImportError:
-        return
-    try:
-        torch.cuda.is_available()
-    except Exception:
-        pass
+def loaded() -> bool:
+    """Whether libmpf.so (and with it a HIP runtime) is loaded in this process."""
+    return _lib is not None
 
 
 def lib():
-    """Load libmpf.so (raises when it has not been built)."""
+    """Load libmpf.so (raises when it has not been built).  The library
+    binds to whichever libamdhip64.so.7 the process loaded first: the
+    system's, or PyTorch-ROCm's when torch was imported before."""
     global _lib
     if _lib is not None:
         return _lib
@@ -99,7 +91,6 @@ def lib():
         raise RuntimeError(
             f"{LIB_PATH} is missing: build the HIP library first "
             "(python -c 'import __graft_entry__ as g; g.build()')")
-    _torch_runtime_first()
     L = C.CDLL(LIB_PATH)
     vp, ip, dp = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double)
     H = C.c_void_p

@@ -21,7 +21,8 @@ slot-equivariant.
 
 Everything runs in the HIP kernels of include/mpf_nn.h: bf16 MFMA
 convolutions with fp32 accumulation, their backward passes and Adam.  torch
-only provides device buffers.  The slot features are built on the device
+only provides device buffers, so importing this module imports torch; do it
+before the first solver of the process (see _torch).  The slot features are built on the device
 from the solver's dense phi (mpf_download_phi_device); a proposal is written
 back with mpf_upload_phi_device.
 
@@ -42,6 +43,11 @@ from . import _nnlib
 from ._lib import FP32, check
 from . import _lib
 
+try:
+    import torch as _torch_module      # before libmpf.so: the two must share torch's HIP runtime
+except ImportError:
+    _torch_module = None
+
 SCALE = 20.0        # targets are 20 x the 16-step increment
 HORIZON = 16
 K_SLOTS = 6
@@ -49,10 +55,20 @@ BASE = 32
 
 
 def _torch():
-    import torch
-    if not torch.cuda.is_available():
-        raise RuntimeError("the surrogate runs on the GPU (HIP) only")
-    return torch
+    """torch, checked to see the GPU.
+
+    PyTorch-ROCm carries its own libamdhip64 under the soname of the system
+    runtime that libmpf.so links against, and the dynamic linker binds every
+    later load to the copy loaded first.  With torch imported first, libmpf.so
+    shares torch's runtime and both see the GPU; with libmpf.so loaded first,
+    torch is bound to the system runtime and finds no GPU."""
+    if _torch_module is None:
+        raise RuntimeError("the surrogate needs PyTorch (ROCm) for its device buffers")
+    if not _torch_module.cuda.is_available():
+        hint = (" libmpf.so was loaded before torch in this process: import mpf.surrogate (or torch) before "
+                "creating the first solver" if _lib.loaded() else "")
+        raise RuntimeError("the surrogate runs on the GPU (HIP) only and torch sees no GPU." + hint)
+    return _torch_module
 
 
 def _stream():

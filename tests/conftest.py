@@ -11,3 +11,10 @@ for p in (ROOT, PKG, os.path.join(ROOT, "tests")):
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: runs HIP kernels on an MI355X (gfx950)")
     config.addinivalue_line("markers", "slow: takes more than a few seconds")
+    if "not gpu" not in (config.option.markexpr or ""):
+        # The surrogate's tests use torch on the GPU in the same process as
+        # the solver's: torch's HIP runtime has to be loaded first.
+        try:
+            import mpf.surrogate  # noqa: F401
+        except ImportError:
+            pass
