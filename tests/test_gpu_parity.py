@@ -183,7 +183,9 @@ def test_kmax_overflow_probe():
 def test_general_path_many_phases(N):
     """Cells with more active solids than the register fast path holds take
     the general path and still match the oracle bit for bit: up to 8 solids
-    in unrolled register arrays (N = 9), more in the scratch arrays (N = 13)."""
+    in unrolled register arrays, one lane per solid (N = 9: hundreds of cells
+    hold the liquid and all 8 solids active at every step, so the register
+    path runs at its full width), more in the scratch arrays (N = 13)."""
     rng = np.random.default_rng(5)
     nx, ny, nz = 8, 8, 8
     phi = np.zeros((N, nx, ny, nz))
@@ -208,6 +210,11 @@ def test_general_path_many_phases(N):
     if N == 13:
         assert s.stats()["max_active"] >= 10   # liquid + 9 or more solids somewhere
     assert np.array_equal(s.phi(), ref)
+    full = np.count_nonzero(O.apt(ref, 50)[1] == N)
+    print(f"N={N}: {full} cells with every phase active after 5 steps")
+    if N == 9:
+        assert full >= 100
+    assert np.array_equal(s.apt()[1], O.apt(ref, 50)[1])
     s.close()
 
 
