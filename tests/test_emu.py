@@ -9,15 +9,16 @@ from emu import emu
 from oracle import oracle as O
 
 
-def _run(phi, temp, tabs, scal, steps, mode=0, precision="fp64", aniso=True, torque=True, use_class=False):
+def _run(phi, temp, tabs, scal, steps, mode=0, precision="fp64", aniso=True, torque=True, use_class=False,
+         kmax=50):
     cur_ref = phi.copy()
     cur = phi.copy()
     used = np.zeros(4, int)
     for _ in range(steps):
-        mf, nf, _ = O.apt(cur_ref, 50)
+        mf, nf, _ = O.apt(cur_ref, kmax)
         ref = O.update(cur_ref, temp, mf, nf, tabs, scal, aniso=aniso, torque=torque)
         out, paths, _ = emu.step(cur if precision == "fp64" else cur_ref, temp, tabs, scal, mode=mode,
-                                 precision=precision, aniso=aniso, torque=torque, use_class=use_class)
+                                 precision=precision, aniso=aniso, torque=torque, use_class=use_class, kmax=kmax)
         used += paths
         yield ref, out, used
         cur_ref, cur = ref, out
@@ -68,6 +69,15 @@ def test_fp64_quad_junction_and_many_phase_cells():
     for ref, out, used in _run(phi, temp, tabs, scal, 3):
         assert np.array_equal(out, ref)
     assert used[3] > 0
+
+
+@pytest.mark.parametrize("kmax", [1, 2, 3])
+def test_fp64_kmax_clamp_bit_exact(kmax):
+    """KMAX below the active count: cells keep the liquid and the lowest
+    grain ids (the two-grain gather and the k-way merge alike)."""
+    c = cases.voronoi_case(nx=10, ny=9, nz=12, n_solid=7, seed=3, qseed=4)
+    for ref, out, used in _run(c["phi"], c["temp"], c["tables"], c["scalars"], 3, kmax=kmax):
+        assert np.array_equal(out, ref)
 
 
 @pytest.mark.parametrize("aniso,torque", [(True, False), (False, False)])
