@@ -78,7 +78,7 @@ def step(phi, T_cooled, tables, scalars, kmax=50, aniso=True, torque=True, preci
     nx, ny = grid[0], grid[1]
     nz = grid[2] if dim == 3 else 1
     liq, cnt, sid, sval = pack(phi)
-    scap = max(sid.shape[0], min(kmax, N - 1), 1)
+    scap = max(sid.shape[0], min(kmax, N - 1), 2)   # the device code reads slots 0 and 1 unconditionally
     sid = np.vstack([sid, np.zeros((scap - sid.shape[0], liq.size), np.uint16)])
     sval = np.vstack([sval, np.zeros((scap - sval.shape[0], liq.size), dt)])
     tab = np.concatenate([np.asarray(t, np.float64).ravel() for t in tables]).astype(dt)
