@@ -231,8 +231,10 @@ def test_c2_fp64_bit_exact_200_steps():
 
 
 def test_c2_fp32_vs_fp64_2000_steps():
-    """128^3, 16 grains, the full 2000 steps in both precisions on the device;
-    the FP32 phi stays within 1e-4 of the FP64 phi."""
+    """128^3, 16 grains, the full 2000 steps in both precisions on the device:
+    mean front within 0.05 cells, every column's front within one cell, grain
+    volume fractions within 1e-2 relative (the max-abs difference jumps at
+    active-set flips, see test_c3_window_fp32_against_fp64)."""
     c = _c2_case()
     ss = []
     try:
