@@ -149,10 +149,13 @@ MPF_DEV bool apt_active(const Real *vals)
 
 // Gather the active set of one cell (APT with the checked KMAX clamp).
 // Returns the total number of active phases (before the clamp) in *ntot and
-// whether the kept solids fit in A (*fits).
+// whether the kept solids fit in A (*fits).  no_liquid: the caller knows the
+// liquid is +0 at all stencil points (the classification pass saw it), so its
+// values are not read.
 template <typename Real, int DIM, int A>
 MPF_DEV void gather(const Consts<Real> &K, const SlotView<Real> &S, size_t ncell,
-                                       int l, int m, int k, Cell<Real, DIM, A> &C, int *ntot, bool *fits)
+                                       int l, int m, int k, Cell<Real, DIM, A> &C, int *ntot, bool *fits,
+                                       bool no_liquid = false)
 {
     constexpr int NP = Cell<Real, DIM, A>::NP;
     constexpr bool SMALL = (A <= 8);
@@ -160,8 +163,13 @@ MPF_DEV void gather(const Consts<Real> &K, const SlotView<Real> &S, size_t ncell
     stencil_cells<Real, DIM>(K, l, m, k, M.pc);
     // liquid (dense) and the counts, issued before either is used
     Real lv[NP];
+    if (no_liquid) {
 #pragma unroll
-    for (int p = 0; p < NP; ++p) lv[p] = S.liq[M.pc[p]];
+        for (int p = 0; p < NP; ++p) lv[p] = Real(0.0);
+    } else {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) lv[p] = S.liq[M.pc[p]];
+    }
     M.init(S, ncell);
     C.liq = apt_active<NP>(lv);
     C.gid[0] = 0;

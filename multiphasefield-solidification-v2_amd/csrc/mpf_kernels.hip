@@ -97,6 +97,10 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
+// x is +0.0 (all bits clear; -0.0 is not)
+__device__ __forceinline__ bool is_plus_zero(float x) { return __float_as_uint(x) == 0u; }
+__device__ __forceinline__ bool is_plus_zero(double x) { return __double_as_longlong(x) == 0ll; }
+
 __device__ __forceinline__ int *wl_counter(const StepArgs<Real> &a, int list, int step)
 {
     return a.wl_count + list * 2 + (step & 1);
@@ -533,7 +537,7 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
                     s_sv[b][hys][lx + 1] = hy.sv;
                 }
                 __syncthreads();
-                bool queue = false, dirty = false;
+                bool queue = false, dirty = false, no_liq = false;
                 int fb = -1; // interface bucket
                 if (valid) {
                     const Off c = own_xy + (Off)k * plane;
@@ -606,6 +610,11 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
                             fb = nd - 1;
                         else
                             queue = true;
+                        // liquid +0 (bitwise) at every stencil point: the
+                        // merge pass need not read it
+                        no_liq = true;
+#pragma unroll
+                        for (int p = 0; p < NP; ++p) no_liq = no_liq && is_plus_zero(pl[p]);
                     }
                 }
                 // output occupancy: an output cell can only hold a grain where
@@ -622,7 +631,7 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
                     int base = 0;
                     if (lane == leader) base = atomicAdd(&s_qn, __popcll(ball));
                     base = __shfl(base, leader, 64);
-                    if (queue) s_q[base + __popcll(ball & lt)] = qe;
+                    if (queue) s_q[base + __popcll(ball & lt)] = (uint16_t)(qe | (no_liq ? 0x8000u : 0u));
                 }
                 ball = __ballot(fb >= 0);
                 if (ball != 0ull) {
@@ -671,7 +680,9 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
             return (int)(cell_base + cidx(l0 + qx, m0 + qy, k0 + qz, K.nx, K.ny));
         };
         int *dst = wl_list(a, WL_MERGE) + s_qbase;
-        for (int i = threadIdx.x; i < nq; i += LTX * LTY) dst[i] = cell_of(s_q[i]);
+        // bit 31 of a merge entry: no liquid in the cell's stencil
+        for (int i = threadIdx.x; i < nq; i += LTX * LTY)
+            dst[i] = (int)((uint32_t)cell_of(s_q[i]) | ((uint32_t)(s_q[i] >> 15) << 31));
         // interface cells, compacted into their buckets: per chunk of the
         // list, each wave counts its entries per bucket, and an entry's place
         // is the bucket's base, the entries of the chunks before and of the
@@ -753,13 +764,15 @@ __global__ __launch_bounds__(256, 4) void k_merge(StepArgs<Real> a)
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x, par ^= 1) {
         int i = base + threadIdx.x;
         bool valid = i < n;
-        const size_t c = valid ? (size_t)wl_list(a, WL_MERGE)[i] : 0;
+        const int e = valid ? wl_list(a, WL_MERGE)[i] : 0;
+        const size_t c = (size_t)(e & 0x7fffffff);
+        const bool no_liq = e < 0;
         const Where W = locate<B>(a, s_coff, c);
         Cell<Real, DIM, FAST_A> C;
         int ntot = 0;
         bool fits = true, front = false;
         if (valid) {
-            gather<Real, DIM, FAST_A>(W.K, W.in, ncell, W.l, W.m, W.k, C, &ntot, &fits);
+            gather<Real, DIM, FAST_A>(W.K, W.in, ncell, W.l, W.m, W.k, C, &ntot, &fits, no_liq);
             front = fits && need_terms && C.liq && C.nsol > 0;
             // interface cells are counted by the interface pass
             if (!front) note_active(a.stats, ntot, K.kmax, a.step, a.halt_mode);
