@@ -452,6 +452,74 @@ def summarize_trace(path, window):
     return out
 
 
+STEP_KERNELS = ("k_tiles", "k_light", "k_merge", "k_interface", "k_front1", "k_front_multi", "k_step_general")
+CLOCK_GHZ, N_SIMD = 2.4, 1024
+
+
+def profile_summary(kind, window, trace, counters, copy_kib=262144):
+    """Per-step PMC summary of the step kernels (the last `window`
+    dispatches of each) from rocprofv3 CSVs, as committed under profiles/:
+    kind 'traffic': counters = (FETCH_SIZE csv, WRITE_SIZE csv), each scaled
+    to bytes by what it reads for the 256 MiB device copy of --copy-probe
+    (the largest __amd_rocclr_copyBuffer dispatch); kind 'valu': counters =
+    (csv with SQ_INSTS_VALU, SQ_INSTS_SALU, SQ_INSTS_VMEM_RD, SQ_INSTS_LDS,
+    SQ_WAVES).  Durations come from the kernel trace."""
+    import csv
+    dur = summarize_trace(trace, window)
+    # the kernels of the timed steps: those dispatched after the first of the
+    # last `window` classification passes (autotuner trials of other launch
+    # modes come before it)
+    rows = list(csv.DictReader(open(trace)))
+    light = sorted(int(r["Start_Timestamp"]) for r in rows if "k_light" in r["Kernel_Name"])
+    t0 = light[-window]
+    timed = {k for k in STEP_KERNELS
+             if any(k in r["Kernel_Name"] and int(r["Start_Timestamp"]) >= t0 for r in rows)}
+    out = {"window_dispatches": window, "kernels": {}}
+    if kind == "traffic":
+        scales, per = [], []
+        for path, name in zip(counters, ("FETCH_SIZE", "WRITE_SIZE")):
+            probe = {}
+            for r in csv.DictReader(open(path)):
+                if r["Counter_Name"] == name and "copyBuffer" in r["Kernel_Name"]:
+                    i = int(r["Dispatch_Id"])
+                    probe[i] = probe.get(i, 0.0) + float(r["Counter_Value"])
+            # the probe is the first device copy of the run
+            scales.append(copy_kib / probe[min(probe)])
+            per.append(summarize_trace(path, window))
+        out["fetch_scale"], out["write_scale"] = round(scales[0], 4), round(scales[1], 4)
+        tot = 0
+        for k in STEP_KERNELS:
+            if k not in timed or k not in per[0]:
+                continue
+            rd = per[0][k]["FETCH_SIZE"] * 1024 * out["fetch_scale"]
+            wr = per[1][k]["WRITE_SIZE"] * 1024 * out["write_scale"]
+            out["kernels"][k] = {"read_bytes": round(rd), "write_bytes": round(wr),
+                                 "duration_ms": round(dur[k]["avg_ms_last"], 4)}
+            tot += rd + wr
+        out["bytes_per_step"] = round(tot)
+        return out
+    c = summarize_trace(counters[0], window)
+    out["model"] = (f"a wavefront-64 VALU instruction occupies a 16-lane SIMD for 4 cycles; {N_SIMD} SIMDs "
+                    f"(256 CUs x 4) at {CLOCK_GHZ} GHz")
+    tv = tt = 0.0
+    for k in STEP_KERNELS:
+        if k not in timed or k not in c:
+            continue
+        v = c[k]["SQ_INSTS_VALU"]
+        t_us = v * 4 / (N_SIMD * CLOCK_GHZ * 1e3)
+        d_us = dur[k]["avg_ms_last"] * 1e3
+        out["kernels"][k] = {"valu_wave_instr": round(v), "salu_wave_instr": round(c[k]["SQ_INSTS_SALU"]),
+                             "vmem_rd_wave_instr": round(c[k]["SQ_INSTS_VMEM_RD"]),
+                             "lds_wave_instr": round(c[k]["SQ_INSTS_LDS"]), "waves": round(c[k]["SQ_WAVES"]),
+                             "valu_time_at_full_issue_us": round(t_us, 1), "duration_us": round(d_us, 1),
+                             "valu_issue_frac": round(t_us / d_us, 3)}
+        tv += v
+        tt += t_us
+    out["valu_wave_instr_per_step"] = round(tv)
+    out["valu_time_at_full_issue_us_per_step"] = round(tt, 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -486,12 +554,20 @@ def main():
     ap.add_argument("--pcie", type=int, default=1, help="the host-buffer (PCIe-inclusive) rate at 128^3 (0 skips it)")
     ap.add_argument("--summarize-trace", default=None,
                     help="print per-kernel averages over the last --steps dispatches of a rocprofv3 CSV and exit")
+    ap.add_argument("--profile-summary", nargs="+", default=None, metavar="ARG",
+                    help="traffic KERNEL_TRACE FETCH_CSV WRITE_CSV | valu KERNEL_TRACE PMC_CSV: print the "
+                         "per-step PMC summary of the step kernels over the last --steps dispatches and exit")
     args = ap.parse_args()
     if args.summarize_trace:
         print(json.dumps(summarize_trace(args.summarize_trace, args.steps), indent=1))
         return
-    if args.surrogate:
-        import mpf.surrogate  # noqa: F401 -- torch's HIP runtime has to be loaded before the solver's
+    if args.profile_summary:
+        kind, trace, *csvs = args.profile_summary
+        print(json.dumps(profile_summary(kind, args.steps, trace, csvs), indent=1))
+        return
+    # torch's HIP runtime is loaded before the solver's (the surrogate needs
+    # it, and every run then binds the same runtime, profiled or not)
+    import mpf.surrogate  # noqa: F401
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the solver has no "
               "multi-GPU path in this build.", file=sys.stderr)
