@@ -202,14 +202,22 @@ int mpf_batch_download_phi_device(mpf_handle *h, int grid, void *phi, int dtype)
  * with the one-solid interface kernel (default from 4096 classification
  * tiles, 512x512x128 cells, up), 2 all three as one launch (default below).
  * mpf_get_launch: out[4] = the three block counts in use and that mode.
+ * mpf_set_tiling: the classification pass's work items -- ranges of
+ * planes_per_item z-planes (4, 8 or 16) of a 64 x 8 column tile (0: by grid
+ * size, 16 from 4096 tiles up, 8 from 1024, else 4; 2D grids have one
+ * plane) -- and its persistent grid (light_blocks, 0: 1024).
+ * mpf_get_tiling: out[2] = the planes per item and blocks in use.
  * mpf_set_graphs: replay each step from a HIP graph captured on first use
  * (one per state parity); not with halt_on_kmax, and kernel timing bypasses
  * it.  mpf_autotune: times launch shapes on the current state (trial_steps
- * per shape, coordinate descent over that mode and the three block
- * counts), keeps the fastest and restores the state; out (optional, 5
- * ints): the chosen shape and its time per step in microseconds. */
+ * per shape, coordinate descent over that mode, the tiling and the three
+ * block counts), keeps the fastest and restores the state; out (optional, 5
+ * ints): the chosen mode and block counts and the time per step in
+ * microseconds (the chosen tiling: mpf_get_tiling). */
 int mpf_set_launch(mpf_handle *h, int merge_blocks, int front_blocks, int general_blocks, int side_stream);
 int mpf_get_launch(mpf_handle *h, int *out);
+int mpf_set_tiling(mpf_handle *h, int planes_per_item, int light_blocks);
+int mpf_get_tiling(mpf_handle *h, int *out);
 int mpf_set_graphs(mpf_handle *h, int on);
 int mpf_autotune(mpf_handle *h, int trial_steps, int *out);
 

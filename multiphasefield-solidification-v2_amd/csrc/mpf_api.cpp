@@ -112,6 +112,8 @@ struct mpf_handle {
     int *wl_count = nullptr;  // [NWC][2]
     int *tiles = nullptr;     // work items of the classification pass (n_tiles * LZC / 4 entries)
     uint16_t *item_busy = nullptr; // their tiles' plane masks
+    int light_q_req = 0;      // planes per classification work item (0: by grid size)
+    int light_grid_req = 0;   // blocks of the classification pass (0: 1024)
     Stats *stats = nullptr;
     int cur = 0, tcur = 0;
     int step_counter = 0;
@@ -233,6 +235,16 @@ static int reset_stats_dev(mpf_handle *h)
 // 512x512x256), one launch on small ones (the cross-stream fork and join
 // cost more than the passes there: 128^3 steps 125 -> 82 us).
 static int default_mode(const mpf_handle *h) { return h->n_tiles >= 4096 ? 1 : 2; }
+
+// Planes per classification work item: as set (mpf_set_tiling), else
+// shorter ranges on grids with few tiles, so that more blocks share the
+// tiles with work.  2D grids have one plane.
+static int light_planes(const mpf_handle *h)
+{
+    if (h->cfg.dim == 2) return LZC;
+    if (h->light_q_req) return h->light_q_req;
+    return h->n_tiles >= 4096 ? LZC : (h->n_tiles >= 1024 ? LZC / 2 : LZC / 4);
+}
 
 // Common creation of a single grid (batch = 0, one instance with the
 // configuration's grid) and of a batch of 3D grids.
@@ -666,9 +678,8 @@ static hipError_t launch_one_step(mpf_handle *h, const int *grids, hipEvent_t *t
     a.tiles = h->tiles;
     a.item_busy = h->item_busy;
     a.n_tiles = h->n_tiles;
-    // planes per classification work item: shorter ranges on grids with few
-    // tiles, so that more blocks share the tiles with work
-    a.light_q = (c.dim == 2 || h->n_tiles >= 4096) ? LZC : (h->n_tiles >= 1024 ? LZC / 2 : LZC / 4);
+    a.light_q = light_planes(h);
+    a.light_grid = h->light_grid_req;
     a.n_temp = h->nT;
     a.step = h->step_counter;
     a.halt_mode = c.halt_on_kmax;
@@ -941,6 +952,27 @@ extern "C" int mpf_set_launch(mpf_handle *h, int merge_blocks, int front_blocks,
     return 0;
 }
 
+extern "C" int mpf_set_tiling(mpf_handle *h, int planes_per_item, int light_blocks)
+{
+    if (!h) return fail(MPF_EINVAL, "mpf_set_tiling: null handle");
+    if (planes_per_item != 0 && planes_per_item != 4 && planes_per_item != 8 && planes_per_item != LZC)
+        return fail(MPF_EINVAL, "mpf_set_tiling: planes_per_item must be 0, 4, 8 or %d (got %d)", LZC,
+                    planes_per_item);
+    if (light_blocks < 0) return fail(MPF_EINVAL, "mpf_set_tiling: negative block count");
+    h->light_q_req = planes_per_item;
+    h->light_grid_req = light_blocks;
+    drop_graphs(h);
+    return 0;
+}
+
+extern "C" int mpf_get_tiling(mpf_handle *h, int *out)
+{
+    if (!h || !out) return fail(MPF_EINVAL, "mpf_get_tiling: null argument");
+    out[0] = light_planes(h);
+    out[1] = h->light_grid_req > 0 ? h->light_grid_req : 1024;
+    return 0;
+}
+
 extern "C" int mpf_get_launch(mpf_handle *h, int *out)
 {
     if (!h || !out) return fail(MPF_EINVAL, "mpf_get_launch: null argument");
@@ -1007,33 +1039,46 @@ extern "C" int mpf_autotune(mpf_handle *h, int trial_steps, int *out)
     const int timing0 = h->kernel_timing, graphs0 = h->use_graphs;
     h->kernel_timing = 0;
     h->use_graphs = 0;
-    // Coordinate descent from the default shape: the side stream, then the
-    // block counts of the merge, interface and many-phase kernels, each over
-    // its candidates with the others held at their best so far.
-    int cur[4];
+    // Coordinate descent from the default shape: the mode of the passes
+    // after the merge pass, the planes per classification work item (3D) and
+    // the classification grid, then the block counts of the merge, interface
+    // and many-phase kernels, each over its candidates with the others held
+    // at their best so far.  Shape: merge, interface and many-phase blocks,
+    // mode, planes per item, classification blocks.
+    constexpr int NS = 6;
+    int cur[NS];
     {
         int g[3];
         const int r0[3] = {h->grid_req[0], h->grid_req[1], h->grid_req[2]};
+        const int q0 = h->light_q_req;
         h->grid_req[0] = h->grid_req[1] = h->grid_req[2] = 0;
+        h->light_q_req = 0;
         step_grids(h, g);
+        cur[4] = light_planes(h);
         h->grid_req[0] = r0[0], h->grid_req[1] = r0[1], h->grid_req[2] = r0[2];
-        cur[0] = g[0], cur[1] = g[1], cur[2] = g[2], cur[3] = default_mode(h);
+        h->light_q_req = q0;
+        cur[0] = g[0], cur[1] = g[1], cur[2] = g[2], cur[3] = default_mode(h), cur[5] = 1024;
     }
+    const int q_req0 = h->light_q_req, lg_req0 = h->light_grid_req;
     static const int cand_m[] = {512, 1024, 2048, 4096, 8192}, cand_g[] = {64, 128, 256, 512};
-    const int *cands[4] = {cand_m, cand_m, cand_g, nullptr};
-    const int ncand[4] = {5, 5, 4, 3};
+    static const int cand_q[] = {4, 8, LZC}, cand_l[] = {512, 1024, 2048};
+    const int *cands[NS] = {cand_m, cand_m, cand_g, nullptr, cand_q, cand_l};
+    const int ncand[NS] = {5, 5, 4, 3, h->cfg.dim == 3 ? 3 : 0, 3};
     double best = 1e300;
-    int pick[4] = {cur[0], cur[1], cur[2], cur[3]};
+    int pick[NS];
+    for (int j = 0; j < NS; ++j) pick[j] = cur[j];
     int rc = 0;
     // the three fastest shapes seen, for the confirmation round
     constexpr int NTOP = 3;
-    int top[NTOP][4];
+    int top[NTOP][NS];
     double topt[NTOP] = {1e300, 1e300, 1e300};
     auto time_shape = [&](const int *shape, int steps, double &t) -> int {
         h->grid_req[0] = shape[0];
         h->grid_req[1] = shape[1];
         h->grid_req[2] = shape[2];
         h->use_side = shape[3];
+        h->light_q_req = shape[4];
+        h->light_grid_req = shape[5];
         h->cur = cur0;
         h->tcur = tcur0;
         h->step_counter = s0;
@@ -1048,26 +1093,27 @@ extern "C" int mpf_autotune(mpf_handle *h, int trial_steps, int *out)
         if (int r = time_shape(shape, trial_steps, t)) return r;
         if (t < best) {
             best = t;
-            for (int j = 0; j < 4; ++j) pick[j] = shape[j];
+            for (int j = 0; j < NS; ++j) pick[j] = shape[j];
         }
         int q = NTOP;
         while (q > 0 && t < topt[q - 1]) --q;
         if (q < NTOP) {
             for (int k = NTOP - 1; k > q; --k) {
                 topt[k] = topt[k - 1];
-                for (int j = 0; j < 4; ++j) top[k][j] = top[k - 1][j];
+                for (int j = 0; j < NS; ++j) top[k][j] = top[k - 1][j];
             }
             topt[q] = t;
-            for (int j = 0; j < 4; ++j) top[q][j] = shape[j];
+            for (int j = 0; j < NS; ++j) top[q][j] = shape[j];
         }
         return 0;
     };
     rc = trial(cur);
-    const int order[4] = {3, 0, 1, 2};
-    for (int oi = 0; oi < 4 && !rc; ++oi) {
+    const int order[NS] = {3, 4, 5, 0, 1, 2};
+    for (int oi = 0; oi < NS && !rc; ++oi) {
         const int dmn = order[oi];
         for (int v = 0; v < ncand[dmn] && !rc; ++v) {
-            int shape[4] = {pick[0], pick[1], pick[2], pick[3]};
+            int shape[NS];
+            for (int j = 0; j < NS; ++j) shape[j] = pick[j];
             const int val = dmn == 3 ? 2 - v : cands[dmn][v];
             if (val == pick[dmn]) continue;
             shape[dmn] = val;
@@ -1085,7 +1131,7 @@ extern "C" int mpf_autotune(mpf_handle *h, int trial_steps, int *out)
             rc = time_shape(top[q], 3 * trial_steps, t);
             if (!rc && t < bestc) {
                 bestc = t;
-                for (int j = 0; j < 4; ++j) pick[j] = top[q][j];
+                for (int j = 0; j < NS; ++j) pick[j] = top[q][j];
             }
         }
         if (!rc) best = bestc;
@@ -1098,8 +1144,12 @@ extern "C" int mpf_autotune(mpf_handle *h, int trial_steps, int *out)
     hipFree(sn.mem);
     h->kernel_timing = timing0;
     h->use_graphs = graphs0;
+    h->light_q_req = q_req0;
+    h->light_grid_req = lg_req0;
     if (rc < 0) return rc;
     if (rc2) return rc2;
+    h->light_q_req = h->cfg.dim == 3 ? pick[4] : 0;
+    h->light_grid_req = pick[5];
     h->grid_req[0] = pick[0];
     h->grid_req[1] = pick[1];
     h->grid_req[2] = pick[2];

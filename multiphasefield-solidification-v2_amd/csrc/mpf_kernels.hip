@@ -1658,9 +1658,9 @@ hipError_t MPF_NAME(mpf_launch_step)(const StepArgs<Real> *args, int dim, const 
                                      const StepStreams *ss, hipEvent_t *tev)
 {
     StepArgs<Real> a = *args;
-    // persistent classification grid: four blocks per CU of the 256, at most
-    // one per work item
-    const unsigned grid = (unsigned)min(a.n_tiles * (LZC / a.light_q), 1024);
+    // persistent classification grid (by default 1024 blocks, four per CU of
+    // the 256), at most one per work item
+    const unsigned grid = (unsigned)min(a.n_tiles * (LZC / a.light_q), a.light_grid > 0 ? a.light_grid : 1024);
     if (a.n_inst > 0) // a batch of 3D grids: one launch per kernel covers every instance
         return MPF_NAME(mpf_step_batch)(a, grid, grids, st, ss, tev);
     a.ntx = (a.K.nx + LTX - 1) / LTX;

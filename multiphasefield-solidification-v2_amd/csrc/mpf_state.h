@@ -113,6 +113,7 @@ struct StepArgs {
     int *tiles;      // work items of k_light (k_tiles -> k_light), n_tiles * LZC / 4 entries
     uint16_t *item_busy; // planes to compute of each item's tile
     int light_q;     // planes per work item: 16, 8 or 4
+    int light_grid;  // blocks of the persistent classification pass (0: the default, 1024)
     size_t n_temp;   // temperature entries of all instances (planes, or cells when temp_dense)
     int step;        // step index (for the ping-pong counters and halt)
     int halt_mode;   // stop at the first KMAX overflow

@@ -30,7 +30,7 @@ EXPORTS = (
     "mpf_batch_create", "mpf_batch_size", "mpf_batch_set_tables", "mpf_batch_upload_phi",
     "mpf_batch_init_columns", "mpf_batch_upload_temp", "mpf_batch_download_temp", "mpf_batch_download_phi",
     "mpf_batch_download_apt", "mpf_batch_observables",
-    "mpf_set_launch", "mpf_get_launch", "mpf_set_graphs", "mpf_autotune",
+    "mpf_set_launch", "mpf_get_launch", "mpf_set_graphs", "mpf_autotune", "mpf_set_tiling", "mpf_get_tiling",
     "mpf_batch_observables_all", "mpf_probe_rays", "mpf_phase_map", "mpf_batch_phase_map",
     "mpf_upload_phi_device", "mpf_download_phi_device", "mpf_batch_download_phi_device",
 )
@@ -142,6 +142,8 @@ def lib():
         "mpf_set_launch": (C.c_int, [H, C.c_int, C.c_int, C.c_int, C.c_int]),
         "mpf_get_launch": (C.c_int, [H, ip]),
         "mpf_set_graphs": (C.c_int, [H, C.c_int]),
+        "mpf_set_tiling": (C.c_int, [H, C.c_int, C.c_int]),
+        "mpf_get_tiling": (C.c_int, [H, ip]),
         "mpf_autotune": (C.c_int, [H, C.c_int, ip]),
         "mpf_batch_observables_all": (C.c_int, [H, dp, ip, dp]),
         "mpf_probe_rays": (C.c_int, [H, C.c_int, ip, ip, dp, dp, C.c_double, C.c_double, dp]),

@@ -187,6 +187,10 @@ class Batch:
     def set_graphs(self, on: bool = True):
         check(_lib.lib().mpf_set_graphs(self._h, int(bool(on))))
 
+    def set_tiling(self, planes_per_item: int = 0, light_blocks: int = 0):
+        from .solver import set_tiling
+        set_tiling(self._h, planes_per_item, light_blocks)
+
     def autotune(self, trial_steps: int = 4) -> dict:
         from .solver import autotune
         return autotune(self._h, trial_steps)

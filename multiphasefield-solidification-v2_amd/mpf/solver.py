@@ -176,6 +176,11 @@ class Solver:
     def set_graphs(self, on: bool = True):
         check(_lib.lib().mpf_set_graphs(self._h, int(bool(on))))
 
+    def set_tiling(self, planes_per_item: int = 0, light_blocks: int = 0):
+        """Classification work items of planes_per_item z-planes (4, 8, 16;
+        0: by grid size) and the classification grid (0: 1024 blocks)."""
+        set_tiling(self._h, planes_per_item, light_blocks)
+
     def autotune(self, trial_steps: int = 4) -> dict:
         return autotune(self._h, trial_steps)
 
@@ -222,15 +227,22 @@ def set_launch(h, merge_blocks, front_blocks, general_blocks, side_stream):
 def get_launch(h) -> dict:
     out = np.zeros(4, np.int32)
     check(_lib.lib().mpf_get_launch(h, ptr(out, C.c_int32)))
+    til = np.zeros(2, np.int32)
+    check(_lib.lib().mpf_get_tiling(h, ptr(til, C.c_int32)))
     return {"merge_blocks": int(out[0]), "front_blocks": int(out[1]), "general_blocks": int(out[2]),
-            "side_stream": int(out[3])}
+            "side_stream": int(out[3]), "light_planes": int(til[0]), "light_blocks": int(til[1])}
+
+
+def set_tiling(h, planes_per_item, light_blocks):
+    check(_lib.lib().mpf_set_tiling(h, int(planes_per_item), int(light_blocks)))
 
 
 def autotune(h, trial_steps) -> dict:
     out = np.zeros(5, np.int32)
     check(_lib.lib().mpf_autotune(h, int(trial_steps), ptr(out, C.c_int32)))
-    return {"merge_blocks": int(out[0]), "front_blocks": int(out[1]), "general_blocks": int(out[2]),
-            "side_stream": int(out[3]), "us_per_step": int(out[4])}
+    r = get_launch(h)
+    r["us_per_step"] = int(out[4])
+    return r
 
 
 def stats_dict(s) -> dict:
