@@ -256,7 +256,10 @@ int mpf_dense_apt(const void *phi, int dtype, int32_t *mf, int32_t *nf, int32_t 
 
 /* kernel_update_phasefield_active_3d(_switchable) (src/gpu_kernels_3d.py:530-830)
  * and kernel_update_phasefield_active (src/gpu_kernels.py:361-515).  phi,
- * phi_new and the tables share dtype; temp has temp_dtype; mf has depth rows. */
+ * phi_new and the tables share dtype; temp has temp_dtype; mf has depth rows.
+ * The tables are gathered into a block allocated and freed in stream order
+ * (hipMallocAsync/hipFreeAsync); the call returns without waiting for the
+ * kernel, as the reference's launch does. */
 int mpf_dense_update(const void *phi, void *phi_new, int dtype, const void *temp, int temp_dtype,
                      const int32_t *mf, const int32_t *nf, const void *wij, const void *aij,
                      const void *mij, const void *n111, const mpf_scalars *s, int n_phase, int nx, int ny,

@@ -1476,12 +1476,19 @@ extern "C" int mpf_dense_update(const void *phi, void *phi_new, int dtype, const
         return rc;
     size_t esz = dtype == MPF_FP64 ? 8 : 4, nn = (size_t)N * N;
     size_t tb = (3 * nn + (size_t)N * 24) * esz;
+    // the gathered tables live in a block allocated and freed in the
+    // caller's stream order: no device-wide serialisation and no wait, so
+    // the call is as asynchronous as the reference's kernel launch
+    hipStream_t st = as_stream(stream);
     void *tab = nullptr;
-    HIP_TRY(hipMalloc(&tab, tb));
+    hipError_t e = hipMallocAsync(&tab, tb, st);
+    if (e != hipSuccess) return fail(MPF_ENOMEM, "mpf_dense_update: hipMallocAsync(%zu): %s", tb, hipGetErrorString(e));
     int rc = dense_update_into("mpf_dense_update", phi, phi_new, dtype, temp, temp_dtype, mf, nf, wij, aij, mij,
-                               n111, s, N, nx, ny, nz, dim, depth, kmax, aniso, torque, stream, tab, true);
-    hipFree(tab);
-    return rc;
+                               n111, s, N, nx, ny, nz, dim, depth, kmax, aniso, torque, stream, tab, false);
+    e = hipFreeAsync(tab, st);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(MPF_EHIP, "mpf_dense_update: hipFreeAsync: %s", hipGetErrorString(e));
+    return 0;
 }
 
 // ---- the update without allocation ------------------------------------------
