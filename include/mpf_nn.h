@@ -88,6 +88,9 @@ int mpf_nn_band_residual(const float *a, const float *b, const float *c, int n_p
                          int z0, int zb, float inv_h, unsigned *out, void *stream);
 int mpf_nn_to_bf16(const float *x, void *y, size_t n, void *stream);
 int mpf_nn_fill(float *x, float v, size_t n, void *stream);
+/* *out = sum of x[0, n) (float atomics per block: exact for integer-valued
+ * sums below 2^24, e.g. the count of a 0/1 mask). */
+int mpf_nn_sum(const float *x, size_t n, float *out, void *stream);
 
 #ifdef __cplusplus
 }

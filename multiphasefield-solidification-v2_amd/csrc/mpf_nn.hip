@@ -854,6 +854,23 @@ __global__ void k_fill_f32(float *x, float v, size_t n)
 
 inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
+// Sum of x[0, n) added to *out (zeroed by the caller): grid-stride partial
+// sums, a wave reduction, one atomic per block.  The order of the atomics is
+// free, so the result is exact (and the same on every run) for sums of
+// integer values below 2^24, such as the counts of a 0/1 mask.
+__global__ __launch_bounds__(256) void k_sum_f32(const float *x, size_t n, float *out)
+{
+    float s = 0.f;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        s += x[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    __shared__ float ws[4];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, (ws[0] + ws[1]) + (ws[2] + ws[3]));
+}
+
 // The 8-channel kernels apply: every pointer 16-byte aligned, every channel
 // count and offset a multiple of 8, and fewer than 2^31 vectors.
 inline bool vec8(std::initializer_list<const void *> ptrs, std::initializer_list<long long> counts, size_t nvec)
@@ -1184,6 +1201,15 @@ int mpf_nn_fill(float *x, float v, size_t n, void *stream)
     const hipStream_t st = (hipStream_t)stream;
     if (!x) return -1;
     if (n) hipLaunchKernelGGL(k_fill_f32, dim3(nblk(n, 256)), dim3(256), 0, st, x, v, n);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int mpf_nn_sum(const float *x, size_t n, float *out, void *stream)
+{
+    const hipStream_t st = (hipStream_t)stream;
+    if (!x || !out) return -1;
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, st, out, 0.f, (size_t)1);
+    if (n) hipLaunchKernelGGL(k_sum_f32, dim3(nblk(n, 256) < 1024 ? nblk(n, 256) : 1024), dim3(256), 0, st, x, n, out);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

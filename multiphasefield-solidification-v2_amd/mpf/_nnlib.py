@@ -12,7 +12,7 @@ NN_EXPORTS = (
     "mpf_nn_conv_fwd", "mpf_nn_conv_bwd_w", "mpf_nn_weights_bf16", "mpf_nn_relu_bwd", "mpf_nn_bias_grad",
     "mpf_nn_pool", "mpf_nn_upsample", "mpf_nn_copy_channels", "mpf_nn_adam", "mpf_nn_slot_features",
     "mpf_nn_slot_pool", "mpf_nn_head_in", "mpf_nn_slot_targets", "mpf_nn_mse", "mpf_nn_slot_apply",
-    "mpf_nn_band_maxdiff", "mpf_nn_band_residual", "mpf_nn_to_bf16", "mpf_nn_fill",
+    "mpf_nn_band_maxdiff", "mpf_nn_band_residual", "mpf_nn_to_bf16", "mpf_nn_fill", "mpf_nn_sum",
 )
 
 _bound = None
@@ -45,6 +45,7 @@ def lib():
         "mpf_nn_band_residual": [vp, vp, vp, i, i, i, i, i, i, f, vp, vp],
         "mpf_nn_to_bf16": [vp, vp, sz, vp],
         "mpf_nn_fill": [vp, f, sz, vp],
+        "mpf_nn_sum": [vp, sz, vp, vp],
     }
     for name in NN_EXPORTS:
         fn = getattr(L, name)

@@ -79,6 +79,21 @@ def _p(t):
     return t.data_ptr()
 
 
+def _zeros(*shape, dtype=None):
+    """A device buffer cleared by the project's fill kernel (4-byte elements;
+    +0.0f has the bits of integer 0)."""
+    torch = _torch()
+    t = torch.empty(shape, dtype=dtype or torch.float32, device="cuda")
+    assert t.element_size() == 4
+    _nnlib.call("mpf_nn_fill", _p(t), 0.0, t.numel(), _stream())
+    return t
+
+
+def _zero(t):
+    assert t.element_size() == 4 and t.is_contiguous()
+    _nnlib.call("mpf_nn_fill", _p(t), 0.0, t.numel(), _stream())
+
+
 class Conv:
     """One convolution layer: fp32 master weights [ntap][cout][cin] (He
     initialisation), bias, Adam moments, bf16 copies for the forward and
@@ -91,10 +106,10 @@ class Conv:
         w = rng.standard_normal((ntap, cout, cin)).astype(np.float32) * std
         dev = "cuda"
         self.w = torch.tensor(w, device=dev)
-        self.b = torch.zeros(cout, device=dev)
-        self.gw, self.gb = torch.zeros_like(self.w), torch.zeros_like(self.b)
-        self.mw, self.vw = torch.zeros_like(self.w), torch.zeros_like(self.w)
-        self.mb, self.vb = torch.zeros_like(self.b), torch.zeros_like(self.b)
+        self.b = _zeros(cout)
+        self.gw, self.gb = _zeros(*self.w.shape), _zeros(cout)
+        self.mw, self.vw = _zeros(*self.w.shape), _zeros(*self.w.shape)
+        self.mb, self.vb = _zeros(cout), _zeros(cout)
         self.wb = torch.empty(self.w.shape, dtype=torch.bfloat16, device=dev)
         self.wf = torch.empty((ntap, cin, cout), dtype=torch.bfloat16, device=dev)
         self.refresh()
@@ -288,7 +303,7 @@ class SlotModel:
         sid = torch.empty((self.k, nv), dtype=torch.int32, device="cuda")
         feat = torch.empty((self.k, nv, 32), dtype=torch.bfloat16, device="cuda")
         aux = torch.empty((nv, 32), dtype=torch.bfloat16, device="cuda")
-        over = torch.zeros(1, dtype=torch.int32, device="cuda")
+        over = _zeros(1, dtype=torch.int32)
         _nnlib.call("mpf_nn_slot_features", _p(phi), n_phase, nx, ny, nz, z0, zb, self.k, _p(orient), _p(tfeat),
                     _p(sid), _p(feat), _p(aux), _p(over), _stream())
         return sid, feat, aux, over
@@ -335,8 +350,8 @@ class SlotModel:
 
     def zero_grad(self):
         for c in self.layers():
-            c.gw.zero_()
-            c.gb.zero_()
+            _zero(c.gw)
+            _zero(c.gb)
 
     def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8):
         self.t += 1
@@ -384,8 +399,10 @@ def sample_loss(model: SlotModel, smp: Sample, backward=True):
     nx, ny, nz = smp.shape
     _nnlib.call("mpf_nn_slot_targets", _p(smp.phi0), _p(smp.phi1), nx, ny, nz, smp.z0, smp.zb, model.k, _p(sid),
                 SCALE, _p(tgt), _p(mask), s)
-    cnt = float(mask.sum().item())
-    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    msum = torch.empty(1, dtype=torch.float32, device="cuda")
+    _nnlib.call("mpf_nn_sum", _p(mask), n, _p(msum), s)
+    cnt = float(msum.item())
+    loss = _zeros(1)
     dout = torch.empty((n, 32), dtype=torch.bfloat16, device="cuda")
     _nnlib.call("mpf_nn_mse", _p(out), 32, _p(tgt), _p(mask), n, 1.0 / max(cnt, 1.0), _p(dout), _p(loss), s)
     if backward:
@@ -518,7 +535,7 @@ class SurrogateStepper:
 
     def _reduce(self, name, *args):
         torch = _torch()
-        out = torch.zeros(1, dtype=torch.int32, device="cuda")
+        out = _zeros(1, dtype=torch.int32)
         _nnlib.call(name, *args, _p(out), _stream())
         return float(np.frombuffer(np.int32(out.item()).tobytes(), np.float32)[0])
 

@@ -181,6 +181,31 @@ def test_band_reductions():
     assert abs(got - ref) <= 1e-6 * abs(ref)
 
 
+@pytest.mark.parametrize("n", [1, 1000, 3 * 128 * 128 * 32 + 17])
+def test_fill_and_sum_kernels(n):
+    """The fill and sum kernels that replace torch's fills and reductions in
+    the training loop: a 0/1 mask's count exactly (integer sums below 2^24),
+    a float sum within rounding, on a buffer the fill kernel set first;
+    the sum overwrites whatever the output held."""
+    torch = _torch()
+    from mpf import _nnlib
+    from mpf.surrogate import _p, _stream, _zeros
+    rng = np.random.default_rng(n)
+    mask = (rng.uniform(size=n) < 0.3).astype(np.float32)
+    md = _dev(mask, torch.float32)
+    out = _dev(np.array([123.0], np.float32), torch.float32)
+    _nnlib.call("mpf_nn_sum", _p(md), n, _p(out), _stream())
+    assert out.item() == mask.sum()
+    z = _zeros(n)
+    assert torch.count_nonzero(z).item() == 0
+    _nnlib.call("mpf_nn_fill", _p(z), 0.5, n, _stream())
+    _nnlib.call("mpf_nn_sum", _p(z), n, _p(out), _stream())
+    assert out.item() == 0.5 * n
+    x = rng.standard_normal(n).astype(np.float32)
+    _nnlib.call("mpf_nn_sum", _p(_dev(x, torch.float32)), n, _p(out), _stream())
+    assert abs(out.item() - x.astype(np.float64).sum()) <= 1e-5 * np.abs(x).sum() + 1e-6
+
+
 def test_slot_features_match_oracle():
     torch = _torch()
     import cases
