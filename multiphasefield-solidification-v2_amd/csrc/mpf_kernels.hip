@@ -369,8 +369,11 @@ __device__ __forceinline__ void finish_cell(ColCell &r)
     r.sid = e ? (uint16_t)NO_ID : r.sid;
     r.sv = e ? Real(0.0) : r.sv;
 }
+// FP64: held to 128 VGPRs (four waves per SIMD, two blocks per CU; a few
+// spilled): 0.51 -> 0.33 ms at the mid-run state of 512x512x256.  The FP32
+// build needs 108 anyway.
 template <int DIM, typename Off, int B>
-__global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
+__global__ __launch_bounds__(LTX * LTY) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? 4 : 1))) void k_light(StepArgs<Real> a0)
 {
     if (a0.halt_mode && a0.stats->halt_step < a0.step) return;
     if (blockIdx.x == 0 && threadIdx.x < NWC) a0.wl_count[threadIdx.x * 2 + ((a0.step + 1) & 1)] = 0;
@@ -437,28 +440,32 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
         // pass starts rewriting the output occupancy)
         const uint32_t busy = a0.item_busy[li];
 
-        // halo cells this thread loads each plane: x halo of its row, y halo
-        // of its column (x periodic; y periodic in 3D, mirror in 2D)
-        int hxl = -1, hxs = 0;
-        if (ly < h && lx < 2) {
-            hxs = (lx == 0) ? 0 : w + 1;
-            hxl = (lx == 0) ? bc_pm(l0, K.nx) : bc_pp(l0 + w - 1, K.nx);
-        }
-        int hym = -1, hys = 0;
-        if (lx < w && ly < 2) {
-            hys = (ly == 0) ? 0 : h + 1;
+        // the one halo cell this thread loads each plane (x periodic; y
+        // periodic in 3D, mirror in 2D): wave 0 the row below the tile, wave
+        // 1 the row above, lanes 0-7 / 8-15 of wave 2 the cells left / right
+        // of the tile's rows; hrow, hcol: its place in the LDS tile (hrow < 0:
+        // none).  All lanes load (the others their own cell), so every wave
+        // issues the same loads.
+        int hrow = -1, hcol = 0, hm = 0, hl = 0;
+        if (ly < 2 && lx < w) {
+            hrow = (ly == 0) ? 0 : h + 1;
+            hcol = lx + 1;
+            hl = l;
             if (DIM == 3)
-                hym = (ly == 0) ? bc_pm(m0, K.ny) : bc_pp(m0 + h - 1, K.ny);
+                hm = (ly == 0) ? bc_pm(m0, K.ny) : bc_pp(m0 + h - 1, K.ny);
             else
-                hym = (ly == 0) ? bc_mm(m0, K.ny) : bc_mp(m0 + h - 1, K.ny);
+                hm = (ly == 0) ? bc_mm(m0, K.ny) : bc_mp(m0 + h - 1, K.ny);
+        } else if (ly == 2 && lx < 16 && (lx & 7) < h) {
+            hrow = (lx & 7) + 1;
+            hcol = (lx < 8) ? 0 : w + 1;
+            hm = m0 + (lx & 7);
+            hl = (lx < 8) ? bc_pm(l0, K.nx) : bc_pp(l0 + w - 1, K.nx);
         }
-        const uint32_t occx = hxl >= 0 ? a.occ_in[tile_of(hxl / LTX, ty, tz)] : 0u;
-        const uint32_t occy = hym >= 0 ? a.occ_in[tile_of(tx, hym / LTY, tz)] : 0u;
-        // in-plane cell offsets: own (clamped into the grid), halo x, halo y
-        // (lanes without a halo role use their own cell)
+        const uint32_t occh = hrow >= 0 ? a.occ_in[tile_of(hl / LTX, hm / LTY, tz)] : 0u;
+        // in-plane cell offsets: own (clamped into the grid) and halo (lanes
+        // without a halo role use their own cell)
         const Off own_xy = (Off)min(m, K.ny - 1) * (Off)K.nx + (Off)min(l, K.nx - 1);
-        const Off hx_xy = hxl >= 0 ? (Off)m * (Off)K.nx + (Off)hxl : own_xy;
-        const Off hy_xy = hym >= 0 ? (Off)hym * (Off)K.nx + (Off)l : own_xy;
+        const Off h_xy = hrow >= 0 ? (Off)hm * (Off)K.nx + (Off)hl : own_xy;
         const Off dum =
             (Off)(threadIdx.x & 63) < plane * (Off)(DIM == 3 ? K.nz : 1) ? (Off)(threadIdx.x & 63) : (Off)0;
 
@@ -468,29 +475,22 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
             pristine_unless(r, occ);
             return r;
         };
-        auto halo_x = [&](int kk) {
-            const bool occ = (occx >> (kk - k0)) & 1u;
-            ColCell r = load_cell<Off>(a.in, occ ? hx_xy + (Off)kk * plane : dum);
-            pristine_unless(r, occ);
-            return r;
-        };
-        auto halo_y = [&](int kk) {
-            const bool occ = (occy >> (kk - k0)) & 1u;
-            ColCell r = load_cell<Off>(a.in, occ ? hy_xy + (Off)kk * plane : dum);
+        auto halo = [&](int kk) {
+            const bool occ = (occh >> (kk - k0)) & 1u;
+            ColCell r = load_cell<Off>(a.in, occ ? h_xy + (Off)kk * plane : dum);
             pristine_unless(r, occ);
             return r;
         };
 
-        // ring[r & 3] holds plane ks + r - 1 of the own column; hr/yr[r & 1]
-        // hold the halo of plane ks + r
-        ColCell ring[4], hr[2], yr[2];
+        // ring[r & 3] holds plane ks + r - 1 of the own column; hv[r & 1]
+        // the halo cell of plane ks + r
+        ColCell ring[4], hv[2];
         ring[0] = (DIM == 3) ? own_cell(bc_mm(ks, K.nz)) : ColCell{};
         ring[1] = own_cell(ks);
         ring[2] = (DIM == 3) ? own_cell(bc_mp(ks, K.nz)) : ColCell{};
         ring[3] = ColCell{};
-        hr[0] = halo_x(ks);
-        yr[0] = halo_y(ks);
-        hr[1] = yr[1] = ColCell{};
+        hv[0] = halo(ks);
+        hv[1] = ColCell{};
         finish_cell(ring[0]);
         finish_cell(ring[1]);
         uint32_t occ_bits = 0;
@@ -502,17 +502,15 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
                 const int k = kb + j;
                 if (k >= ke) break;
                 ColCell &cp = ring[j & 3], &cc = ring[(j + 1) & 3], &cn = ring[(j + 2) & 3];
-                ColCell &hx = hr[j & 1], &hy = yr[j & 1];
+                ColCell &hx = hv[j & 1];
                 // loads of the next plane (the last plane reloads itself; unused)
                 if (DIM == 3) {
                     const int kn = min(k + 1, ke - 1);
                     ring[(j + 3) & 3] = own_cell(bc_mp(kn, K.nz));
-                    hr[(j + 1) & 1] = halo_x(kn);
-                    yr[(j + 1) & 1] = halo_y(kn);
+                    hv[(j + 1) & 1] = halo(kn);
                 }
                 finish_cell(cn);
                 finish_cell(hx);
-                finish_cell(hy);
                 if (!((busy >> (k - k0)) & 1u)) {
                     // plane left as it is (block-uniform branch); its liquid
                     // phase was active
@@ -526,15 +524,10 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
                     s_liq[b][ly + 1][lx + 1] = cc.liq;
                     s_sv[b][ly + 1][lx + 1] = cc.sv;
                 }
-                if (hxl >= 0) {
-                    s_key[b][ly + 1][hxs] = (uint32_t)hx.cnt | ((uint32_t)hx.sid << 8);
-                    s_liq[b][ly + 1][hxs] = hx.liq;
-                    s_sv[b][ly + 1][hxs] = hx.sv;
-                }
-                if (hym >= 0) {
-                    s_key[b][hys][lx + 1] = (uint32_t)hy.cnt | ((uint32_t)hy.sid << 8);
-                    s_liq[b][hys][lx + 1] = hy.liq;
-                    s_sv[b][hys][lx + 1] = hy.sv;
+                if (hrow >= 0) {
+                    s_key[b][hrow][hcol] = (uint32_t)hx.cnt | ((uint32_t)hx.sid << 8);
+                    s_liq[b][hrow][hcol] = hx.liq;
+                    s_sv[b][hrow][hcol] = hx.sv;
                 }
                 __syncthreads();
                 bool queue = false, dirty = false, no_liq = false;
@@ -740,8 +733,11 @@ __global__ __launch_bounds__(LTX * LTY) void k_light(StepArgs<Real> a0)
 // Four waves per SIMD (at most 128 VGPRs, a few spilled): the pass waits on
 // dependent loads of the slot lists, and the extra wave per SIMD measured
 // 3.5% faster than the three the unbounded build allows.
+// FP64: two waves per SIMD (202 VGPRs, none spilled); bounded to four, as
+// in FP32, it spilled 82 VGPRs and took 1.05 instead of 0.70 ms at the
+// mid-run state of 512x512x256.
 template <int DIM, int TC, int B>
-__global__ __launch_bounds__(256, 4) void k_merge(StepArgs<Real> a)
+__global__ __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 4) void k_merge(StepArgs<Real> a)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
     __shared__ uint32_t s_coff[B ? MAX_INST : 1];
@@ -825,6 +821,10 @@ __global__ __launch_bounds__(256, 4) void k_merge(StepArgs<Real> a)
     }
 }
 
+// FP64 interface kernels: held to two waves per SIMD (unbounded they take
+// all 256 VGPRs and run one): 0.545 -> 0.409 ms at the mid-run state of
+// 512x512x256; three or four waves spill far more.
+constexpr int FRONT_WPE = sizeof(Real) == 8 ? 2 : 1;
 // Pass 3: solid-liquid interface cells by bucket: the cells of a bucket of
 // NS solids have an active liquid and exactly NS active solids (both the
 // classification and the merge pass bucket by that count), so their update is
@@ -860,7 +860,7 @@ __device__ __forceinline__ void front_bucket(const StepArgs<Real> &a, const uint
 }
 
 template <int DIM, int TC, int B>
-__global__ __launch_bounds__(128) void k_front1(StepArgs<Real> a)
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(FRONT_WPE))) void k_front1(StepArgs<Real> a)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
     __shared__ uint32_t s_coff[B ? MAX_INST : 1];
@@ -920,7 +920,7 @@ __device__ __forceinline__ void front_multi(const StepArgs<Real> &a, const uint3
 }
 
 template <int DIM, int TC, int B>
-__global__ __launch_bounds__(128) void k_front_multi(StepArgs<Real> a)
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(FRONT_WPE))) void k_front_multi(StepArgs<Real> a)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
     __shared__ uint32_t s_coff[B ? MAX_INST : 1];
@@ -984,7 +984,7 @@ __global__ __launch_bounds__(64) void k_step_general(StepArgs<Real> a)
 // the first blocks, which start first; one launch needs no cross-stream
 // events, whose fork and join cost about 15 us each on this GPU.
 template <int DIM, int TC, int B>
-__global__ __launch_bounds__(128) void k_interface(StepArgs<Real> a, int gg, int gm)
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(FRONT_WPE))) void k_interface(StepArgs<Real> a, int gg, int gm)
 {
     if (a.halt_mode && a.stats->halt_step < a.step) return;
     __shared__ uint32_t s_coff[B ? MAX_INST : 1];
