@@ -77,6 +77,19 @@ def test_tilings_bit_exact(tiling):
         assert np.array_equal(s.phi(), ref)
 
 
+def test_tiling_rejects_bad_arguments():
+    """Planes per work item other than 0, 4, 8 or 16 and negative block
+    counts are refused before any launch; the tiling in use is unchanged."""
+    from mpf._lib import MPFError
+    c = cases.voronoi_case()
+    with _solver(c) as s:
+        before = s.launch()
+        for bad in ((5, 0), (32, 0), (4, -1)):
+            with pytest.raises(MPFError):
+                s.set_tiling(*bad)
+        assert s.launch() == before
+
+
 @pytest.mark.parametrize("which", ["voronoi", "many_phase"])
 def test_graph_replay_matches_eager(which):
     """Steps replayed from captured graphs, in several calls with downloads,
