@@ -206,6 +206,26 @@ def test_fill_and_sum_kernels(n):
     assert abs(out.item() - x.astype(np.float64).sum()) <= 1e-5 * np.abs(x).sum() + 1e-6
 
 
+@pytest.mark.parametrize("nvox,c", [(1, 8), (37, 3), (1000, 32), (4097, 96), (70001, 64), (2051, 512),
+                                    (3 * 64 * 64 * 16 + 5, 256)])
+def test_bias_gradient_kernel(nvox, c):
+    """db += column sums of a bf16 [nvox, c] gradient: the 16-byte kernel
+    (c a multiple of 8: one channel group per thread, several rows per
+    block, also more than 256 channels) and the one-channel-per-thread
+    kernel (c = 3), against FP64 sums; db accumulates onto what it held."""
+    torch = _torch()
+    from mpf import _nnlib
+    from mpf.surrogate import _p, _stream
+    rng = np.random.default_rng(nvox + c)
+    dy = NO.bf16(rng.standard_normal((nvox, c)))
+    db0 = rng.standard_normal(c).astype(np.float32)
+    dbd = _dev(db0, torch.float32)
+    _nnlib.call("mpf_nn_bias_grad", _p(_dev(dy, torch.bfloat16)), _p(dbd), nvox, c, _stream())
+    ref = db0.astype(np.float64) + dy.sum(axis=0)
+    got = _host(dbd)
+    assert np.abs(got - ref).max() <= 1e-5 * (np.abs(dy).sum(axis=0).max() + 1.0)
+
+
 def test_slot_features_match_oracle():
     torch = _torch()
     import cases
