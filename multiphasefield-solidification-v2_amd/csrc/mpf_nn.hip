@@ -28,6 +28,17 @@ typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(3))) s16x4 lds_s16x4;
+
+// LDS transpose read (ds_read_b64_tr_b16): in each group of 16 lanes, lane
+// 16g + j receives element j%4 of the 8-byte rows addressed by lanes
+// 16g + 4i + j/4, i = 0..3 (value i from the i-th of those lanes).
+__device__ __forceinline__ s16x4 lds_tr16(const bf16 *p)
+{
+    return __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4 *)p);
+}
 
 namespace {
 
@@ -258,8 +269,10 @@ __device__ __forceinline__ void advance(const Geom &g, const int *s, int &b, int
 // memory: each wave stages its 16 voxels x 32 channels of dY and of X at each
 // z-tap in LDS with one 16-byte load per lane and tensor (lane l: voxel l/4,
 // channels 8*(l%4)..+7), then lane l reads column l%32, voxels 8*(l/32)..+7
-// of the tiles as its A (dY, rows = co) and B (X, columns = ci) operands, and
-// receives dW rows 8*(r/4) + 4*(l/32) + r%4 (r < 16) of column l%32.
+// of the tiles as its A (dY, rows = co) and B (X, columns = ci) operands with
+// two LDS transpose reads each (ds_read_b64_tr_b16: 8 reads for NZ = 3 where
+// 2-byte reads take 32; 12% faster at 4 x 64^3 voxels, 32 -> 32 channels),
+// and receives dW rows 8*(r/4) + 4*(l/32) + r%4 (r < 16) of column l%32.
 // Block: 4 waves over a 32 x 32 (co, ci) tile and NZ taps that differ only in
 // z (the three z offsets of one (x, y) offset, or the one 1x1x1 tap), which
 // share the dY operand and the coordinate arithmetic.  The waves take
@@ -268,8 +281,8 @@ __device__ __forceinline__ void advance(const Geom &g, const int *s, int &b, int
 // accesses complete in order, so the voxel loop needs only compiler barriers
 // (no s_barrier) between the stores and the reads of a tile.
 // (A version on the fp32 matrix cores, v_mfma_f32_32x32x2f32 with K = 2
-// voxels read straight from memory, ran 3-7.5x slower on the surrogate's
-// layer shapes: 0.40 against 0.075 ms at 64^3 voxels, 32 -> 32 channels.)
+// voxels read straight from memory, ran 2-8.7x slower on the surrogate's
+// layer shapes: 0.40 against 0.066 ms at 64^3 voxels, 32 -> 32 channels.)
 template <int NZ>
 __global__ __launch_bounds__(256) void k_conv_bwd_w(const bf16 *__restrict__ X, const bf16 *__restrict__ dY,
                                                     float *__restrict__ dW, Geom g, int nvox, int vchunk)
@@ -291,6 +304,19 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(const bf16 *__restrict__ X, 
     const int vend = min(vbeg + vchunk, nvox);
     const int vr = lane >> 2, cq = (lane & 3) * 8;
     const bf16x8 zero8 = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+    // Lane l's operand: column l%32, voxels 8*(l/32)..+7 of a 16 x 32 tile,
+    // as two transpose reads of 4 voxels.  With g = l/16 and j = l%16, the
+    // lanes 16g + 4i + m (i, m < 4) address voxel 8*(g/2) + i (+4 for the
+    // second read), channels 16*(g%2) + 4m..+3, so lane 16g + j receives
+    // channel 16*(g%2) + j = l%32 of voxels 8*(g/2) + 0..3.
+    const int trow = 8 * (lane >> 5) + ((lane & 15) >> 2);
+    const int tcol = 16 * ((lane >> 4) & 1) + 4 * (lane & 3);
+    auto tile_column = [&](const bf16 (*t)[32]) {
+        const s16x4 lo = lds_tr16(&t[trow][tcol]);
+        const s16x4 hi = lds_tr16(&t[trow + 4][tcol]);
+        const s16x8 v = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+        return __builtin_bit_cast(bf16x8, v);
+    };
     for (int v0 = vbeg + 16 * wave; v0 < vend; v0 += 64) {
         const int v = v0 + vr;
         const bool ok = v < vend;
@@ -314,12 +340,9 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(const bf16 *__restrict__ X, 
         for (int q = 0; q < NZ; ++q) st8(&s_t[wave][1 + q][vr][cq], lx[q]);
         __builtin_amdgcn_wave_barrier();
         bf16x8 a, bq[NZ];
+        a = tile_column(s_t[wave][0]);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) a[j] = s_t[wave][0][kg * 8 + j][col];
-#pragma unroll
-        for (int q = 0; q < NZ; ++q)
-#pragma unroll
-            for (int j = 0; j < 8; ++j) bq[q][j] = s_t[wave][1 + q][kg * 8 + j][col];
+        for (int q = 0; q < NZ; ++q) bq[q] = tile_column(s_t[wave][1 + q]);
 #pragma unroll
         for (int q = 0; q < NZ; ++q) acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[q], acc[q], 0, 0, 0);
         __builtin_amdgcn_wave_barrier();
