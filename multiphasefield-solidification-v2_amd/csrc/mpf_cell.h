@@ -549,30 +549,6 @@ MPF_DEV void solid_liquid_terms(const Consts<Real> &K, const LiqCtx<Real, DIM> &
     b_sl = calc_b(K, lgrad_ok(K, G) ? fc.ca : Real(0.0));
 }
 
-// Read/write a small register array at a run-time index without dynamic
-// register indexing (which would move the array to scratch memory).
-template <int NA, bool SMALL>
-MPF_DEV int sel_int(const int *a, int t)
-{
-    if constexpr (!SMALL) return a[t];
-    int r = a[0];
-#pragma unroll
-    for (int u = 1; u < NA; ++u)
-        if (u == t) r = a[u];
-    return r;
-}
-template <typename Real, int NA, bool SMALL>
-MPF_DEV void put_real(Real *a, int t, Real v)
-{
-    if constexpr (!SMALL) {
-        a[t] = v;
-    } else {
-#pragma unroll
-        for (int u = 0; u < NA; ++u)
-            if (u == t) a[u] = v;
-    }
-}
-
 // Active set of one cell in gathered form.  Index 0 is the liquid (present iff
 // liq), indices 1..nsol are the active solids in ascending id order.
 template <typename Real, int DIM, int A>

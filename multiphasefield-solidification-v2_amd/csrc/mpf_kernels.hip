@@ -71,32 +71,6 @@ __device__ __forceinline__ void note_active(Stats *st, int ntot, int kmax, int s
     }
 }
 
-// Append the flagged lanes of a wave to a work list with one atomic.  All 64
-// lanes of the wave must be active.
-__device__ __forceinline__ void push_list(bool flag, int cell, int *list, int *count)
-{
-    unsigned long long ball = __ballot(flag);
-    if (ball == 0ull) return;
-    int lane = threadIdx.x & 63;
-    int leader = __ffsll((long long)ball) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(count, __popcll(ball));
-    base = __shfl(base, leader, 64);
-    if (flag) {
-        unsigned long long below = ball & ((1ull << lane) - 1ull);
-        list[base + __popcll(below)] = cell;
-    }
-}
-
-// Tile -> cells.  Consecutive blocks on one XCD (blocks are dealt round-robin
-// over the 8 XCDs) get consecutive tiles so that halos are reused in that
-// XCD's L2.
-__device__ __forceinline__ int xcd_remap(int b, int nb)
-{
-    if ((nb & 7) != 0) return b;
-    return (b & 7) * (nb >> 3) + (b >> 3);
-}
-
 // x is +0.0 (all bits clear; -0.0 is not)
 __device__ __forceinline__ bool is_plus_zero(float x) { return __float_as_uint(x) == 0u; }
 __device__ __forceinline__ bool is_plus_zero(double x) { return __double_as_longlong(x) == 0ll; }

@@ -245,22 +245,6 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16 *__restrict__ X, co
         }
 }
 
-// Coordinates (b, x, y, z) of a voxel advanced by a fixed voxel count whose
-// mixed-radix digits (z fastest) are s[0..3] = (z, y, x, b).
-__device__ __forceinline__ void advance(const Geom &g, const int *s, int &b, int &x, int &y, int &z)
-{
-    z += s[0];
-    int c = z >= g.Z;
-    z -= c ? g.Z : 0;
-    y += s[1] + c;
-    c = y >= g.Y;
-    y -= c ? g.Y : 0;
-    x += s[2] + c;
-    c = x >= g.X;
-    x -= c ? g.X : 0;
-    b += s[3] + c;
-}
-
 // dW[tap][co][ci] += sum over a chunk of voxels of dY[v][co] * X[v + tap][ci],
 // as sums of outer products on the bf16 matrix cores
 // (v_mfma_f32_32x32x16_bf16, K = 16 voxels, fp32 accumulation; the products
