@@ -432,7 +432,6 @@ template <typename Real, int DIM>
 MPF_DEV void solid_liquid_terms(const Consts<Real> &K, const LiqCtx<Real, DIM> &X, int s, Real &lap_sl,
                                 Real &b_sl)
 {
-    constexpr int NP = LiqCtx<Real, DIM>::NP;
     const Real *nrow = K.n111() + (size_t)s * 24;
     Real nrm[12];
 #pragma unroll
@@ -603,7 +602,6 @@ MPF_DEV void cell_update(const Consts<Real> &K, const LiqAt &L, int l, int m, in
     const bool s_liq = EXACT ? XLIQ : S.liq;
     const int s_nsol = EXACT ? A : S.nsol;
     constexpr int NA = A + 1;
-    constexpr int NP = Cell<Real, DIM, A>::NP;
     constexpr bool SMALL = (A <= 8);
     constexpr int UNR_NA = SMALL ? NA : 1;
     constexpr int UNR_A = SMALL ? A : 1;
